@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Write tests/golden/gfn2_golden.json from the reference's own test data.
+
+Reads (never executes) the xtb 6.7.1 unit tests and their molecule stock and
+stores their geometries and expected values as a data fixture.  Values are
+parsed out of the check statements so that nothing is retyped by hand.
+
+Sources:
+  test/unit/test_gfn2.f90      scc (H2O, :48-170), api (propyne, :172-235),
+                               pcem block 1 (water tetramer, :383-457),
+                               mindless-basic (:667-742)
+  test/api/c_api_example.c     propyne charges / dipole / bond order (:105-112)
+  test/unit/test_eeq.f90       EEQ water charges and CN (:43-136)
+  test/unit/test_repulsion.f90 GFN2 repulsion energy of caffeine (:39-150)
+  test/unit/molstock.f90       mindless01-10 and caffeine geometries (:66-364)
+
+Usage:  python make_golden.py [/root/reference]
+"""
+import json
+import os
+import re
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gfn2_golden.json")
+
+SYMBOLS = ("H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu "
+           "Zn Ga Ge As Se Br Kr Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba "
+           "La Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb Lu Hf Ta W Re Os Ir Pt Au Hg Tl Pb Bi "
+           "Po At Rn").split()
+FLT = r"[-+]?\d+\.\d*(?:[eEdD][-+]?\d+)?"
+
+
+def read(path):
+    return open(os.path.join(REF, path)).read()
+
+
+def fnum(t):
+    return float(t.replace("_wp", "").replace("d", "e").replace("D", "e"))
+
+
+def block(src, start, end):
+    i = src.index(start)
+    j = src.index(end, i)
+    return src[i:j]
+
+
+def xyz_from(blk):
+    m = re.search(r"xyz\(3,\s*nat\)\s*=\s*reshape\(\s*(?:&\s*)*\[(.*?)\]", blk, re.S)
+    vals = [fnum(v) for v in re.findall(FLT + r"_wp", m.group(1))]
+    return [vals[k:k + 3] for k in range(0, len(vals), 3)]
+
+
+def numbers_from(blk):
+    m = re.search(r"sym\(nat\)\s*=\s*\[character\(len=\d+\)\s*::\s*&?(.*?)\]", blk, re.S)
+    if m:
+        return [SYMBOLS.index(s) + 1 for s in re.findall(r'"(\w+)"', m.group(1))]
+    m = re.search(r"at\(nat\)\s*=\s*&?\s*\[(.*?)\]", blk, re.S)
+    return [int(v) for v in re.findall(r"\d+", m.group(1))]
+
+
+def checks(blk):
+    """All  check_(error, <expr>, <value>_wp ...) pairs of a test subroutine."""
+    out = []
+    for m in re.finditer(r"check_?\(error,\s*((?:[^,()\n]|\([^)\n]*\))+),\s*(" + FLT + r")_wp", blk):
+        out.append((m.group(1).strip(), fnum(m.group(2))))
+    return out
+
+
+def main():
+    gfn2 = read("test/unit/test_gfn2.f90")
+    stock = read("test/unit/molstock.f90")
+    out = {}
+
+    # --- H2O, SAD guess, raw scf call (test_gfn2.f90:48-170)
+    b = block(gfn2, "subroutine test_gfn2_scc", "end subroutine test_gfn2_scc")
+    c = dict(checks(b))
+    out["h2o_scc"] = {
+        "source": "test/unit/test_gfn2.f90:48-170",
+        "numbers": numbers_from(b), "positions": xyz_from(b), "charge": 0.0, "uhf": 0,
+        "guess": "sad", "maxiter": 50, "accuracy": 1.0, "etemp": 300.0,
+        "expect": {
+            "e_total": [c["res%e_total"], 1e-7], "gnorm": [c["res%gnorm"], 1e-7],
+            "hl_gap_ev": [c["res%hl_gap"], 1e-4], "e_elec": [c["res%e_elec"], 1e-7],
+            "e_es": [c["res%e_es"], 1e-6], "e_aes": [c["res%e_aes"], 1e-6],
+            "e_axc": [c["res%e_axc"], 1e-7], "e_disp": [c["res%e_disp"], 1e-7],
+            "e_rep": [c["res%e_rep"], 1e-7],
+            "q[1]": [c["wfn%q(2)"], 1e-5], "qsh[0]": [c["wfn%qsh(1)"], 1e-5],
+            "dipm[1][1]": [c["wfn%dipm(2,2)"], 1e-5], "dipm[2][0]": [c["wfn%dipm(1,3)"], 1e-5],
+            "qp[0][5]": [c["wfn%qp(6,1)"], 1e-5], "qp[2][0]": [c["wfn%qp(1,3)"], 1e-5],
+            "emo_homo": [c["wfn%emo(wfn%ihomo)"], 1e-5],
+        },
+    }
+
+    # --- propyne, EEQ guess through the calculator (test_gfn2.f90:172-235)
+    b = block(gfn2, "subroutine test_gfn2_api", "end subroutine test_gfn2_api")
+    c = dict(checks(b))
+    capi = read("test/api/c_api_example.c")
+    ce = lambda pat: fnum(re.search(pat + r",\s*(" + FLT + ")", capi).group(1))
+    out["propyne"] = {
+        "source": "test/unit/test_gfn2.f90:172-235; test/api/c_api_example.c:105-112",
+        "numbers": numbers_from(b), "positions": xyz_from(b), "charge": 0.0, "uhf": 0,
+        "guess": "eeq", "maxiter": 30, "accuracy": 1.0, "etemp": 300.0,
+        "expect": {
+            "hl_gap_ev": [c["hl_gap"], 1e-9], "e_total": [c["energy"], 1e-9],
+            "gnorm": [c["norm2(gradient)"], 1e-9],
+            "gradient[2][1]": [c["gradient(2,3)"], 1e-9], "gradient[0][2]": [c["gradient(3,1)"], 1e-9],
+            "gradient[3][0]": [c["gradient(1,4)"], 1e-9], "gradient[6][2]": [c["gradient(3,7)"], 1e-9],
+            "q[5]": [ce(r"check\(q\[5\]"), 1e-8], "dipole[2]": [ce(r"check\(dipole\[2\]"), 1e-6],
+            "wbo_flat[9]": [ce(r"check\(wbo\[9\]"), 1e-8],
+        },
+    }
+
+    # --- water tetramer in vacuum, first block of the pcem test (:383-457)
+    b = block(gfn2, "subroutine test_gfn2_pcem_api", "! reset")
+    c = dict(checks(b))
+    m = re.search(r"at\(nat\)\s*=\s*\[([\d,\s]+)\]", b)
+    out["water_tetramer"] = {
+        "source": "test/unit/test_gfn2.f90:383-457",
+        "numbers": [int(v) for v in m.group(1).split(",")], "positions": xyz_from(b),
+        "charge": 0.0, "uhf": 0, "guess": "eeq", "maxiter": 30, "accuracy": 1.0, "etemp": 300.0,
+        "expect": {
+            "hl_gap_ev": [c["hl_gap"], 1e-10], "e_total": [c["energy"], 1e-10],
+            "gnorm": [c["norm2(gradient)"], 1e-10],
+            "gradient[4][0]": [c["gradient(1,5)"], 1e-10], "gradient[1][1]": [c["gradient(2,2)"], 1e-10],
+            "gradient[3][0]": [c["gradient(1,4)"], 1e-10], "gradient[5][2]": [c["gradient(3,6)"], 1e-10],
+        },
+    }
+
+    # --- mindless01-10 (test_gfn2.f90:667-742, molstock.f90:66-330)
+    b = block(gfn2, "subroutine test_gfn2_mindless_basic", "end subroutine test_gfn2_mindless_basic")
+    arr = lambda nm: [fnum(v) for v in re.findall(FLT + "_wp", re.search(nm + r"\(10\)\s*=\s*&?\s*&?\[(.*?)\]", b, re.S).group(1))]
+    e_ref, g_ref, h_ref = arr("ref_energies"), arr("ref_gnorms"), arr("ref_hlgaps")
+    for k in range(10):
+        name = f"mindless{k + 1:02d}"
+        mb = block(stock, f"subroutine {name}(mol)", f"end subroutine {name}")
+        mu = re.search(r"uhf\s*=\s*(\d+)", mb)
+        out[name] = {
+            "source": f"test/unit/test_gfn2.f90:698-712; test/unit/molstock.f90 ({name})",
+            "numbers": numbers_from(mb), "positions": xyz_from(mb), "charge": 0.0,
+            "uhf": int(mu.group(1)) if mu else 0, "guess": "eeq", "maxiter": 250,
+            "accuracy": 1.0, "etemp": 300.0,
+            "expect": {"e_total": [e_ref[k], 1e-8], "gnorm": [g_ref[k], 1e-8], "hl_gap_ev": [h_ref[k], 1e-8]},
+        }
+
+    # --- EEQ on water (test_eeq.f90:43-136): erf CN and charges
+    eeq = read("test/unit/test_eeq.f90")
+    b = block(eeq, "subroutine test_eeq_water", "end subroutine test_eeq_water")
+    c = dict(checks(b.replace("call check(", "call check_(")))
+    out["eeq_water"] = {
+        "source": "test/unit/test_eeq.f90:43-136",
+        "numbers": numbers_from(b), "positions": xyz_from(b),
+        "expect": {"cn[0]": [c["cn(1)"], 1e-10], "q[0]": [c["q(1)"], 1e-10]},
+    }
+
+    # --- GFN2 repulsion energy of caffeine (test_repulsion.f90:39-150)
+    rep = read("test/unit/test_repulsion.f90")
+    b = block(rep, "subroutine test_repulsion_cluster", "end subroutine test_repulsion_cluster")
+    m = re.search(r"check\(error,\s*energy,\s*(" + FLT + ")_wp", b)
+    cb = block(stock, "subroutine caffeine(mol)", "end subroutine caffeine")
+    out["caffeine_repulsion"] = {
+        "source": "test/unit/test_repulsion.f90:39-150",
+        "numbers": numbers_from(cb), "positions": xyz_from(cb),
+        "expect": {"e_rep": [fnum(m.group(1)), 1e-9]},
+    }
+
+    for k, v in out.items():
+        if "numbers" in v:
+            assert len(v["numbers"]) == len(v["positions"]), k
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"wrote {OUT}: {len(out)} cases")
+
+
+if __name__ == "__main__":
+    main()
