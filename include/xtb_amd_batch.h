@@ -1,0 +1,61 @@
+/*
+ * Batched / served GFN2-xTB single points on one MI355X (gfx950).
+ *
+ * This interface has no counterpart in xtb's C API (include/xtb.h): it is the
+ * extension through which many single points run as one batch.  Every molecule
+ * of a batch is computed exactly as xtb_singlepoint would compute it
+ * (src/api/interface.f90:40-156 -> scf, src/scf_module.F90:74-920) with a
+ * fresh results object, i.e. from the EEQ guess.
+ *
+ * Units: bohr, Hartree, Hartree/bohr; orbital energies in eV.
+ * Errors: functions return 0 (or a non-negative count) on success and a
+ * negative value on failure; the message is available from
+ * xtbamd_batch_error().
+ */
+#pragma once
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xtbamd_batch_s* xtbamd_batch;
+
+typedef struct {
+  int natoms, nshells, nao, iterations, converged, failed, ihomo;
+  double energy, gnorm, hl_gap_ev;
+  double e_elec, e_es, e_aes, e_axc, e_disp, e_rep, e_atm;
+  double dipole[3];
+  /* caller-owned output arrays, may be NULL */
+  double* gradient;   /* [natoms][3] */
+  double* charges;    /* [natoms] */
+  double* qsh;        /* [nshells] */
+  double* dipm;       /* [natoms][3] */
+  double* qp;         /* [natoms][6] xx,xy,yy,xz,yz,zz */
+  double* wbo;        /* [natoms][natoms] */
+  double* emo;        /* [nao] eV */
+  double* focc;       /* [nao] */
+  double* coeff;      /* [nao][nao] row = AO, column = MO */
+  double* density;    /* [nao][nao] */
+} xtbamd_result;
+
+/* numbers/positions/charges/uhf are concatenated over the nmol molecules;
+ * natoms[m] gives the size of molecule m.  guess: 0 = EEQ, 1 = SAD. */
+xtbamd_batch xtbamd_batch_new(int nmol, const int* natoms, const int* numbers, const double* positions,
+                              const double* charges, const int* uhf, double accuracy, double etemp,
+                              int maxiter, int guess);
+/* last error message of the calling thread (batch creation) or of the batch */
+const char* xtbamd_last_error(void);
+int xtbamd_batch_setup(xtbamd_batch b);
+/* one SCC iteration of every active molecule; returns the number still active */
+int xtbamd_batch_iterate(xtbamd_batch b);
+int xtbamd_batch_gradient(xtbamd_batch b);
+/* setup + SCC to convergence + gradient */
+int xtbamd_batch_run(xtbamd_batch b);
+int xtbamd_batch_size(xtbamd_batch b);
+int xtbamd_batch_get(xtbamd_batch b, int m, xtbamd_result* out);
+void xtbamd_batch_free(xtbamd_batch b);
+/* device / runtime information: number of visible GPUs */
+int xtbamd_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,105 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 oracle and
+the reference's golden vectors.
+
+Tolerances: total energy 1e-6 Eh, charges 1e-5 e, gradient 1e-5 Eh/bohr
+(the path's parity bar); golden values use the reference tests' own
+tolerances.  Integer outputs (iteration counts, AO/shell dimensions) must match
+exactly.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.test_oracle_golden import SP_CASES, _lookup
+
+pytestmark = pytest.mark.gpu
+
+TOL_E, TOL_Q, TOL_G = 1e-6, 1e-5, 1e-5
+
+
+def _engine():
+    import xtb_amd
+    if xtb_amd.device_count() < 1:
+        pytest.fail("no GPU visible to the HIP runtime")
+    return xtb_amd
+
+
+def _cmp(r, o, tag):
+    assert r["converged"], tag
+    assert r["natoms"] == len(o["q"]) and r["nao"] == o["nao"] and r["nshells"] == o["nsh"], tag
+    dE = abs(r["energy"] - o["e_total"])
+    dq = np.max(np.abs(r["charges"] - o["q"]))
+    dg = np.max(np.abs(r["gradient"] - o["gradient"]))
+    print(f"{tag}: dE={dE:.2e} dq={dq:.2e} dg={dg:.2e} iters={r['iterations']}/{o['iterations']}", flush=True)
+    assert dE <= TOL_E, f"{tag} energy {r['energy']} vs {o['e_total']}"
+    assert dq <= TOL_Q, f"{tag} charges"
+    assert dg <= TOL_G, f"{tag} gradient"
+    for k in ("e_es", "e_aes", "e_axc", "e_disp", "e_rep"):
+        assert abs(r[k] - o[k]) <= TOL_E, f"{tag} {k}: {r[k]} vs {o[k]}"
+    assert abs(r["hl_gap_ev"] - o["hl_gap_ev"]) <= 1e-4, f"{tag} gap"
+    assert np.max(np.abs(r["dipole"] - o["dipole"])) <= TOL_Q, f"{tag} dipole"
+
+
+@pytest.mark.parametrize("case", SP_CASES)
+def test_golden_single(golden, case):
+    xa = _engine()
+    g = golden[case]
+    r = xa.singlepoint(g["numbers"], g["positions"], g["charge"], g["uhf"], g["accuracy"], g["etemp"],
+                       g["maxiter"], g["guess"])
+    o = po.singlepoint(g["numbers"], g["positions"], charge=g["charge"], uhf=g["uhf"], guess=g["guess"],
+                       maxiter=g["maxiter"], accuracy=g["accuracy"], etemp=g["etemp"])
+    _cmp(r, o, case)
+    res = dict(r)
+    res["e_total"] = r["energy"]
+    res["q"] = r["charges"]
+    assert r["ihomo"] == o["ihomo"]
+    for key, (ref, tol) in g["expect"].items():
+        val = _lookup(res, key)
+        assert abs(val - ref) <= max(tol, 1e-7), f"{case} {key}: {val} vs {ref}"
+
+
+def test_golden_batch_same_alone_and_in_mix(golden):
+    """All golden molecules in one batch: bitwise the alone results."""
+    xa = _engine()
+    cases = [golden[c] for c in SP_CASES if golden[c]["guess"] == "eeq"]
+    mols = [(g["numbers"], g["positions"], g["charge"], g["uhf"]) for g in cases]
+    b = xa.Batch(mols, maxiter=250).run()
+    mix = b.results()
+    b.close()
+    for g, rm in zip(cases, mix):
+        ra = xa.singlepoint(g["numbers"], g["positions"], g["charge"], g["uhf"])
+        assert ra["energy"] == rm["energy"]
+        assert np.array_equal(ra["gradient"], rm["gradient"])
+        assert np.array_equal(ra["charges"], rm["charges"])
+        assert ra["iterations"] == rm["iterations"]
+
+
+def test_water32_cluster():
+    """32 H2O on a jittered 4x4x2 grid (96 atoms, 192 AOs): block Jacobi path."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    z, x = w.water_cluster(32, 0)
+    r = xa.singlepoint(z, x)
+    o = po.singlepoint(z, x)
+    _cmp(r, o, "water32")
+
+
+def test_alkane_c64():
+    """all-trans C64H130 (194 atoms, 386 AOs)."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    z, x = w.alkane(64, 1)
+    r = xa.singlepoint(z, x)
+    o = po.singlepoint(z, x, threads=4)
+    _cmp(r, o, "alkane64")
+
+
+def test_mixed_batch_vs_oracle():
+    """16 molecules of the batched workload's distribution, as one batch."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    mols = w.mixed_batch(16, seed=7, lo=10, hi=120)
+    b = xa.Batch([(z, x, 0.0, 0) for z, x in mols]).run()
+    for m, (z, x) in enumerate(mols):
+        _cmp(b.result(m), po.singlepoint(z, x), f"mix{m}(n={len(z)})")
+    b.close()

@@ -1,0 +1,179 @@
+// Batched GFN2-xTB SCC engine for MI355X: device data layout and launchers.
+//
+// A Batch holds M independent molecules.  Every per-molecule quantity is
+// stored packed: atom-, shell-, AO- and D4-reference-indexed vectors at the
+// offsets of MolDesc, dense matrices at per-molecule offsets.  All kernels of
+// one SCC step take the whole batch; a molecule's arithmetic never depends on
+// the other molecules of the batch (fixed per-molecule reduction trees, no
+// floating-point atomics), so a result is bitwise the same alone and in a mix.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gfn2_common.h"
+
+namespace xtbamd {
+
+struct MolDesc {
+  int n, nsh, nao, nd4, ne;     // atoms, shells, AOs, D4 references, EEQ dim (n+1)
+  int a0, s0, o0, d0;           // offsets of atom/shell/AO/D4 vectors
+  long long m0;                 // offset of the nao x nao matrices
+  long long j0;                 // nsh x nsh
+  long long x0;                 // nd4 x nd4
+  long long p0;                 // n x n
+  long long e0;                 // ne x ne
+  long long v0;                 // Broyden vectors (nbr)
+  long long h0;                 // Broyden history (maxiter x nbr)
+  long long g0;                 // Broyden small matrices (maxiter x maxiter)
+  int nbr, nel, nopen, maxiter;
+  double charge;
+};
+
+// Per-molecule SCC state machine (src/scc_core.f90:434-640).
+struct MolState {
+  int iter;        // iterations done
+  int converged;   // criteria met at least once
+  int lastdiag;
+  int active;      // still iterating
+  int failed;      // error code (0 = fine)
+  int ihomoa, ihomob, ihomo;
+  double eel, eold, egap, ees, eaes, epol, ed, ga, gb, rmsq, efa, efb;
+  double eh0;      // Tr(P H0) in eV
+  double damp;     // simple mixing factor (single atoms)
+  double ep, embd, etotal, gnorm;
+  double dipole[3];
+};
+
+struct Options {
+  double accuracy = 1.0;
+  double etemp = 300.0;
+  int maxiter = 250;
+  int guess = 0;   // 0 EEQ, 1 SAD
+};
+
+struct MoleculeInput {
+  std::vector<int> numbers;
+  std::vector<double> xyz;   // bohr, [n][3]
+  double charge = 0.0;
+  int uhf = 0;
+};
+
+// Per-molecule results copied back to the host.
+struct MolResult {
+  int n = 0, nsh = 0, nao = 0, iterations = 0, converged = 0, failed = 0, ihomo = 0;
+  double energy = 0, gnorm = 0, hl_gap_ev = 0, e_elec = 0, e_es = 0, e_aes = 0, e_axc = 0;
+  double e_disp = 0, e_rep = 0, e_atm = 0;
+  double dipole[3] = {0, 0, 0};
+  std::vector<double> gradient, q, qsh, dipm, qp, wbo, emo, focc, coeff, density;
+};
+
+// Jacobi eigenproblem set: each problem is a dim x dim symmetric matrix at
+// a_off in A, eigenvectors accumulate at w_off in W (column k = vector k).
+struct JacobiProb {
+  int dim, nb, nbp, rounds;   // 32-blocks, padded even count, rounds per sweep
+  long long a_off, w_off, q_off;
+  int ev_off;                 // eigenvalue offset
+  int pad;
+};
+
+// A set of Jacobi problems (one per molecule) with its device work lists.
+struct JacSet {
+  std::vector<JacobiProb> probs;
+  JacobiProb* d_probs = nullptr;
+  int nsmall = 0, npairs = 0, nta = 0, ntw = 0, max_rounds = 0;
+  int* d_small = nullptr;
+  int2* d_pairs = nullptr;
+  int4 *d_ta = nullptr, *d_tw = nullptr;
+  double* qbuf = nullptr;
+  int* rot = nullptr;
+  int* state = nullptr;
+  double* scale = nullptr;
+  int* perm = nullptr;
+  long long qsize = 0;
+};
+
+class Batch {
+ public:
+  Batch(const std::vector<MoleculeInput>& mols, const Options& opt, hipStream_t stream = nullptr);
+  ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+
+  // full single point of every molecule (setup, SCC loop, gradient)
+  void run();
+  void setup();
+  bool scc_iteration();   // returns true while any molecule is active
+  void gradient();
+  MolResult result(int m, bool matrices = false) const;
+  int size() const { return (int)mols_.size(); }
+  const std::string& error() const { return error_; }
+
+  // --- layout (public for the kernel launchers) ---
+  Options opt;
+  hipStream_t stream;
+  int nmol = 0, max_n = 0, max_nsh = 0, max_nao = 0, max_nd4 = 0, max_nbr = 0;
+  int tot_n = 0, tot_nsh = 0, tot_nao = 0, tot_nd4 = 0;
+  long long tot_mm = 0, tot_jj = 0, tot_xx = 0, tot_pp = 0, tot_ee = 0, tot_v = 0, tot_h = 0, tot_g = 0;
+  double intcut = 25.0, neglect = 1e-8, scfconv = 1e-6, qconv = 1e-4;
+  std::vector<MolDesc> desc;
+  MolDesc* d_desc = nullptr;
+  MolState* d_state = nullptr;
+  std::vector<MoleculeInput> mols_;
+  std::string error_;
+
+  // atom vectors
+  int* at = nullptr;
+  double *xyz = nullptr, *cn = nullptr, *cn_d4 = nullptr, *cn_eeq = nullptr, *cn_cov = nullptr;
+  double *q = nullptr, *dipm = nullptr, *qp = nullptr, *radcn = nullptr;
+  double *vs = nullptr, *vd = nullptr, *vq = nullptr, *ashift = nullptr;
+  double *grad = nullptr, *dhdcn = nullptr, *dedcn = nullptr, *tmp2 = nullptr, *eat = nullptr;
+  double *ae01 = nullptr;   // per-atom AES energy partials (3 per atom)
+  // shell vectors
+  int *sh_at = nullptr, *sh_l = nullptr, *sh_ao0 = nullptr, *sh_m = nullptr;
+  int *at_sh0 = nullptr, *at_ao0 = nullptr;
+  double *qsh = nullptr, *zsh = nullptr, *shift = nullptr, *shgam = nullptr, *selfe = nullptr;
+  double *dsedcn = nullptr, *jq = nullptr;
+  // AO vectors
+  int *ao_at = nullptr, *ao_sh = nullptr;
+  double *emo = nullptr, *focc = nullptr, *focca = nullptr, *foccb = nullptr, *ev = nullptr;
+  double *row_q = nullptr, *row_e = nullptr, *row_d = nullptr, *row_qp = nullptr, *row_pd = nullptr;
+  // D4
+  int *d4_at = nullptr, *d4_ref = nullptr;
+  double *gw = nullptr, *zv = nullptr, *dzv = nullptr, *d4dum = nullptr, *c6ref = nullptr;
+  double *zetavec = nullptr, *zerovec = nullptr, *zetadcn = nullptr, *zerodcn = nullptr;
+  // matrices (nao^2)
+  double *S = nullptr, *H0 = nullptr, *H0n = nullptr, *D = nullptr, *Q = nullptr;
+  double *H = nullptr, *C = nullptr, *T = nullptr, *A = nullptr, *W = nullptr, *P = nullptr;
+  // other matrices
+  double *J = nullptr, *dispmat = nullptr, *gab3 = nullptr, *gab5 = nullptr;
+  double *c6 = nullptr, *dc6 = nullptr, *c60 = nullptr, *dc60 = nullptr;
+  double *E = nullptr, *WE = nullptr, *evE = nullptr, *xE = nullptr;
+  // Broyden
+  double *q_in = nullptr, *dq = nullptr, *qlast = nullptr, *dqlast = nullptr;
+  double *bdf = nullptr, *bu = nullptr, *ba = nullptr, *bomega = nullptr;
+  double *yE = nullptr, *erep = nullptr, *wbo = nullptr;
+  // Jacobi bookkeeping
+  JacSet jac_n, jac_e;
+  int* jac_nactive = nullptr;
+  int* perm = nullptr;
+  int* eeq_failed = nullptr;
+  int* h_pinned = nullptr;
+  long long jacobi_rounds_ = 0;
+  const Element* elements_dev = nullptr;
+  int ntiles = 0;
+  int4* d_tiles = nullptr;
+  void jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active);
+  void gemm(int kind, const double* a, const double* b, double* c, const double* scale, const int* perm,
+            bool check_active);
+
+  std::vector<void*> allocs_;
+  bool setup_done_ = false;
+  int iterations_ = 0;
+  template <class T_>
+  T_* alloc(size_t n);
+};
+
+}  // namespace xtbamd

@@ -1,0 +1,782 @@
+// Host side of the batched GFN2-xTB engine: layout, work lists, the SCC
+// driver and the C ABI of the batch interface (include/xtb_amd_batch.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "gfn2_common.h"
+// kernels (single translation unit)
+#include "k_grad.hip"
+#include "k_linalg.hip"
+#include "k_scc.hip"
+#include "k_setup.hip"
+
+#define HIPCHECK(x)                                                                              \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess)                                                                        \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+namespace xtbamd {
+
+namespace {
+
+__global__ void k_fill(double* p, long long n, double v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_int(int* p, long long n, int v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_copy(const double* a, double* b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// EEQ charges from the eigen-decomposition of the constrained system, then the
+// shell charges (iniqshell: src/scc_core.f90:1475-1506); SAD fallback
+// (src/xtb/calculator.f90:479-484)
+__global__ void __launch_bounds__(256) k_eeq_solve(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
+                                                   const int* __restrict__ at, const int* __restrict__ at_sh0,
+                                                   const double* __restrict__ WE, const double* __restrict__ Ed,
+                                                   const double* __restrict__ xE, double* yE, double* q, double* qsh,
+                                                   const double* __restrict__ zsh, int use_eeq, int* failed) {
+  __shared__ double sh[4];
+  const MolDesc md = desc[blockIdx.x];
+  const int ne = md.ne, n = md.n;
+  const long long eoff = md.a0 + blockIdx.x;
+  const double* W = WE + md.e0;
+  double* y = yE + eoff;
+  bool ok = use_eeq != 0;
+  if (ok) {
+    for (int k = threadIdx.x; k < ne; k += blockDim.x) {
+      double s = 0.0;
+      for (int i = 0; i < ne; ++i) s += W[(long long)i * ne + k] * xE[eoff + i];
+      const double lam = Ed[md.e0 + (long long)k * ne + k];  // eigenvalue on the diagonal
+      y[k] = s / lam;
+    }
+    __syncthreads();
+    double tot = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      double s = 0.0;
+      for (int k = 0; k < ne; ++k) s += W[(long long)i * ne + k] * y[k];
+      q[md.a0 + i] = s;
+      tot += s;
+    }
+    tot = block_sum<256>(tot, sh);
+    if (!(fabs(tot - md.charge) <= 1.e-6)) ok = false;
+  }
+  if (!ok) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) q[md.a0 + i] = md.charge / n;
+    if (threadIdx.x == 0 && use_eeq) failed[blockIdx.x] = 1;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const Element& e = el[at[md.a0 + i]];
+    for (int s = 0; s < e.nsh; ++s) {
+      const int k = md.s0 + at_sh0[md.a0 + i] + s;
+      qsh[k] = (zsh[k] / e.zval) * q[md.a0 + i];
+    }
+  }
+}
+
+// canonical orthogonaliser of S: X[:, k] = U_S[:, k] / sqrt(s_k)
+__global__ void k_ortho(const MolDesc* __restrict__ desc, const double* __restrict__ Asd, const double* __restrict__ W,
+                        double* C) {
+  const MolDesc md = desc[blockIdx.y];
+  const long long N = md.nao;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * N) return;
+  const int k = (int)(t % N);
+  const double s = Asd[md.m0 + (long long)k * N + k];
+  C[md.m0 + t] = W[md.m0 + t] / sqrt(s);
+}
+
+// total energy, gradient norm and dipole (src/scf_module.F90:882-913,
+// calc_dipole src/dipole.f90:94-128)
+__global__ void __launch_bounds__(256) k_final(const MolDesc* __restrict__ desc, MolState* state,
+                                               const Element* __restrict__ el, const int* __restrict__ at,
+                                               const double* __restrict__ xyz, const double* __restrict__ grad,
+                                               const double* __restrict__ erep, const double* __restrict__ eatm,
+                                               const double* __restrict__ row_d) {
+  __shared__ double sh[4];
+  const MolDesc md = desc[blockIdx.x];
+  MolState& st = state[blockIdx.x];
+  double ep = 0, eb = 0, g2 = 0, dx = 0, dy = 0, dz = 0;
+  for (int i = threadIdx.x; i < md.n; i += blockDim.x) {
+    ep += erep[md.a0 + i];
+    eb += eatm[md.a0 + i];
+    for (int k = 0; k < 3; ++k) g2 += grad[3 * (md.a0 + i) + k] * grad[3 * (md.a0 + i) + k];
+    const double z = el[at[md.a0 + i]].zval;
+    dx += xyz[3 * (md.a0 + i)] * z;
+    dy += xyz[3 * (md.a0 + i) + 1] * z;
+    dz += xyz[3 * (md.a0 + i) + 2] * z;
+  }
+  for (int mu = threadIdx.x; mu < md.nao; mu += blockDim.x) {
+    dx -= row_d[3 * (md.o0 + mu)];
+    dy -= row_d[3 * (md.o0 + mu) + 1];
+    dz -= row_d[3 * (md.o0 + mu) + 2];
+  }
+  ep = block_sum<256>(ep, sh);
+  eb = block_sum<256>(eb, sh);
+  g2 = block_sum<256>(g2, sh);
+  dx = block_sum<256>(dx, sh);
+  dy = block_sum<256>(dy, sh);
+  dz = block_sum<256>(dz, sh);
+  if (threadIdx.x == 0) {
+    st.ep = ep;
+    st.embd = eb;
+    st.etotal = st.eel + ep + eb;
+    st.gnorm = sqrt(g2);
+    st.dipole[0] = dx;
+    st.dipole[1] = dy;
+    st.dipole[2] = dz;
+  }
+}
+
+inline unsigned grid1(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Aufbau occupation counts (occu: src/scc_core.f90:1112-1170)
+void host_occu(int ndim, int nel, int nopen, int& ihomoa, int& ihomob) {
+  std::vector<int> f(ndim, 0);
+  if (nel % 2 == 0) {
+    const int ih = nel / 2;
+    for (int i = 0; i < ih && i < ndim; ++i) f[i] = 2;
+    if (nopen > 1)
+      for (int i = 1; i <= nopen / 2; ++i) {
+        if (ih - i >= 0 && ih - i < ndim) f[ih - i] -= 1;
+        if (ih + i - 1 < ndim) f[ih + i - 1] += 1;
+      }
+  } else {
+    const int na = nel / 2 + (nopen - 1) / 2 + 1, nb = nel / 2 - (nopen - 1) / 2;
+    for (int i = 0; i < na && i < ndim; ++i) f[i] += 1;
+    for (int i = 0; i < nb && i < ndim; ++i) f[i] += 1;
+  }
+  ihomoa = ihomob = 0;
+  for (int i = 0; i < ndim; ++i) {
+    if (f[i] >= 1) ihomoa = i + 1;
+    if (f[i] == 2) ihomob = i + 1;
+  }
+}
+
+}  // namespace
+
+template <class T_>
+T_* Batch::alloc(size_t n) {
+  void* p = nullptr;
+  HIPCHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T_)));
+  allocs_.push_back(p);
+  return (T_*)p;
+}
+
+Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream_t s)
+    : opt(o), stream(s), mols_(mols) {
+  const Element* el = host_elements();
+  nmol = (int)mols.size();
+  if (nmol == 0) throw std::runtime_error("empty batch");
+  intcut = std::max(20.0, 25.0 - 10.0 * std::log10(opt.accuracy));
+  neglect = 10.0e-9 * opt.accuracy;
+  scfconv = 1.e-6 * opt.accuracy;
+  qconv = 1.e-4 * opt.accuracy;
+  desc.resize(nmol);
+  std::vector<int> h_at, h_sh_at, h_sh_l, h_sh_ao0, h_sh_m, h_at_sh0, h_at_ao0, h_ao_at, h_ao_sh, h_d4_at, h_d4_ref;
+  std::vector<double> h_xyz;
+  std::vector<MolState> h_state(nmol);
+  const int mx = opt.maxiter;
+  for (int m = 0; m < nmol; ++m) {
+    const MoleculeInput& mi = mols[m];
+    MolDesc& d = desc[m];
+    std::memset(&d, 0, sizeof(d));
+    d.n = (int)mi.numbers.size();
+    if (d.n < 1) throw std::runtime_error("molecule without atoms");
+    if ((int)mi.xyz.size() != 3 * d.n) throw std::runtime_error("coordinate array has the wrong size");
+    d.a0 = tot_n;
+    d.s0 = tot_nsh;
+    d.o0 = tot_nao;
+    d.d0 = tot_nd4;
+    double zsum = 0.0;
+    for (int i = 0; i < d.n; ++i) {
+      const int z = mi.numbers[i];
+      if (z < 1 || z > XTB_MAX_Z) throw std::runtime_error("element not parametrised in GFN2-xTB");
+      const Element& e = el[z];
+      h_at.push_back(z);
+      h_at_sh0.push_back(d.nsh);
+      h_at_ao0.push_back(d.nao);
+      for (int s = 0; s < e.nsh; ++s) {
+        h_sh_at.push_back(i);
+        h_sh_l.push_back(e.l[s]);
+        h_sh_ao0.push_back(d.nao);
+        h_sh_m.push_back(s);
+        for (int k = 0; k < 2 * e.l[s] + 1; ++k) {
+          h_ao_at.push_back(i);
+          h_ao_sh.push_back(d.nsh);
+        }
+        d.nsh += 1;
+        d.nao += 2 * e.l[s] + 1;
+      }
+      for (int r = 0; r < e.nref; ++r) {
+        h_d4_at.push_back(i);
+        h_d4_ref.push_back(r);
+      }
+      d.nd4 += e.nref;
+      zsum += e.zval;
+      for (int k = 0; k < 3; ++k) h_xyz.push_back(mi.xyz[3 * i + k]);
+    }
+    d.ne = d.n + 1;
+    d.charge = mi.charge;
+    d.nel = (int)std::lround(zsum - mi.charge);
+    d.nopen = mi.uhf;
+    if (d.nopen == 0 && d.nel % 2 != 0) d.nopen = 1;
+    if (d.nel > 2 * d.nao) throw std::runtime_error("Not enough basis functions for filling orbitals");
+    d.nbr = d.nsh + 9 * d.n;
+    d.maxiter = mx;
+    d.m0 = tot_mm;
+    d.j0 = tot_jj;
+    d.x0 = tot_xx;
+    d.p0 = tot_pp;
+    d.e0 = tot_ee;
+    d.v0 = tot_v;
+    d.h0 = tot_h;
+    d.g0 = tot_g;
+    tot_n += d.n;
+    tot_nsh += d.nsh;
+    tot_nao += d.nao;
+    tot_nd4 += d.nd4;
+    tot_mm += (long long)d.nao * d.nao;
+    tot_jj += (long long)d.nsh * d.nsh;
+    tot_xx += (long long)d.nd4 * d.nd4;
+    tot_pp += (long long)d.n * d.n;
+    tot_ee += (long long)d.ne * d.ne;
+    tot_v += d.nbr;
+    tot_h += (long long)mx * d.nbr;
+    tot_g += 3LL * mx * mx + 4LL * mx;
+    max_n = std::max(max_n, d.n);
+    max_nsh = std::max(max_nsh, d.nsh);
+    max_nao = std::max(max_nao, d.nao);
+    max_nd4 = std::max(max_nd4, d.nd4);
+    max_nbr = std::max(max_nbr, d.nbr);
+    MolState& st = h_state[m];
+    std::memset(&st, 0, sizeof(st));
+    st.active = 1;
+    st.damp = 0.20;
+    host_occu(d.nao, d.nel, d.nopen, st.ihomoa, st.ihomob);
+    st.ihomo = st.ihomoa;
+  }
+  // device allocations
+  d_desc = alloc<MolDesc>(nmol);
+  d_state = alloc<MolState>(nmol);
+  at = alloc<int>(tot_n);
+  xyz = alloc<double>(3 * tot_n);
+  cn = alloc<double>(tot_n); cn_d4 = alloc<double>(tot_n); cn_eeq = alloc<double>(tot_n); cn_cov = alloc<double>(tot_n);
+  q = alloc<double>(tot_n); dipm = alloc<double>(3 * tot_n); qp = alloc<double>(6 * tot_n); radcn = alloc<double>(tot_n);
+  vs = alloc<double>(tot_n); vd = alloc<double>(3 * tot_n); vq = alloc<double>(6 * tot_n); ashift = alloc<double>(tot_n);
+  grad = alloc<double>(3 * tot_n); dhdcn = alloc<double>(tot_n); dedcn = alloc<double>(tot_n);
+  tmp2 = alloc<double>(tot_n); eat = alloc<double>(tot_n); ae01 = alloc<double>(4 * tot_n);
+  sh_at = alloc<int>(tot_nsh); sh_l = alloc<int>(tot_nsh); sh_ao0 = alloc<int>(tot_nsh); sh_m = alloc<int>(tot_nsh);
+  at_sh0 = alloc<int>(tot_n); at_ao0 = alloc<int>(tot_n);
+  qsh = alloc<double>(tot_nsh); zsh = alloc<double>(tot_nsh); shift = alloc<double>(tot_nsh);
+  shgam = alloc<double>(tot_nsh); selfe = alloc<double>(tot_nsh); dsedcn = alloc<double>(tot_nsh);
+  jq = alloc<double>(tot_nsh);
+  ao_at = alloc<int>(tot_nao); ao_sh = alloc<int>(tot_nao);
+  emo = alloc<double>(tot_nao); focc = alloc<double>(tot_nao); focca = alloc<double>(tot_nao);
+  foccb = alloc<double>(tot_nao); ev = alloc<double>(tot_nao);
+  row_q = alloc<double>(tot_nao); row_e = alloc<double>(tot_nao); row_d = alloc<double>(3 * tot_nao);
+  row_qp = alloc<double>(6 * tot_nao); row_pd = alloc<double>(tot_nao);
+  d4_at = alloc<int>(tot_nd4); d4_ref = alloc<int>(tot_nd4);
+  gw = alloc<double>(tot_nd4); zv = alloc<double>(tot_nd4); dzv = alloc<double>(tot_nd4); d4dum = alloc<double>(tot_nd4);
+  c6ref = alloc<double>((size_t)(XTB_MAX_Z + 1) * (XTB_MAX_Z + 1) * 49);
+  zetavec = alloc<double>(XTB_MAX_REF * tot_n); zerovec = alloc<double>(XTB_MAX_REF * tot_n);
+  zetadcn = alloc<double>(XTB_MAX_REF * tot_n); zerodcn = alloc<double>(XTB_MAX_REF * tot_n);
+  S = alloc<double>(tot_mm); H0 = alloc<double>(tot_mm); H0n = alloc<double>(tot_mm);
+  D = alloc<double>(3 * tot_mm); Q = alloc<double>(6 * tot_mm);
+  H = alloc<double>(tot_mm); C = alloc<double>(tot_mm); T = alloc<double>(tot_mm);
+  A = alloc<double>(tot_mm); W = alloc<double>(tot_mm); P = alloc<double>(tot_mm);
+  J = alloc<double>(tot_jj); dispmat = alloc<double>(tot_xx);
+  gab3 = alloc<double>(tot_pp); gab5 = alloc<double>(tot_pp);
+  c6 = alloc<double>(tot_pp); dc6 = alloc<double>(tot_pp); c60 = alloc<double>(tot_pp); dc60 = alloc<double>(tot_pp);
+  E = alloc<double>(tot_ee); WE = alloc<double>(tot_ee);
+  evE = alloc<double>(tot_n + nmol); xE = alloc<double>(tot_n + nmol);
+  q_in = alloc<double>(tot_v); dq = alloc<double>(tot_v); qlast = alloc<double>(tot_v); dqlast = alloc<double>(tot_v);
+  bdf = alloc<double>(tot_h); bu = alloc<double>(tot_h); ba = alloc<double>(tot_g);
+  bomega = alloc<double>((long long)nmol * mx);
+  // reference C6 of all element pairs present
+  {
+    std::vector<int> present(XTB_MAX_Z + 1, 0);
+    for (int z : h_at) present[z] = 1;
+    std::vector<double> tab((size_t)(XTB_MAX_Z + 1) * (XTB_MAX_Z + 1) * 49, 0.0);
+    for (int za = 1; za <= XTB_MAX_Z; ++za) {
+      if (!present[za]) continue;
+      for (int zb = 1; zb <= XTB_MAX_Z; ++zb) {
+        if (!present[zb]) continue;
+        for (int r = 0; r < el[za].nref; ++r)
+          for (int s2 = 0; s2 < el[zb].nref; ++s2)
+            tab[((size_t)za * (XTB_MAX_Z + 1) + zb) * 49 + r * 7 + s2] = host_ref_c6(za, r, zb, s2);
+      }
+    }
+    HIPCHECK(hipMemcpy(c6ref, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  auto up_i = [&](int* d_, const std::vector<int>& h) {
+    if (!h.empty()) HIPCHECK(hipMemcpy(d_, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  };
+  up_i(at, h_at); up_i(sh_at, h_sh_at); up_i(sh_l, h_sh_l); up_i(sh_ao0, h_sh_ao0); up_i(sh_m, h_sh_m);
+  up_i(at_sh0, h_at_sh0); up_i(at_ao0, h_at_ao0); up_i(ao_at, h_ao_at); up_i(ao_sh, h_ao_sh);
+  up_i(d4_at, h_d4_at); up_i(d4_ref, h_d4_ref);
+  HIPCHECK(hipMemcpy(xyz, h_xyz.data(), h_xyz.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_desc, desc.data(), nmol * sizeof(MolDesc), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d_state, h_state.data(), nmol * sizeof(MolState), hipMemcpyHostToDevice));
+  Element* d_el = alloc<Element>(XTB_MAX_Z + 1);
+  HIPCHECK(hipMemcpy(d_el, el, (XTB_MAX_Z + 1) * sizeof(Element), hipMemcpyHostToDevice));
+  elements_dev = d_el;
+  // GEMM tiles over (molecule, 64x64 tile)
+  std::vector<int4> tl;
+  for (int m = 0; m < nmol; ++m) {
+    const int nt = (desc[m].nao + 63) / 64;
+    for (int a = 0; a < nt; ++a)
+      for (int b = 0; b < nt; ++b) tl.push_back(make_int4(m, a, b, 0));
+  }
+  ntiles = (int)tl.size();
+  d_tiles = alloc<int4>(ntiles);
+  HIPCHECK(hipMemcpy(d_tiles, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice));
+  // Jacobi problem sets: S and H (dim nao, matrices A/W at m0), EEQ (dim ne at e0)
+  auto make_set = [&](bool eeq, JacSet& js) {
+    long long qoff = 0;
+    for (int m = 0; m < nmol; ++m) {
+      JacobiProb p;
+      std::memset(&p, 0, sizeof(p));
+      p.dim = eeq ? desc[m].ne : desc[m].nao;
+      p.nb = (p.dim + 31) / 32;
+      p.nbp = p.nb + (p.nb & 1);
+      p.rounds = std::max(1, p.nbp - 1);
+      p.a_off = eeq ? desc[m].e0 : desc[m].m0;
+      p.w_off = p.a_off;
+      p.ev_off = eeq ? desc[m].a0 + m : desc[m].o0;
+      p.q_off = qoff;
+      if (p.dim > 64) qoff += (long long)(p.nbp / 2) * 4096;
+      js.probs.push_back(p);
+    }
+    js.qsize = std::max<long long>(qoff, 4096);
+    js.d_probs = alloc<JacobiProb>(nmol);
+    HIPCHECK(hipMemcpy(js.d_probs, js.probs.data(), nmol * sizeof(JacobiProb), hipMemcpyHostToDevice));
+    std::vector<int> small;
+    std::vector<int2> pairs;
+    std::vector<int4> ta, tw;
+    js.max_rounds = 0;
+    for (int m = 0; m < nmol; ++m) {
+      const JacobiProb& p = js.probs[m];
+      if (p.dim <= 64) { small.push_back(m); continue; }
+      js.max_rounds = std::max(js.max_rounds, p.rounds);
+      const int np = p.nbp / 2;
+      for (int k = 0; k < np; ++k) pairs.push_back(make_int2(m, k));
+      for (int k = 0; k < np; ++k)
+        for (int l = k; l < np; ++l) ta.push_back(make_int4(m, k, l, 0));
+      const int nrb = (p.dim + 63) / 64;
+      for (int rb = 0; rb < nrb; ++rb)
+        for (int l = 0; l < np; ++l) tw.push_back(make_int4(m, rb, l, 0));
+    }
+    js.nsmall = (int)small.size();
+    js.npairs = (int)pairs.size();
+    js.nta = (int)ta.size();
+    js.ntw = (int)tw.size();
+    js.d_small = alloc<int>(small.size());
+    js.d_pairs = alloc<int2>(pairs.size());
+    js.d_ta = alloc<int4>(ta.size());
+    js.d_tw = alloc<int4>(tw.size());
+    if (!small.empty()) HIPCHECK(hipMemcpy(js.d_small, small.data(), small.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!pairs.empty()) HIPCHECK(hipMemcpy(js.d_pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!ta.empty()) HIPCHECK(hipMemcpy(js.d_ta, ta.data(), ta.size() * sizeof(int4), hipMemcpyHostToDevice));
+    if (!tw.empty()) HIPCHECK(hipMemcpy(js.d_tw, tw.data(), tw.size() * sizeof(int4), hipMemcpyHostToDevice));
+    js.qbuf = alloc<double>(js.qsize);
+    js.rot = alloc<int>(js.qsize / 4096 + 1);
+    js.state = alloc<int>(nmol * kJacState);
+    js.scale = alloc<double>(nmol);
+    js.perm = nullptr;
+  };
+  make_set(false, jac_n);
+  make_set(true, jac_e);
+  jac_nactive = alloc<int>(1);
+  yE = alloc<double>(tot_n + nmol);
+  erep = alloc<double>(tot_n);
+  perm = alloc<int>(tot_nao + tot_n + nmol);
+  eeq_failed = alloc<int>(nmol);
+  wbo = alloc<double>(tot_pp);
+  (void)evE;
+  HIPCHECK(hipHostMalloc((void**)&h_pinned, 64 * sizeof(int), hipHostMallocDefault));
+}
+
+Batch::~Batch() {
+  for (void* p : allocs_) (void)hipFree(p);
+  if (h_pinned) (void)hipHostFree(h_pinned);
+}
+
+// ---------------------------------------------------------------------------
+void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active) {
+  const int nb = nmol;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, jac_nactive, 1LL, 0);
+  hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
+                     check_active ? (const MolState*)d_state : (const MolState*)nullptr, jac_nactive);
+  if (js.nsmall)
+    hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
+                       (const double*)js.scale, js.state);
+  if (js.npairs) {
+    const int maxr = kMaxSweeps * js.max_rounds + 2;
+    int r = 0;
+    while (r < maxr) {
+      const int chunk = js.max_rounds;
+      for (int c = 0; c < chunk && r < maxr; ++c, ++r) {
+        hipLaunchKernelGGL(k_jac_pairs, dim3(js.npairs), dim3(256), 0, stream, js.d_probs, js.d_pairs, r,
+                           (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, js.state, jac_nactive);
+        hipLaunchKernelGGL(k_jac_apply_a, dim3(js.nta), dim3(256), 0, stream, js.d_probs, js.d_ta, r, Am,
+                           (const double*)js.qbuf, (const int*)js.rot, js.state);
+        hipLaunchKernelGGL(k_jac_apply_w, dim3(js.ntw), dim3(256), 0, stream, js.d_probs, js.d_tw, r, Wm,
+                           (const double*)js.qbuf, (const int*)js.rot, js.state);
+      }
+      HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      if (h_pinned[0] <= 0) break;
+      jacobi_rounds_ += chunk;
+    }
+  }
+  if (evout)
+    hipLaunchKernelGGL(k_jac_sort, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm,
+                       check_active ? (const MolState*)d_state : (const MolState*)nullptr);
+}
+
+void Batch::gemm(int kind, const double* a, const double* b, double* c, const double* scale_, const int* perm_,
+                 bool check_active) {
+  const MolState* st = check_active ? d_state : nullptr;
+  switch (kind) {
+    case 0:
+      hipLaunchKernelGGL((k_gemm<false, false, false, false>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
+                         c, scale_, perm_, 1.0, st);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_gemm<true, false, false, false>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
+                         c, scale_, perm_, 1.0, st);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_gemm<false, true, true, false>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
+                         c, scale_, perm_, 1.0, st);
+      break;
+    default:
+      hipLaunchKernelGGL((k_gemm<false, false, false, true>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
+                         c, scale_, perm_, 1.0, st);
+      break;
+  }
+}
+
+void Batch::setup() {
+  const Element* el = elements_dev;
+  const dim3 gat(max_n, nmol);
+  hipLaunchKernelGGL(k_cn, gat, dim3(64), 0, stream, d_desc, el, at, xyz, cn, cn_cov, cn_d4, cn_eeq);
+  hipLaunchKernelGGL(k_shell_setup, dim3(grid1(max_nsh, 64), nmol), dim3(64), 0, stream, d_desc, el, at, sh_at, sh_m,
+                     cn, selfe, dsedcn, shgam, zsh);
+  const long long maxpairs = (long long)max_n * (max_n - 1) / 2;
+  if (maxpairs > 0)
+    hipLaunchKernelGGL(k_sdqh0, dim3(grid1(maxpairs, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0,
+                       at_ao0, selfe, intcut, S, H0, H0n, D, Q);
+  hipLaunchKernelGGL(k_sdq_onsite, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0,
+                     at_ao0, selfe, intcut, S, H0, H0n, D, Q);
+  hipLaunchKernelGGL(k_coulomb, dim3(grid1((long long)max_nsh * max_nsh, 256), nmol), dim3(256), 0, stream, d_desc,
+                     el, at, xyz, sh_at, sh_m, J);
+  hipLaunchKernelGGL(k_d4_weights, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, cn_d4, gw);
+  hipLaunchKernelGGL(k_dispmat, dim3(grid1((long long)max_nd4 * max_nd4, 256), nmol), dim3(256), 0, stream, d_desc,
+                     el, at, xyz, d4_at, d4_ref, gw, c6ref, dispmat);
+  hipLaunchKernelGGL(k_aes_setup, dim3(grid1((long long)max_n * max_n, 256), nmol), dim3(256), 0, stream, d_desc, el,
+                     at, xyz, cn, radcn, gab3, gab5);
+  // initial charges: EEQ (default) or SAD
+  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nmol, 64)), dim3(64), 0, stream, eeq_failed, (long long)nmol, 0);
+  if (opt.guess == 0) {
+    const long long ne = max_n + 1;
+    hipLaunchKernelGGL(k_eeq_matrix, dim3(grid1(ne * ne, 256), nmol), dim3(256), 0, stream, d_desc, el, at, xyz,
+                       cn_eeq, E, xE);
+    jacobi(jac_e, E, WE, nullptr, false);
+  }
+  hipLaunchKernelGGL(k_eeq_solve, dim3(nmol), dim3(256), 0, stream, d_desc, el, at, at_sh0, (const double*)WE,
+                     (const double*)E, (const double*)xE, yE, q, qsh, zsh,
+                     opt.guess == 0 ? 1 : 0, eeq_failed);
+  hipLaunchKernelGGL(k_fill, dim3(grid1(3 * tot_n, 256)), dim3(256), 0, stream, dipm, 3LL * tot_n, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(grid1(6 * tot_n, 256)), dim3(256), 0, stream, qp, 6LL * tot_n, 0.0);
+  // orthogonaliser of S -> initial orbital basis C
+  hipLaunchKernelGGL(k_copy, dim3(grid1(tot_mm, 256)), dim3(256), 0, stream, (const double*)S, A, tot_mm);
+  jacobi(jac_n, A, W, nullptr, false);
+  hipLaunchKernelGGL(k_ortho, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream, d_desc,
+                     (const double*)A, (const double*)W, C);
+  HIPCHECK(hipGetLastError());
+  setup_done_ = true;
+}
+
+bool Batch::scc_iteration() {
+  const Element* el = elements_dev;
+  const double thr2 = neglect * 1.0e-2;
+  hipLaunchKernelGGL(k_shell_pot, dim3(grid1(max_nsh, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)J,
+                     (const double*)qsh, (const double*)shgam, shift, 1, (const MolState*)d_state);
+  hipLaunchKernelGGL(k_aes_pot, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, q, dipm, qp,
+                     gab3, gab5, 1, vs, vd, vq, (const MolState*)d_state);
+  hipLaunchKernelGGL(k_d4_zeta, dim3(grid1(max_nd4, 64), nmol), dim3(64), 0, stream, d_desc, el, at, d4_at, d4_ref,
+                     gw, q, zv, dzv, (const MolState*)d_state);
+  hipLaunchKernelGGL(k_d4_symv, dim3(grid1(max_nd4, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)dispmat,
+                     (const double*)zv, d4dum, (const MolState*)d_state);
+  hipLaunchKernelGGL(k_shift_final, dim3(grid1(max_nsh, 64), nmol), dim3(64), 0, stream, d_desc, el, at, sh_at,
+                     (const double*)d4dum, (const double*)dzv, shift, (const MolState*)d_state);
+  hipLaunchKernelGGL(k_fock, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream, d_desc, S, H0,
+                     D, Q, shift, vs, vd, vq, ao_sh, ao_at, thr2, H, (const MolState*)d_state);
+  // generalized eigenproblem in the current orbital basis C (S-orthonormal)
+  gemm(0, H, C, T, nullptr, nullptr, true);   // T = H C
+  gemm(1, C, T, A, nullptr, nullptr, true);   // A = C^T H C
+  jacobi(jac_n, A, W, emo, true);
+  gemm(3, C, W, T, nullptr, perm, true);      // T = C W(:, perm)
+  hipLaunchKernelGGL(k_copy_active, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream,
+                     d_desc, (const MolState*)d_state, (const double*)T, C);
+  hipLaunchKernelGGL(k_occ_fermi, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, (const double*)emo, focca, foccb,
+                     focc, opt.etemp, (const double*)qsh, (const double*)dipm, (const double*)qp, q_in);
+  gemm(2, C, C, P, focc, nullptr, true);      // P = C diag(f) C^T
+  hipLaunchKernelGGL(k_population, dim3(grid1(max_nao, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)P,
+                     (const double*)S, (const double*)H0, (const double*)D, (const double*)Q, row_q, row_e, row_d,
+                     row_qp, (const MolState*)d_state, 0);
+  hipLaunchKernelGGL(k_pop_reduce, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0,
+                     at_ao0, (const double*)zsh, (const double*)row_q, (const double*)row_d, (const double*)row_qp,
+                     qsh, q, dipm, qp, (const MolState*)d_state, 0);
+  hipLaunchKernelGGL(k_shell_pot, dim3(grid1(max_nsh, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)J,
+                     (const double*)qsh, (const double*)shgam, jq, 0, (const MolState*)d_state);
+  hipLaunchKernelGGL(k_aes_energy, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, q, dipm,
+                     qp, gab3, gab5, ae01, (const MolState*)d_state);
+  hipLaunchKernelGGL(k_d4_zeta, dim3(grid1(max_nd4, 64), nmol), dim3(64), 0, stream, d_desc, el, at, d4_at, d4_ref,
+                     gw, q, zv, (double*)nullptr, (const MolState*)d_state);
+  hipLaunchKernelGGL(k_d4_symv, dim3(grid1(max_nd4, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)dispmat,
+                     (const double*)zv, d4dum, (const MolState*)d_state);
+  hipLaunchKernelGGL(k_scc_control, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, (const double*)jq,
+                     (const double*)shgam, (const double*)row_e, (const double*)ae01, (const double*)zv,
+                     (const double*)d4dum, sh_at, qsh, q, dipm, qp, q_in, dq, qlast, dqlast, bdf, bu, ba, bomega,
+                     scfconv, qconv);
+  HIPCHECK(hipGetLastError());
+  iterations_ += 1;
+  // any molecule still active?
+  hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, stream, (const MolState*)d_state, nmol, jac_nactive);
+  HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  return h_pinned[0] > 0;
+}
+
+void Batch::gradient() {
+  const Element* el = elements_dev;
+  // energy-weighted density into T
+  hipLaunchKernelGGL(k_pew_scale, dim3(grid1(max_nao, 64), nmol), dim3(64), 0, stream, d_desc, (const double*)focc,
+                     (const double*)emo, row_pd);
+  gemm(2, C, C, T, row_pd, nullptr, false);
+  hipLaunchKernelGGL(k_aes_pot, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, q, dipm, qp,
+                     gab3, gab5, 0, vs, vd, vq, (const MolState*)nullptr);
+  hipLaunchKernelGGL(k_fill, dim3(grid1(3 * tot_n, 256)), dim3(256), 0, stream, grad, 3LL * tot_n, 0.0);
+  const dim3 gat(max_n, nmol);
+  hipLaunchKernelGGL(k_grad_h0, gat, dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0, at_ao0, (const double*)selfe,
+                     (const double*)dsedcn, (const double*)P, (const double*)T, (const double*)H0n, (const double*)S,
+                     (const double*)shift, (const double*)vs, (const double*)vd, (const double*)vq, intcut, grad,
+                     dhdcn);
+  hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_desc, xyz, q, dipm, qp, radcn, gab3, gab5, grad, tmp2);
+  hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, cn_cov, q,
+                     zetavec, zerovec, zetadcn, zerodcn);
+  hipLaunchKernelGGL(k_d4_c6, dim3(grid1((long long)max_n * max_n, 256), nmol), dim3(256), 0, stream, d_desc, el, at,
+                     c6ref, zetavec, zerovec, zetadcn, zerodcn, c6, dc6, c60, dc60);
+  hipLaunchKernelGGL(k_d4_grad2, gat, dim3(64), 0, stream, d_desc, el, at, xyz, (const double*)c6, (const double*)dc6,
+                     grad, dedcn);
+  hipLaunchKernelGGL(k_d4_atm, gat, dim3(256), 0, stream, d_desc, el, at, xyz, (const double*)c60, (const double*)dc60,
+                     grad, dedcn, eat);
+  hipLaunchKernelGGL(k_grad_pairs, gat, dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0, (const double*)qsh, grad,
+                     erep);
+  hipLaunchKernelGGL(k_cn_chain, gat, dim3(64), 0, stream, d_desc, el, at, xyz, (const double*)cn,
+                     (const double*)dhdcn, (const double*)tmp2, (const double*)dedcn, grad);
+  // bond orders from P S
+  gemm(0, P, S, A, nullptr, nullptr, false);
+  hipLaunchKernelGGL(k_wbo, dim3(grid1((long long)max_n * max_n, 256), nmol), dim3(256), 0, stream, d_desc, xyz,
+                     at_ao0, at, el, (const double*)A, wbo);
+  hipLaunchKernelGGL(k_final, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, el, at, xyz, (const double*)grad,
+                     (const double*)erep, (const double*)eat, (const double*)row_d);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(stream));
+}
+
+void Batch::run() {
+  setup();
+  int guard = 0;
+  while (scc_iteration()) {
+    if (++guard > opt.maxiter + 2) break;
+  }
+  gradient();
+}
+
+MolResult Batch::result(int m, bool matrices) const {
+  MolResult r;
+  const MolDesc& d = desc[m];
+  MolState st;
+  HIPCHECK(hipMemcpy(&st, d_state + m, sizeof(MolState), hipMemcpyDeviceToHost));
+  r.n = d.n;
+  r.nsh = d.nsh;
+  r.nao = d.nao;
+  r.iterations = st.iter;
+  r.converged = st.converged;
+  r.failed = st.failed;
+  r.ihomo = st.ihomo;
+  r.energy = st.etotal;
+  r.gnorm = st.gnorm;
+  r.hl_gap_ev = st.egap;
+  r.e_elec = st.eel + st.embd;
+  r.e_es = st.ees;
+  r.e_aes = st.eaes;
+  r.e_axc = st.epol;
+  r.e_disp = st.ed + st.embd;
+  r.e_rep = st.ep;
+  r.e_atm = st.embd;
+  for (int k = 0; k < 3; ++k) r.dipole[k] = st.dipole[k];
+  auto get = [&](std::vector<double>& v, const double* src, long long off, long long cnt) {
+    v.resize(cnt);
+    HIPCHECK(hipMemcpy(v.data(), src + off, cnt * sizeof(double), hipMemcpyDeviceToHost));
+  };
+  get(r.gradient, grad, 3LL * d.a0, 3LL * d.n);
+  get(r.q, q, d.a0, d.n);
+  get(r.qsh, qsh, d.s0, d.nsh);
+  get(r.dipm, dipm, 3LL * d.a0, 3LL * d.n);
+  get(r.qp, qp, 6LL * d.a0, 6LL * d.n);
+  get(r.wbo, wbo, d.p0, (long long)d.n * d.n);
+  get(r.emo, emo, d.o0, d.nao);
+  get(r.focc, focc, d.o0, d.nao);
+  if (matrices) {
+    get(r.coeff, C, d.m0, (long long)d.nao * d.nao);
+    get(r.density, P, d.m0, (long long)d.nao * d.nao);
+  }
+  return r;
+}
+
+}  // namespace xtbamd
+
+// ===========================================================================
+// C ABI of the batch interface (include/xtb_amd_batch.h)
+#include "../../include/xtb_amd_batch.h"
+
+struct xtbamd_batch_s {
+  xtbamd::Batch* batch = nullptr;
+};
+
+namespace {
+thread_local std::string g_last_error;
+int fail(const std::exception& e) {
+  g_last_error = e.what();
+  return -1;
+}
+}  // namespace
+
+extern "C" {
+
+const char* xtbamd_last_error(void) { return g_last_error.c_str(); }
+
+int xtbamd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+xtbamd_batch xtbamd_batch_new(int nmol, const int* natoms, const int* numbers, const double* positions,
+                              const double* charges, const int* uhf, double accuracy, double etemp, int maxiter,
+                              int guess) {
+  try {
+    if (nmol <= 0 || !natoms || !numbers || !positions) throw std::runtime_error("invalid batch arguments");
+    std::vector<xtbamd::MoleculeInput> mols(nmol);
+    long long off = 0;
+    for (int m = 0; m < nmol; ++m) {
+      const int n = natoms[m];
+      if (n <= 0) throw std::runtime_error("molecule without atoms");
+      mols[m].numbers.assign(numbers + off, numbers + off + n);
+      mols[m].xyz.assign(positions + 3 * off, positions + 3 * (off + n));
+      mols[m].charge = charges ? charges[m] : 0.0;
+      mols[m].uhf = uhf ? uhf[m] : 0;
+      off += n;
+    }
+    xtbamd::Options o;
+    o.accuracy = accuracy;
+    o.etemp = etemp;
+    o.maxiter = maxiter;
+    o.guess = guess;
+    auto* h = new xtbamd_batch_s;
+    h->batch = new xtbamd::Batch(mols, o);
+    return h;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return nullptr;
+  }
+}
+
+int xtbamd_batch_setup(xtbamd_batch b) {
+  try { b->batch->setup(); return 0; } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_iterate(xtbamd_batch b) {
+  try {
+    const bool any = b->batch->scc_iteration();
+    return any ? 1 : 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_gradient(xtbamd_batch b) {
+  try { b->batch->gradient(); return 0; } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_run(xtbamd_batch b) {
+  try { b->batch->run(); return 0; } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_size(xtbamd_batch b) { return b ? b->batch->size() : 0; }
+
+int xtbamd_batch_get(xtbamd_batch b, int m, xtbamd_result* out) {
+  try {
+    if (!b || !out || m < 0 || m >= b->batch->size()) throw std::runtime_error("invalid result request");
+    const bool mats = out->coeff || out->density;
+    xtbamd::MolResult r = b->batch->result(m, mats);
+    out->natoms = r.n;
+    out->nshells = r.nsh;
+    out->nao = r.nao;
+    out->iterations = r.iterations;
+    out->converged = r.converged;
+    out->failed = r.failed;
+    out->ihomo = r.ihomo;
+    out->energy = r.energy;
+    out->gnorm = r.gnorm;
+    out->hl_gap_ev = r.hl_gap_ev;
+    out->e_elec = r.e_elec;
+    out->e_es = r.e_es;
+    out->e_aes = r.e_aes;
+    out->e_axc = r.e_axc;
+    out->e_disp = r.e_disp;
+    out->e_rep = r.e_rep;
+    out->e_atm = r.e_atm;
+    for (int k = 0; k < 3; ++k) out->dipole[k] = r.dipole[k];
+    auto put = [](double* dst, const std::vector<double>& v) {
+      if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+    };
+    put(out->gradient, r.gradient);
+    put(out->charges, r.q);
+    put(out->qsh, r.qsh);
+    put(out->dipm, r.dipm);
+    put(out->qp, r.qp);
+    put(out->wbo, r.wbo);
+    put(out->emo, r.emo);
+    put(out->focc, r.focc);
+    put(out->coeff, r.coeff);
+    put(out->density, r.density);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+void xtbamd_batch_free(xtbamd_batch b) {
+  if (!b) return;
+  delete b->batch;
+  delete b;
+}
+
+}  // extern "C"

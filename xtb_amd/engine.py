@@ -1,0 +1,151 @@
+"""ctypes binding of the batch C ABI (include/xtb_amd_batch.h).
+
+``singlepoint`` mirrors the behaviour of xtb_singlepoint for one molecule
+(src/api/interface.f90:40-156); ``Batch`` runs many molecules as one batch.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libxtb_amd.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class Result(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int) for k in ("natoms", "nshells", "nao", "iterations", "converged", "failed", "ihomo")] + \
+        [(k, ctypes.c_double) for k in ("energy", "gnorm", "hl_gap_ev", "e_elec", "e_es", "e_aes", "e_axc",
+                                        "e_disp", "e_rep", "e_atm")] + \
+        [("dipole", ctypes.c_double * 3)] + \
+        [(k, _dp) for k in ("gradient", "charges", "qsh", "dipm", "qp", "wbo", "emo", "focc", "coeff", "density")]
+
+
+def library():
+    """Load libxtb_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"{LIBPATH} is missing: build it with `python -m xtb_amd.build`")
+    lib = ctypes.CDLL(LIBPATH)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.xtbamd_batch_new.restype = ctypes.c_void_p
+    lib.xtbamd_batch_new.argtypes = [ctypes.c_int, ip, ip, _dp, _dp, ip, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_int]
+    lib.xtbamd_last_error.restype = ctypes.c_char_p
+    for f in ("xtbamd_batch_setup", "xtbamd_batch_iterate", "xtbamd_batch_gradient", "xtbamd_batch_run",
+              "xtbamd_batch_size"):
+        getattr(lib, f).argtypes = [ctypes.c_void_p]
+        getattr(lib, f).restype = ctypes.c_int
+    lib.xtbamd_batch_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Result)]
+    lib.xtbamd_batch_get.restype = ctypes.c_int
+    lib.xtbamd_batch_free.argtypes = [ctypes.c_void_p]
+    lib.xtbamd_batch_free.restype = None
+    lib.xtbamd_device_count.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def device_count():
+    return library().xtbamd_device_count()
+
+
+def _err(lib):
+    return lib.xtbamd_last_error().decode()
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+class Batch:
+    """Many GFN2-xTB single points computed together on the GPU.
+
+    molecules: list of (numbers, positions[bohr], charge, uhf) tuples.
+    """
+
+    def __init__(self, molecules, accuracy=1.0, etemp=300.0, maxiter=250, guess="eeq"):
+        lib = library()
+        self._lib = lib
+        nat = np.array([len(m[0]) for m in molecules], dtype=np.int32)
+        num = np.ascontiguousarray(np.concatenate([np.asarray(m[0], dtype=np.int32) for m in molecules]))
+        pos = np.ascontiguousarray(np.concatenate([np.asarray(m[1], dtype=np.float64).reshape(-1, 3)
+                                                   for m in molecules]))
+        chg = np.array([float(m[2]) if len(m) > 2 else 0.0 for m in molecules], dtype=np.float64)
+        uhf = np.array([int(m[3]) if len(m) > 3 else 0 for m in molecules], dtype=np.int32)
+        g = {"eeq": 0, "sad": 1}[guess] if isinstance(guess, str) else int(guess)
+        ip = ctypes.POINTER(ctypes.c_int)
+        h = lib.xtbamd_batch_new(len(molecules), nat.ctypes.data_as(ip), num.ctypes.data_as(ip), _ptr(pos),
+                                 _ptr(chg), uhf.ctypes.data_as(ip), accuracy, etemp, maxiter, g)
+        if not h:
+            raise RuntimeError(_err(lib))
+        self._h = h
+        self.natoms = nat
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RuntimeError(_err(self._lib))
+        return rc
+
+    def setup(self):
+        self._check(self._lib.xtbamd_batch_setup(self._h))
+
+    def iterate(self):
+        return self._check(self._lib.xtbamd_batch_iterate(self._h))
+
+    def gradient(self):
+        self._check(self._lib.xtbamd_batch_gradient(self._h))
+
+    def run(self):
+        self._check(self._lib.xtbamd_batch_run(self._h))
+        return self
+
+    def __len__(self):
+        return len(self.natoms)
+
+    def result(self, m, matrices=False):
+        lib = self._lib
+        r = Result()
+        self._check(lib.xtbamd_batch_get(self._h, m, ctypes.byref(r)))   # sizes first
+        n, nsh, nao = r.natoms, r.nshells, r.nao
+        out = {"gradient": np.zeros((n, 3)), "charges": np.zeros(n), "qsh": np.zeros(nsh),
+               "dipm": np.zeros((n, 3)), "qp": np.zeros((n, 6)), "wbo": np.zeros((n, n)),
+               "emo": np.zeros(nao), "focc": np.zeros(nao)}
+        if matrices:
+            out["coeff"] = np.zeros((nao, nao))
+            out["density"] = np.zeros((nao, nao))
+        for k, v in out.items():
+            setattr(r, k, _ptr(v))
+        self._check(lib.xtbamd_batch_get(self._h, m, ctypes.byref(r)))
+        for k in ("natoms", "nshells", "nao", "iterations", "converged", "failed", "ihomo", "energy", "gnorm",
+                  "hl_gap_ev", "e_elec", "e_es", "e_aes", "e_axc", "e_disp", "e_rep", "e_atm"):
+            out[k] = getattr(r, k)
+        out["dipole"] = np.array(r.dipole[:])
+        return out
+
+    def results(self, matrices=False):
+        return [self.result(m, matrices) for m in range(len(self))]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.xtbamd_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def singlepoint(numbers, positions, charge=0.0, uhf=0, accuracy=1.0, etemp=300.0, maxiter=250, guess="eeq",
+                matrices=False):
+    b = Batch([(numbers, positions, charge, uhf)], accuracy, etemp, maxiter, guess)
+    try:
+        b.run()
+        return b.result(0, matrices)
+    finally:
+        b.close()
