@@ -1,0 +1,143 @@
+"""Benchmark: GFN2-xTB SCC single points with gradient on one MI355X.
+
+Workload: 512 independent neutral singlet molecules, each a seeded water
+cluster or all-trans alkane (probability 0.5), sizes log-uniform in 10..200
+atoms.  A step is one batched pass (setup, SCC to convergence, gradient) over
+all 512 molecules with inputs resident on the device.  The plain path runs the
+same molecules one at a time through the same FP64 engine.
+
+Prints one JSON line (see the fields below).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from xtb_amd import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _cpu_worker(args):
+    z, x = args
+    from oracle import pyoracle as po
+    t = time.perf_counter()
+    r = po.singlepoint(z, x, lapack=True, threads=1)
+    return r["e_total"], time.perf_counter() - t
+
+
+def cpu_baseline(mols, budget_s, procs):
+    """The FP64 C restatement of the reference path (oracle), one molecule per
+    process, over the first molecules of the workload until budget_s elapses."""
+    import multiprocessing as mp
+    ctx = mp.get_context("fork")   # runs before anything touches the GPU
+    t0 = time.perf_counter()
+    energies, done = [], 0
+    with ctx.Pool(procs) as pool:
+        it = pool.imap(_cpu_worker, mols, chunksize=1)
+        for e, _ in it:
+            energies.append(e)
+            done += 1
+            if time.perf_counter() - t0 > budget_s and done % procs == 0:
+                pool.terminate()
+                break
+    wall = time.perf_counter() - t0
+    return done, wall, energies
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nmol", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--plain", type=int, default=512, help="molecules timed on the plain one-at-a-time path")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    mols = workloads.mixed_batch(args.nmol, seed=args.seed)
+    natoms = np.array([len(z) for z, _ in mols])
+    naos = np.array([workloads.ao_count(z) for z, _ in mols])
+
+    cpu = None
+    if not args.no_cpu:
+        done, wall, ecpu = cpu_baseline(mols, args.cpu_budget, args.cpu_procs)
+        cpu = {"value": done / wall, "unit": "single-points/s", "cores": args.cpu_procs, "kind": "port",
+               "sample": f"first {done} molecules of the workload, one per process, FP64 C oracle with "
+                         f"MKL dsygvd/dgemm when loadable, {wall:.1f} s wall"}
+
+    import xtb_amd  # first GPU use
+    batch_in = [(z, x, 0.0, 0) for z, x in mols]
+
+    # plain path: one molecule per batch, synchronous
+    lat = []
+    eplain = []
+    for m in range(min(args.plain, len(mols))):
+        t = time.perf_counter()
+        r = xtb_amd.singlepoint(mols[m][0], mols[m][1])
+        lat.append(time.perf_counter() - t)
+        eplain.append(r["energy"])
+    lat = np.array(lat)
+
+    # batched path
+    b = xtb_amd.Batch(batch_in)
+    for _ in range(args.warmup):
+        b.run()
+    times = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        b.run()
+        times.append(time.perf_counter() - t)
+    res = b.results()
+    ebatch = np.array([r["energy"] for r in res])
+    conv = all(r["converged"] for r in res)
+    iters = np.array([r["iterations"] for r in res])
+    ms = 1e3 * float(np.mean(times))
+
+    dE_plain = float(np.max(np.abs(ebatch[:len(eplain)] - np.array(eplain)))) if eplain else None
+    dE_cpu = None
+    if cpu is not None:
+        ec = np.array(ecpu)
+        dE_cpu = float(np.max(np.abs(ebatch[:len(ec)] - ec)))
+
+    out = {
+        "metric": "GFN2 SCC single-points/s + p50/p99 latency, batched vs plain, dE<=1e-6 Eh",
+        "value": len(mols) / (ms * 1e-3),
+        "unit": "single-points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{len(mols)} molecules (seeded water clusters / all-trans alkanes, "
+                               f"10-200 atoms log-uniform, seed {args.seed}), energy+gradient, batched",
+                   "atoms": int(natoms.sum()), "aos": int(naos.sum()), "max_ao": int(naos.max()),
+                   "parallelism": "single GPU"},
+        "batched": {"p50_ms": ms, "p99_ms": 1e3 * float(np.max(times)), "converged": conv,
+                    "scc_iterations_mean": float(iters.mean())},
+        "plain": {"molecules": len(lat), "value": len(lat) / float(lat.sum()) if len(lat) else None,
+                  "p50_ms": 1e3 * float(np.percentile(lat, 50)) if len(lat) else None,
+                  "p99_ms": 1e3 * float(np.percentile(lat, 99)) if len(lat) else None},
+        "parity": {"max_dE_batched_vs_plain": dE_plain, "max_dE_vs_oracle": dE_cpu},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    if out["plain"]["value"]:
+        out["speedup_batched_vs_plain"] = out["value"] / out["plain"]["value"]
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

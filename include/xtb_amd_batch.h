@@ -44,6 +44,10 @@ xtbamd_batch xtbamd_batch_new(int nmol, const int* natoms, const int* numbers, c
                               int maxiter, int guess);
 /* last error message of the calling thread (batch creation) or of the batch */
 const char* xtbamd_last_error(void);
+/* restart guess of molecule m (call before setup/run): shell charges [nshells],
+ * atomic dipoles [natoms][3], quadrupoles [natoms][6] of a previous result;
+ * mirrors the wavefunction reuse of xtb_singlepoint (src/api/interface.f90:83-101) */
+int xtbamd_batch_set_guess(xtbamd_batch b, int m, const double* qsh, const double* dipm, const double* qp);
 int xtbamd_batch_setup(xtbamd_batch b);
 /* one SCC iteration of every active molecule; returns the number still active */
 int xtbamd_batch_iterate(xtbamd_batch b);

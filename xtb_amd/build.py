@@ -21,6 +21,16 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+def _flang_runtime():
+    import glob
+    root = os.path.dirname(os.path.dirname(os.path.realpath(shutil.which("amdflang") or "/opt/rocm/bin/amdflang")))
+    hits = glob.glob(os.path.join(root, "**", "libflang_rt.runtime.a"), recursive=True) + \
+        glob.glob("/opt/rocm/lib/llvm/lib/clang/*/lib/*/libflang_rt.runtime.a")
+    if not hits:
+        raise RuntimeError("flang runtime library not found")
+    return hits[0]
+
+
 def build(verbose=True):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
@@ -42,7 +52,7 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + fobjs
     if fobjs:
-        link += ["-L/opt/rocm/lib/llvm/lib", "-lflang_rt.runtime"]
+        link += [_flang_runtime()]
     _run(link)
     os.replace(tmp, LIB)
     return LIB

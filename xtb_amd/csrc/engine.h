@@ -107,6 +107,9 @@ class Batch {
   void setup();
   bool scc_iteration();   // returns true while any molecule is active
   void gradient();
+  // restart guess for molecule m: shell charges, atomic dipoles [n][3] and
+  // quadrupoles [n][6] of a previous wavefunction (used by the next setup)
+  void set_guess(int m, const double* qsh_in, const double* dipm_in, const double* qp_in);
   MolResult result(int m, bool matrices = false) const;
   int size() const { return (int)mols_.size(); }
   const std::string& error() const { return error_; }
@@ -170,6 +173,8 @@ class Batch {
             bool check_active);
 
   std::vector<void*> allocs_;
+  std::vector<int> h_sh_at_;
+  std::vector<std::vector<double>> guess_qsh_, guess_dipm_, guess_qp_;
   bool setup_done_ = false;
   int iterations_ = 0;
   template <class T_>

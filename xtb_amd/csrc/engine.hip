@@ -330,6 +330,7 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
   up_i(at, h_at); up_i(sh_at, h_sh_at); up_i(sh_l, h_sh_l); up_i(sh_ao0, h_sh_ao0); up_i(sh_m, h_sh_m);
   up_i(at_sh0, h_at_sh0); up_i(at_ao0, h_at_ao0); up_i(ao_at, h_ao_at); up_i(ao_sh, h_ao_sh);
   up_i(d4_at, h_d4_at); up_i(d4_ref, h_d4_ref);
+  h_sh_at_ = h_sh_at;
   HIPCHECK(hipMemcpy(xyz, h_xyz.data(), h_xyz.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(d_desc, desc.data(), nmol * sizeof(MolDesc), hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(d_state, h_state.data(), nmol * sizeof(MolState), hipMemcpyHostToDevice));
@@ -450,6 +451,19 @@ void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
                        check_active ? (const MolState*)d_state : (const MolState*)nullptr);
 }
 
+void Batch::set_guess(int m, const double* qsh_in, const double* dipm_in, const double* qp_in) {
+  if (m < 0 || m >= nmol) throw std::runtime_error("invalid molecule index");
+  if (guess_qsh_.empty()) {
+    guess_qsh_.resize(nmol);
+    guess_dipm_.resize(nmol);
+    guess_qp_.resize(nmol);
+  }
+  const MolDesc& d = desc[m];
+  guess_qsh_[m].assign(qsh_in, qsh_in + d.nsh);
+  guess_dipm_[m].assign(dipm_in, dipm_in + 3 * d.n);
+  guess_qp_[m].assign(qp_in, qp_in + 6 * d.n);
+}
+
 void Batch::gemm(int kind, const double* a, const double* b, double* c, const double* scale_, const int* perm_,
                  bool check_active) {
   const MolState* st = check_active ? d_state : nullptr;
@@ -505,6 +519,20 @@ void Batch::setup() {
                      opt.guess == 0 ? 1 : 0, eeq_failed);
   hipLaunchKernelGGL(k_fill, dim3(grid1(3 * tot_n, 256)), dim3(256), 0, stream, dipm, 3LL * tot_n, 0.0);
   hipLaunchKernelGGL(k_fill, dim3(grid1(6 * tot_n, 256)), dim3(256), 0, stream, qp, 6LL * tot_n, 0.0);
+  // restart guesses replace the EEQ charges (src/api/interface.f90:83-101, src/scf_module.F90:292)
+  for (int m = 0; m < nmol; ++m) {
+    if (guess_qsh_.empty() || guess_qsh_[m].empty()) continue;
+    const MolDesc& d = desc[m];
+    std::vector<double> qa(d.n, 0.0);
+    for (int s = 0; s < d.nsh; ++s) qa[h_sh_at_[d.s0 + s]] += guess_qsh_[m][s];
+    HIPCHECK(hipMemcpyAsync(qsh + d.s0, guess_qsh_[m].data(), d.nsh * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(q + d.a0, qa.data(), d.n * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHECK(hipMemcpyAsync(dipm + 3 * d.a0, guess_dipm_[m].data(), 3 * d.n * sizeof(double), hipMemcpyHostToDevice,
+                            stream));
+    HIPCHECK(hipMemcpyAsync(qp + 6 * d.a0, guess_qp_[m].data(), 6 * d.n * sizeof(double), hipMemcpyHostToDevice,
+                            stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
   // orthogonaliser of S -> initial orbital basis C
   hipLaunchKernelGGL(k_copy, dim3(grid1(tot_mm, 256)), dim3(256), 0, stream, (const double*)S, A, tot_mm);
   jacobi(jac_n, A, W, nullptr, false);
@@ -710,6 +738,14 @@ xtbamd_batch xtbamd_batch_new(int nmol, const int* natoms, const int* numbers, c
     g_last_error = e.what();
     return nullptr;
   }
+}
+
+int xtbamd_batch_set_guess(xtbamd_batch b, int m, const double* qsh, const double* dipm, const double* qp) {
+  try {
+    if (!b || !qsh || !dipm || !qp) throw std::runtime_error("invalid guess arguments");
+    b->batch->set_guess(m, qsh, dipm, qp);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
 }
 
 int xtbamd_batch_setup(xtbamd_batch b) {

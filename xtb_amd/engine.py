@@ -40,6 +40,8 @@ def library():
               "xtbamd_batch_size"):
         getattr(lib, f).argtypes = [ctypes.c_void_p]
         getattr(lib, f).restype = ctypes.c_int
+    lib.xtbamd_batch_set_guess.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
+    lib.xtbamd_batch_set_guess.restype = ctypes.c_int
     lib.xtbamd_batch_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Result)]
     lib.xtbamd_batch_get.restype = ctypes.c_int
     lib.xtbamd_batch_free.argtypes = [ctypes.c_void_p]
@@ -89,6 +91,13 @@ class Batch:
         if rc < 0:
             raise RuntimeError(_err(self._lib))
         return rc
+
+    def set_guess(self, m, qsh, dipm, qp):
+        """Restart guess of molecule m from a previous result's qsh, dipm, qp."""
+        qsh = np.ascontiguousarray(qsh, dtype=np.float64)
+        dipm = np.ascontiguousarray(dipm, dtype=np.float64)
+        qp = np.ascontiguousarray(qp, dtype=np.float64)
+        self._check(self._lib.xtbamd_batch_set_guess(self._h, m, _ptr(qsh), _ptr(dipm), _ptr(qp)))
 
     def setup(self):
         self._check(self._lib.xtbamd_batch_setup(self._h))
