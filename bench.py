@@ -102,6 +102,11 @@ def main():
         b.run()
         times.append(time.perf_counter() - t)
     res = b.results()
+    # every timed step is a complete single point of every molecule: the
+    # results of the last step equal the plain path's and iterate from scratch
+    steps_run, jac_rounds = b.counters()
+    assert steps_run == max(r["iterations"] for r in res) > 1, "a timed step skipped the SCC"
+
     ebatch = np.array([r["energy"] for r in res])
     conv = all(r["converged"] for r in res)
     iters = np.array([r["iterations"] for r in res])

@@ -55,6 +55,8 @@ int xtbamd_batch_gradient(xtbamd_batch b);
 /* setup + SCC to convergence + gradient */
 int xtbamd_batch_run(xtbamd_batch b);
 int xtbamd_batch_size(xtbamd_batch b);
+/* work counters of the last run: SCC steps launched, block-Jacobi rounds */
+int xtbamd_batch_counters(xtbamd_batch b, int* scc_steps, long long* jacobi_rounds);
 int xtbamd_batch_get(xtbamd_batch b, int m, xtbamd_result* out);
 void xtbamd_batch_free(xtbamd_batch b);
 /* device / runtime information: number of visible GPUs */

@@ -103,3 +103,57 @@ def test_mixed_batch_vs_oracle():
     for m, (z, x) in enumerate(mols):
         _cmp(b.result(m), po.singlepoint(z, x), f"mix{m}(n={len(z)})")
     b.close()
+
+
+def test_c_api_program(tmp_path):
+    """The xtb.h conformance program (propyne golden values, error log,
+    restart, unsupported calls) against libxtb_amd.so."""
+    import os
+    import subprocess
+    _engine()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "capi_gfn2"
+    subprocess.run(["gcc", "-O1", "-std=c11", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "c_api", "capi_gfn2.c"), "-L", os.path.join(root, "xtb_amd"),
+                    "-lxtb_amd", "-Wl,-rpath," + os.path.join(root, "xtb_amd"), "-lm", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+
+
+def test_restart_guess_reproduces_energy():
+    """A second single point seeded with the converged charges and multipoles
+    (the reference's results-object reuse) lands on the same energy."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    z, x = w.water_cluster(8, 3)
+    b = xa.Batch([(z, x, 0.0, 0)]).run()
+    r1 = b.result(0)
+    b.close()
+    b2 = xa.Batch([(z, x, 0.0, 0)])
+    b2.set_guess(0, r1["qsh"], r1["dipm"], r1["qp"])
+    b2.run()
+    r2 = b2.result(0)
+    b2.close()
+    assert r2["iterations"] < r1["iterations"]
+    assert abs(r2["energy"] - r1["energy"]) < 1e-7
+    assert np.max(np.abs(r2["gradient"] - r1["gradient"])) < 1e-5
+
+
+def test_rerun_same_batch_repeats_full_single_point():
+    """Running a batch again restarts every SCC from the initial guess:
+    bitwise identical results and iteration counts."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    mols = w.mixed_batch(6, seed=11, lo=10, hi=80)
+    b = xa.Batch([(z, x, 0.0, 0) for z, x in mols])
+    b.run()
+    first = b.results()
+    b.run()
+    second = b.results()
+    assert b.counters()[0] == max(r["iterations"] for r in second)
+    b.close()
+    for r1, r2 in zip(first, second):
+        assert r1["iterations"] == r2["iterations"] > 1
+        assert r1["energy"] == r2["energy"]
+        assert np.array_equal(r1["gradient"], r2["gradient"])

@@ -86,7 +86,9 @@ struct JacSet {
   int nsmall = 0, npairs = 0, nta = 0, ntw = 0, max_rounds = 0;
   int* d_small = nullptr;
   int2* d_pairs = nullptr;
-  int4 *d_ta = nullptr, *d_tw = nullptr;
+  int4 *d_ta = nullptr, *d_tw = nullptr, *d_to = nullptr;
+  int nto = 0;
+  unsigned long long* offmax = nullptr;
   double* qbuf = nullptr;
   int* rot = nullptr;
   int* state = nullptr;
@@ -153,27 +155,41 @@ class Batch {
   // other matrices
   double *J = nullptr, *dispmat = nullptr, *gab3 = nullptr, *gab5 = nullptr;
   double *c6 = nullptr, *dc6 = nullptr, *c60 = nullptr, *dc60 = nullptr;
-  double *E = nullptr, *WE = nullptr, *evE = nullptr, *xE = nullptr;
+  double *E = nullptr, *evE = nullptr, *xE = nullptr;
   // Broyden
   double *q_in = nullptr, *dq = nullptr, *qlast = nullptr, *dqlast = nullptr;
   double *bdf = nullptr, *bu = nullptr, *ba = nullptr, *bomega = nullptr;
   double *yE = nullptr, *erep = nullptr, *wbo = nullptr;
   // Jacobi bookkeeping
-  JacSet jac_n, jac_e;
+  JacSet jac_n;
   int* jac_nactive = nullptr;
   int* perm = nullptr;
   int* eeq_failed = nullptr;
+  double* ns_c = nullptr;
+  unsigned long long* ns_res = nullptr;
+  int *ns_stage = nullptr, *ns_act = nullptr, *ns_failed = nullptr;
   int* h_pinned = nullptr;
   long long jacobi_rounds_ = 0;
+  long long oa_steps_ = 0;
   const Element* elements_dev = nullptr;
   int ntiles = 0;
   int4* d_tiles = nullptr;
-  void jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active);
+  void jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w = false);
+  // refine W (init I) towards the eigenvectors of A; leaves A <- W^T A W
+  void refine(double* Am, double* Wm);
+  int oa_max_steps = 6;
+  double* Sm = nullptr;
+  double* G = nullptr;
+  int* oa_act = nullptr;
+  unsigned long long* oa_stat = nullptr;
   void gemm(int kind, const double* a, const double* b, double* c, const double* scale, const int* perm,
-            bool check_active);
+            bool check_active, double alpha = 1.0, const int* mask = nullptr);
+  void orthogonalizer();   // C = S^(-1/2) by Newton-Schulz iterations
+  void eeq_solve();        // EEQ charges by Gaussian elimination
 
   std::vector<void*> allocs_;
   std::vector<int> h_sh_at_;
+  std::vector<MolState> h_state0_;   // initial SCC state, restored by every setup()
   std::vector<std::vector<double>> guess_qsh_, guess_dipm_, guess_qp_;
   bool setup_done_ = false;
   int iterations_ = 0;

@@ -42,45 +42,96 @@ __global__ void k_copy(const double* a, double* b, long long n) {
   if (i < n) b[i] = a[i];
 }
 
-// EEQ charges from the eigen-decomposition of the constrained system, then the
-// shell charges (iniqshell: src/scc_core.f90:1475-1506); SAD fallback
-// (src/xtb/calculator.f90:479-484)
-__global__ void __launch_bounds__(256) k_eeq_solve(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
-                                                   const int* __restrict__ at, const int* __restrict__ at_sh0,
-                                                   const double* __restrict__ WE, const double* __restrict__ Ed,
-                                                   const double* __restrict__ xE, double* yE, double* q, double* qsh,
-                                                   const double* __restrict__ zsh, int use_eeq, int* failed) {
+// EEQ charges: Gaussian elimination with partial pivoting of the constrained
+// (n+1) x (n+1) system E y = x in place (the reference solves it with dsysv,
+// src/eeq_model.f90:678-1015), then the shell charges (iniqshell:
+// src/scc_core.f90:1475-1506); uniform charges if the solve fails
+// (src/xtb/calculator.f90:479-484).  One workgroup per molecule.
+__global__ void __launch_bounds__(256) k_eeq_gesv(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
+                                                  const int* __restrict__ at, const int* __restrict__ at_sh0, double* E,
+                                                  double* xE, double* yE, double* q, double* qsh,
+                                                  const double* __restrict__ zsh, int* failed) {
   __shared__ double sh[4];
+  __shared__ double shv[4];
+  __shared__ int shi[4];
+  __shared__ int piv;
   const MolDesc md = desc[blockIdx.x];
   const int ne = md.ne, n = md.n;
-  const long long eoff = md.a0 + blockIdx.x;
-  const double* W = WE + md.e0;
-  double* y = yE + eoff;
-  bool ok = use_eeq != 0;
-  if (ok) {
-    for (int k = threadIdx.x; k < ne; k += blockDim.x) {
-      double s = 0.0;
-      for (int i = 0; i < ne; ++i) s += W[(long long)i * ne + k] * xE[eoff + i];
-      const double lam = Ed[md.e0 + (long long)k * ne + k];  // eigenvalue on the diagonal
-      y[k] = s / lam;
+  double* M = E + md.e0;
+  double* b = xE + md.a0 + blockIdx.x;
+  double* y = yE + md.a0 + blockIdx.x;
+  const int tid = threadIdx.x;
+  bool ok = true;
+  for (int k = 0; k < ne; ++k) {
+    // pivot: largest |M[i][k]|, i >= k, smallest index on ties
+    double best = -1.0;
+    int bi = k;
+    for (int i = k + tid; i < ne; i += blockDim.x) {
+      const double v = fabs(M[(long long)i * ne + k]);
+      if (v > best) { best = v; bi = i; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if ((tid & 63) == 0) { shv[tid >> 6] = best; shi[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      double bv = shv[0];
+      int bx = shi[0];
+      for (int w = 1; w < 4; ++w)
+        if (shv[w] > bv || (shv[w] == bv && shi[w] < bx)) { bv = shv[w]; bx = shi[w]; }
+      piv = (bv > 0.0) ? bx : -1;
     }
     __syncthreads();
-    double tot = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int p = piv;
+    if (p < 0) { ok = false; break; }
+    if (p != k) {
+      for (int j = tid; j < ne; j += blockDim.x) {
+        const double t = M[(long long)k * ne + j];
+        M[(long long)k * ne + j] = M[(long long)p * ne + j];
+        M[(long long)p * ne + j] = t;
+      }
+      if (tid == 0) { const double t = b[k]; b[k] = b[p]; b[p] = t; }
+      __syncthreads();
+    }
+    // multipliers l_ik = M_ik / M_kk stored below the diagonal, then the
+    // rank-1 update of the trailing block and of the right-hand side
+    const double dkk = M[(long long)k * ne + k];
+    for (int i = k + 1 + tid; i < ne; i += blockDim.x) M[(long long)i * ne + k] /= dkk;
+    __syncthreads();
+    const int m = ne - k - 1;
+    for (long long e = tid; e < (long long)m * m; e += blockDim.x) {
+      const int i = k + 1 + (int)(e / m), j = k + 1 + (int)(e % m);
+      M[(long long)i * ne + j] -= M[(long long)i * ne + k] * M[(long long)k * ne + j];
+    }
+    for (int i = k + 1 + tid; i < ne; i += blockDim.x) b[i] -= M[(long long)i * ne + k] * b[k];
+    __syncthreads();
+  }
+  if (ok) {
+    for (int k = ne - 1; k >= 0; --k) {
       double s = 0.0;
-      for (int k = 0; k < ne; ++k) s += W[(long long)i * ne + k] * y[k];
-      q[md.a0 + i] = s;
-      tot += s;
+      for (int j = k + 1 + tid; j < ne; j += blockDim.x) s += M[(long long)k * ne + j] * y[j];
+      s = block_sum<256>(s, sh);
+      if (tid == 0) y[k] = (b[k] - s) / M[(long long)k * ne + k];
+      __syncthreads();
+    }
+    double tot = 0.0;
+    for (int i = tid; i < n; i += blockDim.x) {
+      q[md.a0 + i] = y[i];
+      tot += y[i];
     }
     tot = block_sum<256>(tot, sh);
     if (!(fabs(tot - md.charge) <= 1.e-6)) ok = false;
   }
   if (!ok) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) q[md.a0 + i] = md.charge / n;
-    if (threadIdx.x == 0 && use_eeq) failed[blockIdx.x] = 1;
+    for (int i = tid; i < n; i += blockDim.x) q[md.a0 + i] = md.charge / n;
+    if (tid == 0) failed[blockIdx.x] = 1;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = tid; i < n; i += blockDim.x) {
     const Element& e = el[at[md.a0 + i]];
     for (int s = 0; s < e.nsh; ++s) {
       const int k = md.s0 + at_sh0[md.a0 + i] + s;
@@ -89,16 +140,118 @@ __global__ void __launch_bounds__(256) k_eeq_solve(const MolDesc* __restrict__ d
   }
 }
 
-// canonical orthogonaliser of S: X[:, k] = U_S[:, k] / sqrt(s_k)
-__global__ void k_ortho(const MolDesc* __restrict__ desc, const double* __restrict__ Asd, const double* __restrict__ W,
-                        double* C) {
+// SAD guess: zero atomic charges except the total charge spread evenly
+__global__ void k_sad_guess(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
+                            const int* __restrict__ at, const int* __restrict__ at_sh0, double* q, double* qsh,
+                            const double* __restrict__ zsh) {
+  const MolDesc md = desc[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= md.n) return;
+  const double qi = md.charge / md.n;
+  q[md.a0 + i] = qi;
+  const Element& e = el[at[md.a0 + i]];
+  for (int s = 0; s < e.nsh; ++s) {
+    const int k = md.s0 + at_sh0[md.a0 + i] + s;
+    qsh[k] = (zsh[k] / e.zval) * qi;
+  }
+}
+
+// --- Newton-Schulz iteration for S^(-1/2) (coupled form, Higham, "Functions
+// of Matrices", 2008, eq. 6.35): Y0 = S/c, Z0 = I,
+// T = (3I - Z Y)/2, Y <- Y T, Z <- T Z; Z -> (S/c)^(-1/2).
+// c = max row sum |S_ij| bounds the spectrum of S from above.
+__global__ void __launch_bounds__(256) k_ns_bound(const MolDesc* __restrict__ desc, const double* __restrict__ S,
+                                                  double* cbound) {
+  __shared__ double sh[4];
+  const MolDesc md = desc[blockIdx.x];
+  const int N = md.nao;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    double r = 0.0;
+    for (int j = 0; j < N; ++j) r += fabs(S[md.m0 + (long long)i * N + j]);
+    m = fmax(m, r);
+  }
+  m = block_max<256>(m, sh);
+  if (threadIdx.x == 0) cbound[blockIdx.x] = m;
+}
+
+__global__ void k_ns_init(const MolDesc* __restrict__ desc, const double* __restrict__ S,
+                          const double* __restrict__ cbound, double* Y, double* Z) {
   const MolDesc md = desc[blockIdx.y];
   const long long N = md.nao;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N * N) return;
-  const int k = (int)(t % N);
-  const double s = Asd[md.m0 + (long long)k * N + k];
-  C[md.m0 + t] = W[md.m0 + t] / sqrt(s);
+  Y[md.m0 + t] = S[md.m0 + t] / cbound[blockIdx.y];
+  Z[md.m0 + t] = (t / N == t % N) ? 1.0 : 0.0;
+}
+
+// T = (3I - P)/2 with P = Z Y, and max |I - P| per molecule
+__global__ void __launch_bounds__(256) k_ns_form(const MolDesc* __restrict__ desc, const int* __restrict__ act,
+                                                 double* P, unsigned long long* res) {
+  __shared__ double sh[4];
+  const MolDesc md = desc[blockIdx.y];
+  if (!act[blockIdx.y]) return;
+  const long long N = md.nao;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double r = 0.0;
+  if (t < N * N) {
+    const double d = (t / N == t % N) ? 1.0 : 0.0;
+    const double p = P[md.m0 + t];
+    r = fabs(d - p);
+    P[md.m0 + t] = 0.5 * (3.0 * d - p);
+  }
+  r = block_max<256>(r, sh);
+  if (threadIdx.x == 0 && r > 0.0) atomicMax(&res[blockIdx.y], (unsigned long long)__double_as_longlong(r));
+}
+
+// per-molecule convergence: residual < 1e-10, then one more update (quadratic
+// convergence takes it to rounding level); act = 0 once done
+__global__ void __launch_bounds__(256) k_ns_mark(int nmol, int* stage, int* act, unsigned long long* res, int iter,
+                                                 int maxiter, int* failed, int* nactive) {
+  __shared__ double sh[4];
+  double c = 0.0;
+  for (int m = threadIdx.x; m < nmol; m += blockDim.x) {
+    const double r = __longlong_as_double((long long)res[m]);
+    res[m] = 0ull;
+    if (stage[m] == 1) stage[m] = 2;
+    else if (stage[m] == 0 && r < 1e-10) stage[m] = 1;
+    if (stage[m] == 0 && iter >= maxiter) { stage[m] = 2; failed[m] = 1; }
+    act[m] = stage[m] < 2 ? 1 : 0;
+    c += act[m];
+  }
+  c = block_sum<256>(c, sh);
+  if (threadIdx.x == 0) nactive[0] = (int)c;
+}
+
+__global__ void k_ident_active(const MolDesc* __restrict__ desc, const MolState* __restrict__ st, double* W) {
+  const MolDesc md = desc[blockIdx.y];
+  if (!st[blockIdx.y].active) return;
+  const long long N = md.nao;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < N * N) W[md.m0 + t] = (t / N == t % N) ? 1.0 : 0.0;
+}
+
+__global__ void k_copy_state_active(const MolState* __restrict__ st, int nmol, int* act) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < nmol) act[m] = st[m].active;
+}
+
+__global__ void k_copy_mask(const MolDesc* __restrict__ desc, const int* __restrict__ act, const double* __restrict__ a,
+                            double* b) {
+  const MolDesc md = desc[blockIdx.y];
+  if (!act[blockIdx.y]) return;
+  const long long n2 = (long long)md.nao * md.nao;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n2) b[md.m0 + t] = a[md.m0 + t];
+}
+
+__global__ void k_ns_final(const MolDesc* __restrict__ desc, const double* __restrict__ Z,
+                           const double* __restrict__ cbound, double* C) {
+  const MolDesc md = desc[blockIdx.y];
+  const long long N = md.nao;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * N) return;
+  C[md.m0 + t] = Z[md.m0 + t] / sqrt(cbound[blockIdx.y]);
 }
 
 // total energy, gradient norm and dipole (src/scf_module.F90:882-913,
@@ -303,7 +456,7 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
   J = alloc<double>(tot_jj); dispmat = alloc<double>(tot_xx);
   gab3 = alloc<double>(tot_pp); gab5 = alloc<double>(tot_pp);
   c6 = alloc<double>(tot_pp); dc6 = alloc<double>(tot_pp); c60 = alloc<double>(tot_pp); dc60 = alloc<double>(tot_pp);
-  E = alloc<double>(tot_ee); WE = alloc<double>(tot_ee);
+  E = alloc<double>(tot_ee);
   evE = alloc<double>(tot_n + nmol); xE = alloc<double>(tot_n + nmol);
   q_in = alloc<double>(tot_v); dq = alloc<double>(tot_v); qlast = alloc<double>(tot_v); dqlast = alloc<double>(tot_v);
   bdf = alloc<double>(tot_h); bu = alloc<double>(tot_h); ba = alloc<double>(tot_g);
@@ -333,7 +486,7 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
   h_sh_at_ = h_sh_at;
   HIPCHECK(hipMemcpy(xyz, h_xyz.data(), h_xyz.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(d_desc, desc.data(), nmol * sizeof(MolDesc), hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(d_state, h_state.data(), nmol * sizeof(MolState), hipMemcpyHostToDevice));
+  h_state0_ = h_state;
   Element* d_el = alloc<Element>(XTB_MAX_Z + 1);
   HIPCHECK(hipMemcpy(d_el, el, (XTB_MAX_Z + 1) * sizeof(Element), hipMemcpyHostToDevice));
   elements_dev = d_el;
@@ -369,7 +522,7 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
     HIPCHECK(hipMemcpy(js.d_probs, js.probs.data(), nmol * sizeof(JacobiProb), hipMemcpyHostToDevice));
     std::vector<int> small;
     std::vector<int2> pairs;
-    std::vector<int4> ta, tw;
+    std::vector<int4> ta, tw, to;
     js.max_rounds = 0;
     for (int m = 0; m < nmol; ++m) {
       const JacobiProb& p = js.probs[m];
@@ -382,19 +535,25 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
       const int nrb = (p.dim + 63) / 64;
       for (int rb = 0; rb < nrb; ++rb)
         for (int l = 0; l < np; ++l) tw.push_back(make_int4(m, rb, l, 0));
+      for (int a = 0; a < nrb; ++a)
+        for (int b = a; b < nrb; ++b) to.push_back(make_int4(m, a, b, 0));
     }
     js.nsmall = (int)small.size();
     js.npairs = (int)pairs.size();
     js.nta = (int)ta.size();
     js.ntw = (int)tw.size();
+    js.nto = (int)to.size();
     js.d_small = alloc<int>(small.size());
     js.d_pairs = alloc<int2>(pairs.size());
     js.d_ta = alloc<int4>(ta.size());
     js.d_tw = alloc<int4>(tw.size());
+    js.d_to = alloc<int4>(to.size());
     if (!small.empty()) HIPCHECK(hipMemcpy(js.d_small, small.data(), small.size() * sizeof(int), hipMemcpyHostToDevice));
     if (!pairs.empty()) HIPCHECK(hipMemcpy(js.d_pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
     if (!ta.empty()) HIPCHECK(hipMemcpy(js.d_ta, ta.data(), ta.size() * sizeof(int4), hipMemcpyHostToDevice));
     if (!tw.empty()) HIPCHECK(hipMemcpy(js.d_tw, tw.data(), tw.size() * sizeof(int4), hipMemcpyHostToDevice));
+    if (!to.empty()) HIPCHECK(hipMemcpy(js.d_to, to.data(), to.size() * sizeof(int4), hipMemcpyHostToDevice));
+    js.offmax = alloc<unsigned long long>(nmol);
     js.qbuf = alloc<double>(js.qsize);
     js.rot = alloc<int>(js.qsize / 4096 + 1);
     js.state = alloc<int>(nmol * kJacState);
@@ -402,12 +561,20 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
     js.perm = nullptr;
   };
   make_set(false, jac_n);
-  make_set(true, jac_e);
   jac_nactive = alloc<int>(1);
   yE = alloc<double>(tot_n + nmol);
   erep = alloc<double>(tot_n);
   perm = alloc<int>(tot_nao + tot_n + nmol);
   eeq_failed = alloc<int>(nmol);
+  ns_c = alloc<double>(nmol);
+  Sm = alloc<double>(tot_mm);
+  G = alloc<double>(tot_mm);
+  oa_act = alloc<int>(nmol);
+  oa_stat = alloc<unsigned long long>(4LL * nmol);
+  ns_res = alloc<unsigned long long>(nmol);
+  ns_stage = alloc<int>(nmol);
+  ns_act = alloc<int>(nmol);
+  ns_failed = alloc<int>(nmol);
   wbo = alloc<double>(tot_pp);
   (void)evE;
   HIPCHECK(hipHostMalloc((void**)&h_pinned, 64 * sizeof(int), hipHostMallocDefault));
@@ -419,36 +586,71 @@ Batch::~Batch() {
 }
 
 // ---------------------------------------------------------------------------
-void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active) {
+void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w) {
   const int nb = nmol;
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, stream, jac_nactive, 1LL, 0);
   hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
-                     check_active ? (const MolState*)d_state : (const MolState*)nullptr, jac_nactive);
+                     check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0);
   if (js.nsmall)
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
-                       (const double*)js.scale, js.state);
+                       (const double*)js.scale, js.state, keep_w ? 1 : 0);
   if (js.npairs) {
-    const int maxr = kMaxSweeps * js.max_rounds + 2;
     int r = 0;
-    while (r < maxr) {
-      const int chunk = js.max_rounds;
-      for (int c = 0; c < chunk && r < maxr; ++c, ++r) {
-        hipLaunchKernelGGL(k_jac_pairs, dim3(js.npairs), dim3(256), 0, stream, js.d_probs, js.d_pairs, r,
-                           (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, js.state, jac_nactive);
-        hipLaunchKernelGGL(k_jac_apply_a, dim3(js.nta), dim3(256), 0, stream, js.d_probs, js.d_ta, r, Am,
-                           (const double*)js.qbuf, (const int*)js.rot, js.state);
-        hipLaunchKernelGGL(k_jac_apply_w, dim3(js.ntw), dim3(256), 0, stream, js.d_probs, js.d_tw, r, Wm,
-                           (const double*)js.qbuf, (const int*)js.rot, js.state);
-      }
+    for (int sweep = 0; sweep <= kMaxSweeps; ++sweep) {
+      hipLaunchKernelGGL(k_jac_offmax, dim3(js.nto), dim3(256), 0, stream, js.d_probs, js.d_to, (const double*)Am,
+                         (const int*)js.state, js.offmax);
+      hipLaunchKernelGGL(k_jac_mark, dim3(1), dim3(256), 0, stream, js.d_probs, nb, (const double*)js.scale,
+                         js.state, js.offmax, sweep, jac_nactive);
       HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
       if (h_pinned[0] <= 0) break;
-      jacobi_rounds_ += chunk;
+      for (int c = 0; c < js.max_rounds; ++c, ++r) {
+        hipLaunchKernelGGL(k_jac_pairs, dim3(js.npairs), dim3(256), 0, stream, js.d_probs, js.d_pairs, r,
+                           (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state);
+        hipLaunchKernelGGL(k_jac_apply_a, dim3(js.nta), dim3(256), 0, stream, js.d_probs, js.d_ta, r, Am,
+                           (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+        hipLaunchKernelGGL(k_jac_apply_w, dim3(js.ntw), dim3(256), 0, stream, js.d_probs, js.d_tw, r, Wm,
+                           (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+      }
+      jacobi_rounds_ += js.max_rounds;
     }
   }
   if (evout)
     hipLaunchKernelGGL(k_jac_sort, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm,
                        check_active ? (const MolState*)d_state : (const MolState*)nullptr);
+}
+
+void Batch::refine(double* Am, double* Wm) {
+  const dim3 gmm(grid1((long long)max_nao * max_nao, 256), nmol);
+  // refining molecules = SCC-active ones
+  hipLaunchKernelGGL(k_copy_state_active, dim3(grid1(nmol, 64)), dim3(64), 0, stream, (const MolState*)d_state, nmol,
+                     oa_act);
+  hipLaunchKernelGGL(k_fill, dim3(grid1(4LL * nmol, 256)), dim3(256), 0, stream, (double*)oa_stat, 4LL * nmol, 0.0);
+  hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_desc, (const int*)oa_act, (const double*)Am, Sm);
+  for (int step = 0; step <= oa_max_steps; ++step) {
+    if (step > 0) {
+      gemm(1, Wm, Wm, G, nullptr, nullptr, false, 1.0, oa_act);   // X^T X
+      gemm(0, Am, Wm, T, nullptr, nullptr, false, 1.0, oa_act);   // A X
+      gemm(1, Wm, T, Sm, nullptr, nullptr, false, 1.0, oa_act);   // X^T A X
+    }
+    hipLaunchKernelGGL(k_oa_e, gmm, dim3(256), 0, stream, d_desc, (const int*)oa_act, (const double*)Sm,
+                       step > 0 ? (const double*)G : (const double*)nullptr, H, oa_stat);
+    hipLaunchKernelGGL(k_oa_mark, dim3(1), dim3(256), 0, stream, nmol, (const MolState*)d_state, oa_act, oa_stat, step,
+                       oa_max_steps, jac_nactive);
+    HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (h_pinned[0] <= 0) break;
+    if (step == 0) {
+      hipLaunchKernelGGL(k_oa_update, gmm, dim3(256), 0, stream, d_desc, (const int*)oa_act, (const double*)H, Wm, 1);
+    } else {
+      gemm(0, Wm, H, P, nullptr, nullptr, false, 1.0, oa_act);    // X E
+      hipLaunchKernelGGL(k_oa_update, gmm, dim3(256), 0, stream, d_desc, (const int*)oa_act, (const double*)P, Wm, 0);
+    }
+    oa_steps_ += 1;
+  }
+  // A <- X^T A X for the refined molecules (the Jacobi sweeps continue from there)
+  hipLaunchKernelGGL(k_copy_state_active, dim3(grid1(nmol, 64)), dim3(64), 0, stream, (const MolState*)d_state, nmol,
+                     oa_act);
+  hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_desc, (const int*)oa_act, (const double*)Sm, Am);
 }
 
 void Batch::set_guess(int m, const double* qsh_in, const double* dipm_in, const double* qp_in) {
@@ -465,30 +667,68 @@ void Batch::set_guess(int m, const double* qsh_in, const double* dipm_in, const 
 }
 
 void Batch::gemm(int kind, const double* a, const double* b, double* c, const double* scale_, const int* perm_,
-                 bool check_active) {
-  const MolState* st = check_active ? d_state : nullptr;
+                 bool check_active, double alpha, const int* mask) {
+  // SCC-active molecules: the 'active' field of MolState, read with its stride
+  const int* m = mask;
+  int ms = 1;
+  if (check_active) {
+    m = &d_state[0].active;
+    ms = (int)(sizeof(MolState) / sizeof(int));
+  }
   switch (kind) {
     case 0:
       hipLaunchKernelGGL((k_gemm<false, false, false, false>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
-                         c, scale_, perm_, 1.0, st);
+                         c, scale_, perm_, alpha, m, ms);
       break;
     case 1:
       hipLaunchKernelGGL((k_gemm<true, false, false, false>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
-                         c, scale_, perm_, 1.0, st);
+                         c, scale_, perm_, alpha, m, ms);
       break;
     case 2:
       hipLaunchKernelGGL((k_gemm<false, true, true, false>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
-                         c, scale_, perm_, 1.0, st);
+                         c, scale_, perm_, alpha, m, ms);
       break;
     default:
       hipLaunchKernelGGL((k_gemm<false, false, false, true>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
-                         c, scale_, perm_, 1.0, st);
+                         c, scale_, perm_, alpha, m, ms);
       break;
   }
 }
 
+void Batch::orthogonalizer() {
+  const dim3 gmm(grid1((long long)max_nao * max_nao, 256), nmol);
+  hipLaunchKernelGGL(k_ns_bound, dim3(nmol), dim3(256), 0, stream, d_desc, (const double*)S, ns_c);
+  hipLaunchKernelGGL(k_ns_init, gmm, dim3(256), 0, stream, d_desc, (const double*)S, (const double*)ns_c, A, W);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nmol, 64)), dim3(64), 0, stream, ns_stage, (long long)nmol, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nmol, 64)), dim3(64), 0, stream, ns_act, (long long)nmol, 1);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nmol, 64)), dim3(64), 0, stream, ns_failed, (long long)nmol, 0);
+  hipLaunchKernelGGL(k_fill, dim3(grid1(2 * nmol, 64)), dim3(64), 0, stream, (double*)ns_res, (long long)nmol, 0.0);
+  const int maxit = 60;
+  for (int it = 0; it <= maxit; ++it) {
+    // Y in A, Z in W
+    gemm(0, W, A, P, nullptr, nullptr, false, 1.0, ns_act);           // P = Z Y
+    hipLaunchKernelGGL(k_ns_form, gmm, dim3(256), 0, stream, d_desc, (const int*)ns_act, P, ns_res);
+    hipLaunchKernelGGL(k_ns_mark, dim3(1), dim3(256), 0, stream, nmol, ns_stage, ns_act, ns_res, it, maxit, ns_failed,
+                       jac_nactive);
+    HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+    if (h_pinned[0] <= 0) break;
+    gemm(0, A, P, H, nullptr, nullptr, false, 1.0, ns_act);           // Y T
+    gemm(0, P, W, T, nullptr, nullptr, false, 1.0, ns_act);           // T Z
+    hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_desc, (const int*)ns_act, (const double*)H, A);
+    hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_desc, (const int*)ns_act, (const double*)T, W);
+  }
+  hipLaunchKernelGGL(k_ns_final, gmm, dim3(256), 0, stream, d_desc, (const double*)W, (const double*)ns_c, C);
+}
+
 void Batch::setup() {
   const Element* el = elements_dev;
+  // every single point starts from the initial SCC state (iteration 0, active)
+  HIPCHECK(hipMemcpyAsync(d_state, h_state0_.data(), nmol * sizeof(MolState), hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  iterations_ = 0;
+  jacobi_rounds_ = 0;
+  oa_steps_ = 0;
   const dim3 gat(max_n, nmol);
   hipLaunchKernelGGL(k_cn, gat, dim3(64), 0, stream, d_desc, el, at, xyz, cn, cn_cov, cn_d4, cn_eeq);
   hipLaunchKernelGGL(k_shell_setup, dim3(grid1(max_nsh, 64), nmol), dim3(64), 0, stream, d_desc, el, at, sh_at, sh_m,
@@ -512,14 +752,15 @@ void Batch::setup() {
     const long long ne = max_n + 1;
     hipLaunchKernelGGL(k_eeq_matrix, dim3(grid1(ne * ne, 256), nmol), dim3(256), 0, stream, d_desc, el, at, xyz,
                        cn_eeq, E, xE);
-    jacobi(jac_e, E, WE, nullptr, false);
+    hipLaunchKernelGGL(k_eeq_gesv, dim3(nmol), dim3(256), 0, stream, d_desc, el, at, at_sh0, E, xE, yE, q, qsh,
+                       (const double*)zsh, eeq_failed);
+  } else {
+    hipLaunchKernelGGL(k_sad_guess, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, at_sh0, q,
+                       qsh, (const double*)zsh);
   }
-  hipLaunchKernelGGL(k_eeq_solve, dim3(nmol), dim3(256), 0, stream, d_desc, el, at, at_sh0, (const double*)WE,
-                     (const double*)E, (const double*)xE, yE, q, qsh, zsh,
-                     opt.guess == 0 ? 1 : 0, eeq_failed);
   hipLaunchKernelGGL(k_fill, dim3(grid1(3 * tot_n, 256)), dim3(256), 0, stream, dipm, 3LL * tot_n, 0.0);
   hipLaunchKernelGGL(k_fill, dim3(grid1(6 * tot_n, 256)), dim3(256), 0, stream, qp, 6LL * tot_n, 0.0);
-  // restart guesses replace the EEQ charges (src/api/interface.f90:83-101, src/scf_module.F90:292)
+  // restart guesses replace the initial charges (src/api/interface.f90:83-101, src/scf_module.F90:292)
   for (int m = 0; m < nmol; ++m) {
     if (guess_qsh_.empty() || guess_qsh_[m].empty()) continue;
     const MolDesc& d = desc[m];
@@ -533,11 +774,8 @@ void Batch::setup() {
                             stream));
     HIPCHECK(hipStreamSynchronize(stream));
   }
-  // orthogonaliser of S -> initial orbital basis C
-  hipLaunchKernelGGL(k_copy, dim3(grid1(tot_mm, 256)), dim3(256), 0, stream, (const double*)S, A, tot_mm);
-  jacobi(jac_n, A, W, nullptr, false);
-  hipLaunchKernelGGL(k_ortho, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream, d_desc,
-                     (const double*)A, (const double*)W, C);
+  // S-orthonormal initial orbital basis C = S^(-1/2)
+  orthogonalizer();
   HIPCHECK(hipGetLastError());
   setup_done_ = true;
 }
@@ -560,7 +798,15 @@ bool Batch::scc_iteration() {
   // generalized eigenproblem in the current orbital basis C (S-orthonormal)
   gemm(0, H, C, T, nullptr, nullptr, true);   // T = H C
   gemm(1, C, T, A, nullptr, nullptr, true);   // A = C^T H C
-  jacobi(jac_n, A, W, emo, true);
+  // C holds the previous eigenvectors from the second iteration on: A is
+  // nearly diagonal, refine W = I towards its eigenvectors with GEMMs first
+  const bool warm = iterations_ > 0 && oa_max_steps > 0;
+  if (warm) {
+    hipLaunchKernelGGL(k_ident_active, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream,
+                       d_desc, (const MolState*)d_state, W);
+    refine(A, W);
+  }
+  jacobi(jac_n, A, W, emo, true, warm);
   gemm(3, C, W, T, nullptr, perm, true);      // T = C W(:, perm)
   hipLaunchKernelGGL(k_copy_active, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream,
                      d_desc, (const MolState*)d_state, (const double*)T, C);
@@ -765,6 +1011,13 @@ int xtbamd_batch_gradient(xtbamd_batch b) {
 
 int xtbamd_batch_run(xtbamd_batch b) {
   try { b->batch->run(); return 0; } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_counters(xtbamd_batch b, int* scc_steps, long long* jacobi_rounds) {
+  if (!b) return -1;
+  if (scc_steps) *scc_steps = b->batch->iterations_;
+  if (jacobi_rounds) *jacobi_rounds = b->batch->jacobi_rounds_;
+  return 0;
 }
 
 int xtbamd_batch_size(xtbamd_batch b) { return b ? b->batch->size() : 0; }

@@ -14,17 +14,18 @@ namespace xtbamd {
 
 constexpr int kTile = 64, kBK = 16;
 
-// C = op(A) * op(B) [* diag(scale) on the k index] for square nao x nao
-// matrices; optional column permutation of B (B(:, perm[j])).
+// C = alpha op(A) * op(B) [* diag(scale) on the k index] for square nao x nao
+// matrices; optional column permutation of B (B(:, perm[j])).  Molecules with
+// mask[m * mstride] == 0 are skipped (mask may be null).
 template <bool TA, bool TB, bool SCALE, bool PERM>
 __global__ void __launch_bounds__(256) k_gemm(const int4* __restrict__ tiles, const MolDesc* __restrict__ desc,
                                               const double* __restrict__ A, const double* __restrict__ B, double* C,
                                               const double* __restrict__ scale, const int* __restrict__ perm,
-                                              double alpha, const MolState* __restrict__ mstate) {
+                                              double alpha, const int* __restrict__ mask, int mstride) {
   __shared__ double As[kBK][kTile + 1];
   __shared__ double Bs[kBK][kTile + 1];
   const int4 tl = tiles[blockIdx.x];
-  if (mstate && !mstate[tl.x].active) return;
+  if (mask && !mask[(long long)tl.x * mstride]) return;
   const MolDesc md = desc[tl.x];
   const int N = md.nao;
   const double* Am = A + md.m0;
@@ -89,25 +90,44 @@ __global__ void __launch_bounds__(256) k_gemm(const int4* __restrict__ tiles, co
 
 // instantiations used by the engine
 template __global__ void k_gemm<false, false, false, false>(const int4*, const MolDesc*, const double*, const double*,
-                                                            double*, const double*, const int*, double, const MolState*);
+                                                            double*, const double*, const int*, double, const int*, int);
 template __global__ void k_gemm<true, false, false, false>(const int4*, const MolDesc*, const double*, const double*,
-                                                           double*, const double*, const int*, double, const MolState*);
+                                                           double*, const double*, const int*, double, const int*, int);
 template __global__ void k_gemm<false, true, true, false>(const int4*, const MolDesc*, const double*, const double*,
-                                                          double*, const double*, const int*, double, const MolState*);
+                                                          double*, const double*, const int*, double, const int*, int);
 template __global__ void k_gemm<false, false, false, true>(const int4*, const MolDesc*, const double*, const double*,
-                                                           double*, const double*, const int*, double, const MolState*);
+                                                           double*, const double*, const int*, double, const int*, int);
 
 // ---------------------------------------------------------------------------
 // Cyclic parallel Jacobi on a 64x64 symmetric matrix held in LDS.
-// a, q: 64 x 65 (padded rows).  Returns the number of rotations applied.
+// a, q: 64 x 65 (padded rows).  Each sweep starts with the largest
+// off-diagonal magnitude; the solve stops when it is <= tol or after maxsweep
+// sweeps.  Returns the number of rounds that applied a rotation.
 constexpr int kLd = 65;
 
-__device__ int jacobi64(double* a, double* q, double tol, int maxsweep, double* cs, int* pq, int* flags) {
+__device__ int jacobi64(double* a, double* q, double tol, int maxsweep, double* cs, int* pq, int* flags,
+                        double* red) {
   const int tid = threadIdx.x;
+  // this thread's 2x2 pair blocks (k1 <= k2) of the 32 x 32 pair grid: fixed for all rounds
+  int kb1[3], kb2[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int b = tid + 256 * j;
+    int k2 = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while (k2 * (k2 + 1) / 2 > b) --k2;
+    while ((k2 + 1) * (k2 + 2) / 2 <= b) ++k2;
+    kb2[j] = b < 528 ? k2 : -1;
+    kb1[j] = b - k2 * (k2 + 1) / 2;
+  }
   int total = 0;
   for (int sweep = 0; sweep < maxsweep; ++sweep) {
-    if (tid == 0) flags[1] = 0;
-    __syncthreads();
+    double m = 0.0;
+    for (int e = tid; e < 4096; e += blockDim.x) {
+      const int u = e >> 6, v = e & 63;
+      if (u < v) m = fmax(m, fabs(a[u * kLd + v]));
+    }
+    m = block_max<256>(m, red);
+    if (m <= tol) break;
     for (int r = 0; r < 63; ++r) {
       if (tid < 64) {
         bool rot = false;
@@ -134,19 +154,15 @@ __device__ int jacobi64(double* a, double* q, double tol, int maxsweep, double* 
           cs[3 * k + 2] = t;
         }
         const unsigned long long any = __ballot(rot);
-        if (tid == 0) {
-          flags[0] = any ? 1 : 0;
-          if (any) flags[1] += 1;
-        }
+        if (tid == 0) flags[0] = any ? 1 : 0;
       }
       __syncthreads();
       if (flags[0]) {
-        for (int b = tid; b < 528; b += blockDim.x) {
-          // triangular (k1 <= k2) index of the 32 x 32 pair blocks
-          int k2 = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
-          while (k2 * (k2 + 1) / 2 > b) --k2;
-          while ((k2 + 1) * (k2 + 2) / 2 <= b) ++k2;
-          const int k1 = b - k2 * (k2 + 1) / 2;
+        total += 1;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const int k1 = kb1[j], k2 = kb2[j];
+          if (k2 < 0) continue;
           const int p1 = pq[2 * k1], q1 = pq[2 * k1 + 1], p2 = pq[2 * k2], q2 = pq[2 * k2 + 1];
           const double c1 = cs[3 * k1], s1 = cs[3 * k1 + 1], c2 = cs[3 * k2], s2 = cs[3 * k2 + 1];
           if (k1 == k2) {
@@ -172,23 +188,26 @@ __device__ int jacobi64(double* a, double* q, double tol, int maxsweep, double* 
             a[q1 * kLd + q2] = nqs; a[q2 * kLd + q1] = nqs;
           }
         }
-        for (int e = tid; e < 64 * 32; e += blockDim.x) {
-          const int u = e >> 5, k = e & 31;
+        {
+          // Q <- Q J: thread owns pair k = tid & 31 and rows tid/32 + 8j (blockDim 256)
+          const int k = tid & 31;
           const double s = cs[3 * k + 1];
-          if (s == 0.0) continue;
-          const double c = cs[3 * k];
-          const int p = pq[2 * k], qq = pq[2 * k + 1];
-          const double up = q[u * kLd + p], uq = q[u * kLd + qq];
-          q[u * kLd + p] = c * up - s * uq;
-          q[u * kLd + qq] = s * up + c * uq;
+          if (s != 0.0) {
+            const double c = cs[3 * k];
+            const int p = pq[2 * k], qq = pq[2 * k + 1];
+            const int u0 = tid >> 5;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              const int u = u0 + 8 * j;
+              const double up = q[u * kLd + p], uq = q[u * kLd + qq];
+              q[u * kLd + p] = c * up - s * uq;
+              q[u * kLd + qq] = s * up + c * uq;
+            }
+          }
         }
       }
       __syncthreads();
     }
-    const int nrot = flags[1];
-    __syncthreads();
-    total += nrot;
-    if (nrot == 0) break;
   }
   return total;
 }
@@ -200,17 +219,21 @@ __device__ __forceinline__ void circle_pair(int nbp, int r, int k, int& bp, int&
   if (bp > bq) { const int t = bp; bp = bq; bq = t; }
 }
 
-// state per problem: [0] converged, [1] failed, [2..4] rotation counts of sweeps mod 3, [5] sweeps
+// state per problem: [0] done, [1] failed, [5] outer sweeps (block path) or
+// inner rounds with rotations (dim <= 64)
 constexpr int kJacState = 8;
 constexpr int kMaxSweeps = 40;
+// inner sweeps per pivot subproblem of the block path
+constexpr int kInnerSweeps = 3;
 
 // problems of dimension <= 64: one workgroup solves to convergence in LDS
 __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict__ probs, const int* __restrict__ list,
                                                    double* A, double* W, const double* __restrict__ scale_in,
-                                                   int* state) {
+                                                   int* state, int keep_w) {
   __shared__ double a[64 * kLd];
   __shared__ double q[64 * kLd];
   __shared__ double cs[96];
+  __shared__ double red[4];
   __shared__ int pq[64];
   __shared__ int flags[2];
   const int pidx = list[blockIdx.x];
@@ -226,11 +249,27 @@ __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict_
   }
   __syncthreads();
   const double tol = 1e-12 * scale_in[pidx];
-  const int nrot = jacobi64(a, q, tol, kMaxSweeps, cs, pq, flags);
+  const int nrot = jacobi64(a, q, tol, kMaxSweeps, cs, pq, flags, red);
+  __syncthreads();
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
     const int u = e / n, v = e % n;
     Ap[(long long)u * n + v] = a[u * kLd + v];
-    Wp[(long long)u * n + v] = q[u * kLd + v];
+    if (!keep_w) Wp[(long long)u * n + v] = q[u * kLd + v];
+  }
+  if (keep_w && nrot > 0) {
+    // W <- W Q, with the old W staged in LDS
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+      const int u = e >> 6, v = e & 63;
+      a[u * kLd + v] = (u < n && v < n) ? Wp[(long long)u * n + v] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+      const int u = e / n, v = e % n;
+      double acc = 0.0;
+      for (int k = 0; k < n; ++k) acc = fma(a[u * kLd + k], q[k * kLd + v], acc);
+      Wp[(long long)u * n + v] = acc;
+    }
   }
   if (threadIdx.x == 0) {
     state[pidx * kJacState + 0] = 1;
@@ -242,52 +281,22 @@ __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict_
 // global index of the u-th row (0..63) of the pivot subproblem (bp, bq)
 __device__ __forceinline__ int sub_index(int bp, int bq, int u) { return u < 32 ? 32 * bp + u : 32 * bq + (u - 32); }
 
-__device__ __forceinline__ bool jac_skip(const JacobiProb& pr, int* st, int r, int k, bool bookkeeping,
-                                         int* nactive) {
-  if (st[0]) return true;
-  const int s = r / pr.rounds, lr = r % pr.rounds;
-  if (s >= 1 && lr == 0 && st[2 + (s - 1) % 3] == 0) {
-    if (bookkeeping && k == 0) {
-      st[0] = 1;
-      st[5] = s;
-      atomicSub(nactive, 1);
-    }
-    return true;
-  }
-  if (s >= 1 && st[2 + (s - 1) % 3] == 0) return true;
-  if (s >= kMaxSweeps) {
-    if (bookkeeping && k == 0 && lr == 0) {
-      st[0] = 1;
-      st[1] = 1;
-      atomicSub(nactive, 1);
-    }
-    return true;
-  }
-  return false;
-}
-
 // one workgroup per (problem, pair): diagonalise the 64x64 pivot subproblem
+// (at most kInnerSweeps sweeps)
 __global__ void __launch_bounds__(256) k_jac_pairs(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
                                                    int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
-                                                   const double* __restrict__ scale_in, int* state, int* nactive) {
+                                                   const double* __restrict__ scale_in, const int* __restrict__ state) {
   __shared__ double a[64 * kLd];
   __shared__ double q[64 * kLd];
   __shared__ double cs[96];
+  __shared__ double red[4];
   __shared__ int pq[64];
   __shared__ int flags[2];
-  __shared__ int skip;
   const int2 it = list[blockIdx.x];
   const JacobiProb pr = probs[it.x];
-  int* st = state + it.x * kJacState;
-  if (threadIdx.x == 0) {
-    skip = jac_skip(pr, st, r, it.y, true, nactive) ? 1 : 0;
-    const int s = r / pr.rounds, lr = r % pr.rounds;
-    if (!skip && it.y == 0 && lr == 0) st[2 + (s + 1) % 3] = 0;
-  }
-  __syncthreads();
-  if (skip) return;
+  if (state[it.x * kJacState]) return;
   const int n = pr.dim;
-  const int lr = r % pr.rounds, s = r / pr.rounds;
+  const int lr = r % pr.rounds;
   int bp, bq;
   circle_pair(pr.nbp, lr, it.y, bp, bq);
   const double* Ap = A + pr.a_off;
@@ -299,27 +308,24 @@ __global__ void __launch_bounds__(256) k_jac_pairs(const JacobiProb* __restrict_
   }
   __syncthreads();
   const double tol = 1e-12 * scale_in[it.x];
-  const int nrot = jacobi64(a, q, tol, kMaxSweeps, cs, pq, flags);
+  const int nrot = jacobi64(a, q, tol, kInnerSweeps, cs, pq, flags, red);
+  __syncthreads();
   double* Qp = Qbuf + pr.q_off + (long long)it.y * 64 * 64;
   if (nrot > 0)
     for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) Qp[e] = q[(e >> 6) * kLd + (e & 63)];
-  if (threadIdx.x == 0) {
-    rotflag[(int)(pr.q_off / 4096) + it.y] = nrot > 0 ? 1 : 0;
-    if (nrot > 0) atomicAdd(&st[2 + s % 3], 1);
-  }
+  if (threadIdx.x == 0) rotflag[(int)(pr.q_off / 4096) + it.y] = nrot > 0 ? 1 : 0;
 }
 
 // A[I_k, I_l] <- Q_k^T A[I_k, I_l] Q_l for pair tiles k <= l
 __global__ void __launch_bounds__(256) k_jac_apply_a(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
                                                      int r, double* A, const double* __restrict__ Qbuf,
-                                                     const int* __restrict__ rotflag, int* state) {
+                                                     const int* __restrict__ rotflag, const int* __restrict__ state) {
   __shared__ double t[64 * kLd];
   __shared__ double qk[64 * kLd];
   __shared__ double ql[64 * kLd];
   const int4 it = list[blockIdx.x];
   const JacobiProb pr = probs[it.x];
-  int* st = state + it.x * kJacState;
-  if (jac_skip(pr, st, r, -1, false, nullptr)) return;
+  if (state[it.x * kJacState]) return;
   const int k = it.y, l = it.z;
   const int qb = (int)(pr.q_off / 4096);
   const int rk = rotflag[qb + k], rl = rotflag[qb + l];
@@ -403,13 +409,12 @@ __global__ void __launch_bounds__(256) k_jac_apply_a(const JacobiProb* __restric
 // W[rows, I_l] <- W[rows, I_l] Q_l  for 64-row blocks
 __global__ void __launch_bounds__(256) k_jac_apply_w(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
                                                      int r, double* W, const double* __restrict__ Qbuf,
-                                                     const int* __restrict__ rotflag, int* state) {
+                                                     const int* __restrict__ rotflag, const int* __restrict__ state) {
   __shared__ double t[64 * kLd];
   __shared__ double ql[64 * kLd];
   const int4 it = list[blockIdx.x];
   const JacobiProb pr = probs[it.x];
-  int* st = state + it.x * kJacState;
-  if (jac_skip(pr, st, r, -1, false, nullptr)) return;
+  if (state[it.x * kJacState]) return;
   const int l = it.z, rb = it.y;
   const int qb = (int)(pr.q_off / 4096);
   if (!rotflag[qb + l]) return;
@@ -456,17 +461,19 @@ __global__ void __launch_bounds__(256) k_jac_apply_w(const JacobiProb* __restric
 }
 
 // initialise W = I and the per-problem scale max|A_ii| and state; molecules
-// that finished their SCC are marked converged and skipped
+// that finished their SCC are marked done and skipped
 __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__ probs, const double* __restrict__ A,
                                                   double* W, double* scale, int* state,
-                                                  const MolState* __restrict__ mstate, int* nactive) {
+                                                  const MolState* __restrict__ mstate, unsigned long long* offmax,
+                                                  int keep_w) {
   __shared__ double sh[4];
   const JacobiProb pr = probs[blockIdx.x];
+  int* st = state + blockIdx.x * kJacState;
   if (mstate && !mstate[blockIdx.x].active) {
     if (threadIdx.x == 0) {
-      int* st = state + blockIdx.x * kJacState;
       for (int i = 0; i < kJacState; ++i) st[i] = 0;
       st[0] = 1;
+      offmax[blockIdx.x] = 0ull;
     }
     return;
   }
@@ -475,14 +482,165 @@ __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__
   const double* Ap = A + pr.a_off;
   double m = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(Ap[(long long)i * n + i]));
-  for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x) Wp[e] = (e / n == e % n) ? 1.0 : 0.0;
+  if (!keep_w)
+    for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x) Wp[e] = (e / n == e % n) ? 1.0 : 0.0;
   m = block_max<256>(m, sh);
   if (threadIdx.x == 0) {
     scale[blockIdx.x] = fmax(m, 1.0);
-    int* st = state + blockIdx.x * kJacState;
     for (int i = 0; i < kJacState; ++i) st[i] = 0;
-    if (n > 64) atomicAdd(nactive, 1);
+    offmax[blockIdx.x] = 0ull;
   }
+}
+
+// largest off-diagonal magnitude of every unfinished block-path problem, one
+// workgroup per upper-triangular 64 x 64 tile; max over non-negative doubles
+// is taken on their bit patterns, which order the same way (order independent)
+__global__ void __launch_bounds__(256) k_jac_offmax(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
+                                                    const double* __restrict__ A, const int* __restrict__ state,
+                                                    unsigned long long* offmax) {
+  __shared__ double sh[4];
+  const int4 it = list[blockIdx.x];
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return;
+  const int n = pr.dim;
+  const double* Ap = A + pr.a_off;
+  double m = 0.0;
+  for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+    const int gu = it.y * 64 + (e >> 6), gv = it.z * 64 + (e & 63);
+    if (gu < gv && gv < n) m = fmax(m, fabs(Ap[(long long)gu * n + gv]));
+  }
+  m = block_max<256>(m, sh);
+  if (threadIdx.x == 0 && m > 0.0) atomicMax(&offmax[it.x], (unsigned long long)__double_as_longlong(m));
+}
+
+// convergence test at the start of an outer sweep: done when every
+// off-diagonal element is <= tol, failed after kMaxSweeps sweeps; counts the
+// problems still running into nactive and resets offmax
+__global__ void __launch_bounds__(256) k_jac_mark(const JacobiProb* __restrict__ probs, int nprob,
+                                                  const double* __restrict__ scale, int* state,
+                                                  unsigned long long* offmax, int sweep, int* nactive) {
+  __shared__ double sh[4];
+  double c = 0.0;
+  for (int p = threadIdx.x; p < nprob; p += blockDim.x) {
+    int* st = state + p * kJacState;
+    if (st[0]) continue;
+    if (probs[p].dim <= 64) continue;   // solved by k_jac_small
+    const double m = __longlong_as_double((long long)offmax[p]);
+    offmax[p] = 0ull;
+    if (m <= 1e-12 * scale[p]) {
+      st[0] = 1;
+      st[5] = sweep;
+    } else if (sweep >= kMaxSweeps) {
+      st[0] = 1;
+      st[1] = 1;
+      st[5] = sweep;
+    } else {
+      c += 1.0;
+    }
+  }
+  c = block_sum<256>(c, sh);
+  if (threadIdx.x == 0) nactive[0] = (int)c;
+}
+
+// ---------------------------------------------------------------------------
+// Iterative refinement of an approximate eigenvector matrix X of a symmetric A
+// (Ogita & Aishima, "Iterative refinement for symmetric eigenvalue
+// decomposition", Japan J. Indust. Appl. Math. 35 (2018) 1007-1035, Alg. 1):
+//   R = I - X^T X, S = X^T A X, l_i = s_ii / (1 - r_ii),
+//   e_ij = (s_ij + l_j r_ij) / (l_j - l_i)   (well separated pair),
+//   e_ij = r_ij / 2                         (pair left to the Jacobi sweeps),
+//   X <- X + X E.
+// A pair counts as well separated when |s_ij + l_j r_ij| <= kOaSep |l_j - l_i|.
+constexpr double kOaSep = 0.25;
+
+// E from S (and R = I - XtX, or none when X = I); per-molecule maxima of the
+// off-diagonal |s_ij|, of |r_ij| and of the applied |e_ij|, and max |s_ii|
+__global__ void __launch_bounds__(256) k_oa_e(const MolDesc* __restrict__ desc, const int* __restrict__ act,
+                                              const double* __restrict__ Sm, const double* __restrict__ XtX,
+                                              double* E, unsigned long long* stat) {
+  __shared__ double sh[4];
+  const MolDesc md = desc[blockIdx.y];
+  if (!act[blockIdx.y]) return;
+  const long long N = md.nao;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double soff = 0.0, rmax = 0.0, emax = 0.0, dmax = 0.0;
+  if (t < N * N) {
+    const long long i = t / N, j = t % N;
+    const double* S = Sm + md.m0;
+    const double sij = S[t];
+    double rij = 0.0, rii = 0.0, rjj = 0.0;
+    if (XtX) {
+      const double* G = XtX + md.m0;
+      rij = (i == j ? 1.0 : 0.0) - G[t];
+      rii = 1.0 - G[i * N + i];
+      rjj = 1.0 - G[j * N + j];
+    }
+    double e;
+    if (i == j) {
+      e = 0.5 * rij;
+      dmax = fabs(sij);
+    } else {
+      const double li = S[i * N + i] / (1.0 - rii), lj = S[j * N + j] / (1.0 - rjj);
+      const double num = sij + lj * rij, den = lj - li;
+      if (den != 0.0 && fabs(num) <= kOaSep * fabs(den)) {
+        e = num / den;
+        emax = fabs(e);
+      } else {
+        e = 0.5 * rij;
+      }
+      soff = fabs(sij);
+    }
+    rmax = fabs(rij);
+    E[md.m0 + t] = e;
+  }
+  soff = block_max<256>(soff, sh);
+  rmax = block_max<256>(rmax, sh);
+  emax = block_max<256>(emax, sh);
+  dmax = block_max<256>(dmax, sh);
+  if (threadIdx.x == 0) {
+    unsigned long long* st = stat + 4LL * blockIdx.y;
+    if (soff > 0.0) atomicMax(&st[0], (unsigned long long)__double_as_longlong(soff));
+    if (rmax > 0.0) atomicMax(&st[1], (unsigned long long)__double_as_longlong(rmax));
+    if (emax > 0.0) atomicMax(&st[2], (unsigned long long)__double_as_longlong(emax));
+    if (dmax > 0.0) atomicMax(&st[3], (unsigned long long)__double_as_longlong(dmax));
+  }
+}
+
+// stop refining a molecule when its off-diagonal is at the Jacobi tolerance
+// and X is orthonormal, when no well-separated pair is left to correct, or
+// after maxstep steps; act = 1 keeps it for one more update X <- X + X E
+__global__ void __launch_bounds__(256) k_oa_mark(int nmol, const MolState* __restrict__ mstate, int* act,
+                                                 unsigned long long* stat, int step, int maxstep, int* nactive) {
+  __shared__ double sh[4];
+  double c = 0.0;
+  for (int m = threadIdx.x; m < nmol; m += blockDim.x) {
+    unsigned long long* st = stat + 4LL * m;
+    if (act[m]) {
+      const double soff = __longlong_as_double((long long)st[0]);
+      const double rmax = __longlong_as_double((long long)st[1]);
+      const double emax = __longlong_as_double((long long)st[2]);
+      const double scale = fmax(__longlong_as_double((long long)st[3]), 1.0);
+      const bool done = (soff <= 1e-12 * scale && rmax <= 1e-12) || (emax <= 1e-15 && rmax <= 1e-12) ||
+                        step >= maxstep;
+      if (done) act[m] = 0;
+    }
+    for (int k = 0; k < 4; ++k) st[k] = 0ull;
+    c += act[m];
+  }
+  c = block_sum<256>(c, sh);
+  if (threadIdx.x == 0) nactive[0] = (int)c;
+}
+
+// X <- X + D (D = X E) for refining molecules; with identity = 1, X <- I + E
+__global__ void k_oa_update(const MolDesc* __restrict__ desc, const int* __restrict__ act,
+                            const double* __restrict__ D, double* X, int identity) {
+  const MolDesc md = desc[blockIdx.y];
+  if (!act[blockIdx.y]) return;
+  const long long N = md.nao;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * N) return;
+  const double x = identity ? ((t / N == t % N) ? 1.0 : 0.0) : X[md.m0 + t];
+  X[md.m0 + t] = x + D[md.m0 + t];
 }
 
 // eigenvalues ascending: ev[k] = A[perm[k]][perm[k]] (stable rank sort)

@@ -47,6 +47,9 @@ def library():
     lib.xtbamd_batch_free.argtypes = [ctypes.c_void_p]
     lib.xtbamd_batch_free.restype = None
     lib.xtbamd_device_count.restype = ctypes.c_int
+    lib.xtbamd_batch_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_longlong)]
+    lib.xtbamd_batch_counters.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -111,6 +114,12 @@ class Batch:
     def run(self):
         self._check(self._lib.xtbamd_batch_run(self._h))
         return self
+
+    def counters(self):
+        """(SCC steps launched, block-Jacobi rounds) of the last run."""
+        a, b = ctypes.c_int(), ctypes.c_longlong()
+        self._check(self._lib.xtbamd_batch_counters(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def __len__(self):
         return len(self.natoms)
