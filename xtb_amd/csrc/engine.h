@@ -30,6 +30,7 @@ struct MolDesc {
   long long g0;                 // Broyden small matrices (maxiter x maxiter)
   int nbr, nel, nopen, maxiter;
   double charge;
+  int mol;                      // index of the molecule (slot) in the batch
 };
 
 // Per-molecule SCC state machine (src/scc_core.f90:434-640).
@@ -106,6 +107,9 @@ class Batch {
 
   // full single point of every molecule (setup, SCC loop, gradient)
   void run();
+  // molecules the next setup()/gradient() work on (default: all)
+  void select(const std::vector<int>& mols);
+  void select_all();
   void setup();
   bool scc_iteration();   // returns true while any molecule is active
   void gradient();
@@ -125,6 +129,11 @@ class Batch {
   double intcut = 25.0, neglect = 1e-8, scfconv = 1e-6, qconv = 1e-4;
   std::vector<MolDesc> desc;
   MolDesc* d_desc = nullptr;
+  MolDesc* d_sub = nullptr;     // descriptors of the selected molecules
+  int nsub = 0;
+  int sub_max_n = 0, sub_max_nsh = 0, sub_max_nao = 0, sub_max_nd4 = 0;
+  int* phase_act = nullptr;     // 1 for selected molecules (per slot)
+  std::vector<int> sel_;
   MolState* d_state = nullptr;
   std::vector<MoleculeInput> mols_;
   std::string error_;
@@ -177,16 +186,32 @@ class Batch {
   void jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w = false);
   // refine W (init I) towards the eigenvectors of A; leaves A <- W^T A W
   void refine(double* Am, double* Wm);
-  int oa_max_steps = 6;
+  int oa_max_steps = 0;     // refinement steps before the Jacobi sweeps (0 = off)
+  int inner_sweeps = 1;    // sweeps of each 64x64 pivot problem per block-Jacobi round
+  bool use_mfma = true;   // FP64 matrix-core GEMM (else FP64 VALU FMA)
   double* Sm = nullptr;
   double* G = nullptr;
   int* oa_act = nullptr;
+  int* oa_ok = nullptr;
   unsigned long long* oa_stat = nullptr;
   void gemm(int kind, const double* a, const double* b, double* c, const double* scale, const int* perm,
             bool check_active, double alpha = 1.0, const int* mask = nullptr);
   void orthogonalizer();   // C = S^(-1/2) by Newton-Schulz iterations
   void eeq_solve();        // EEQ charges by Gaussian elimination
 
+  // HIP-event timing of kernel families over a run (roofline measurement):
+  // bit 1 = k_gemm, bit 2 = k_jac_pairs.  Algorithmic work is counted on the
+  // device: 2 N^3 flops per molecule a GEMM launch processes; 36864 flops per
+  // inner Jacobi round of a 64x64 pivot problem (4096 elements rotated on
+  // both sides and in Q, 3 flops each).
+  int prof_mask = 0;
+  std::vector<hipEvent_t> prof_ev_[3];
+  int prof_used_[3] = {0, 0, 0};
+  unsigned long long* prof_work_ = nullptr;   // [3]
+  void prof_begin(int kind);
+  void prof_end(int kind, const int* mask, int mstride);
+  int prof_result(int kind, int* launches, double* total_ms, double* work);
+  void prof_enable(int mask);
   std::vector<void*> allocs_;
   std::vector<int> h_sh_at_;
   std::vector<MolState> h_state0_;   // initial SCC state, restored by every setup()

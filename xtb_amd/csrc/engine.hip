@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -58,8 +59,8 @@ __global__ void __launch_bounds__(256) k_eeq_gesv(const MolDesc* __restrict__ de
   const MolDesc md = desc[blockIdx.x];
   const int ne = md.ne, n = md.n;
   double* M = E + md.e0;
-  double* b = xE + md.a0 + blockIdx.x;
-  double* y = yE + md.a0 + blockIdx.x;
+  double* b = xE + md.a0 + md.mol;
+  double* y = yE + md.a0 + md.mol;
   const int tid = threadIdx.x;
   bool ok = true;
   for (int k = 0; k < ne; ++k) {
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(256) k_eeq_gesv(const MolDesc* __restrict__ de
   }
   if (!ok) {
     for (int i = tid; i < n; i += blockDim.x) q[md.a0 + i] = md.charge / n;
-    if (tid == 0) failed[blockIdx.x] = 1;
+    if (tid == 0) failed[md.mol] = 1;
   }
   __syncthreads();
   for (int i = tid; i < n; i += blockDim.x) {
@@ -138,6 +139,28 @@ __global__ void __launch_bounds__(256) k_eeq_gesv(const MolDesc* __restrict__ de
       qsh[k] = (zsh[k] / e.zval) * q[md.a0 + i];
     }
   }
+}
+
+// zero the atomic dipoles and quadrupoles (or the gradient) of the selected molecules
+__global__ void k_zero_atoms(const MolDesc* __restrict__ desc, double* v3, double* v6) {
+  const MolDesc md = desc[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= md.n) return;
+  for (int k = 0; k < 3; ++k) v3[3 * (md.a0 + i) + k] = 0.0;
+  if (v6)
+    for (int k = 0; k < 6; ++k) v6[6 * (md.a0 + i) + k] = 0.0;
+}
+
+// Newton-Schulz bookkeeping for the selected molecules: stage 0 (running),
+// the others stay done (stage 2)
+__global__ void k_ns_begin(int nmol, const int* __restrict__ sel, int* stage, int* act, int* failed,
+                           unsigned long long* res) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmol) return;
+  stage[m] = sel[m] ? 0 : 2;
+  act[m] = sel[m] ? 1 : 0;
+  if (sel[m]) failed[m] = 0;
+  res[m] = 0ull;
 }
 
 // SAD guess: zero atomic charges except the total charge spread evenly
@@ -172,7 +195,7 @@ __global__ void __launch_bounds__(256) k_ns_bound(const MolDesc* __restrict__ de
     m = fmax(m, r);
   }
   m = block_max<256>(m, sh);
-  if (threadIdx.x == 0) cbound[blockIdx.x] = m;
+  if (threadIdx.x == 0) cbound[md.mol] = m;
 }
 
 __global__ void k_ns_init(const MolDesc* __restrict__ desc, const double* __restrict__ S,
@@ -181,7 +204,7 @@ __global__ void k_ns_init(const MolDesc* __restrict__ desc, const double* __rest
   const long long N = md.nao;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N * N) return;
-  Y[md.m0 + t] = S[md.m0 + t] / cbound[blockIdx.y];
+  Y[md.m0 + t] = S[md.m0 + t] / cbound[md.mol];
   Z[md.m0 + t] = (t / N == t % N) ? 1.0 : 0.0;
 }
 
@@ -190,8 +213,9 @@ __global__ void __launch_bounds__(256) k_ns_form(const MolDesc* __restrict__ des
                                                  double* P, unsigned long long* res) {
   __shared__ double sh[4];
   const MolDesc md = desc[blockIdx.y];
-  if (!act[blockIdx.y]) return;
+  if (!act[md.mol]) return;
   const long long N = md.nao;
+  if ((long long)blockIdx.x * blockDim.x >= N * N) return;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double r = 0.0;
   if (t < N * N) {
@@ -201,7 +225,7 @@ __global__ void __launch_bounds__(256) k_ns_form(const MolDesc* __restrict__ des
     P[md.m0 + t] = 0.5 * (3.0 * d - p);
   }
   r = block_max<256>(r, sh);
-  if (threadIdx.x == 0 && r > 0.0) atomicMax(&res[blockIdx.y], (unsigned long long)__double_as_longlong(r));
+  if (threadIdx.x == 0 && r > 0.0) atomicMax(&res[md.mol], (unsigned long long)__double_as_longlong(r));
 }
 
 // per-molecule convergence: residual < 1e-10, then one more update (quadratic
@@ -231,6 +255,16 @@ __global__ void k_ident_active(const MolDesc* __restrict__ desc, const MolState*
   if (t < N * N) W[md.m0 + t] = (t / N == t % N) ? 1.0 : 0.0;
 }
 
+// algorithmic flops of one batched GEMM launch: 2 N^3 per molecule processed
+__global__ void k_count_gemm(const MolDesc* __restrict__ desc, int nmol, const int* __restrict__ mask, int mstride,
+                             unsigned long long* work) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmol) return;
+  if (mask && !mask[(long long)m * mstride]) return;
+  const unsigned long long N = (unsigned long long)desc[m].nao;
+  atomicAdd(work, 2ull * N * N * N);
+}
+
 __global__ void k_copy_state_active(const MolState* __restrict__ st, int nmol, int* act) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m < nmol) act[m] = st[m].active;
@@ -239,7 +273,7 @@ __global__ void k_copy_state_active(const MolState* __restrict__ st, int nmol, i
 __global__ void k_copy_mask(const MolDesc* __restrict__ desc, const int* __restrict__ act, const double* __restrict__ a,
                             double* b) {
   const MolDesc md = desc[blockIdx.y];
-  if (!act[blockIdx.y]) return;
+  if (!act[md.mol]) return;
   const long long n2 = (long long)md.nao * md.nao;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n2) b[md.m0 + t] = a[md.m0 + t];
@@ -251,7 +285,7 @@ __global__ void k_ns_final(const MolDesc* __restrict__ desc, const double* __res
   const long long N = md.nao;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N * N) return;
-  C[md.m0 + t] = Z[md.m0 + t] / sqrt(cbound[blockIdx.y]);
+  C[md.m0 + t] = Z[md.m0 + t] / sqrt(cbound[md.mol]);
 }
 
 // total energy, gradient norm and dipole (src/scf_module.F90:882-913,
@@ -263,7 +297,7 @@ __global__ void __launch_bounds__(256) k_final(const MolDesc* __restrict__ desc,
                                                const double* __restrict__ row_d) {
   __shared__ double sh[4];
   const MolDesc md = desc[blockIdx.x];
-  MolState& st = state[blockIdx.x];
+  MolState& st = state[md.mol];
   double ep = 0, eb = 0, g2 = 0, dx = 0, dy = 0, dz = 0;
   for (int i = threadIdx.x; i < md.n; i += blockDim.x) {
     ep += erep[md.a0 + i];
@@ -333,6 +367,10 @@ T_* Batch::alloc(size_t n) {
 
 Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream_t s)
     : opt(o), stream(s), mols_(mols) {
+  // XTBAMD_GEMM=mfma selects the FP64 matrix-core GEMM instead of VALU FMA
+  if (const char* g = std::getenv("XTBAMD_GEMM")) use_mfma = std::string(g) == "mfma";
+  if (const char* r = std::getenv("XTBAMD_REFINE_STEPS")) oa_max_steps = std::atoi(r);
+  if (const char* r = std::getenv("XTBAMD_INNER_SWEEPS")) inner_sweeps = std::max(1, std::atoi(r));
   const Element* el = host_elements();
   nmol = (int)mols.size();
   if (nmol == 0) throw std::runtime_error("empty batch");
@@ -385,6 +423,7 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
       for (int k = 0; k < 3; ++k) h_xyz.push_back(mi.xyz[3 * i + k]);
     }
     d.ne = d.n + 1;
+    d.mol = m;
     d.charge = mi.charge;
     d.nel = (int)std::lround(zsum - mi.charge);
     d.nopen = mi.uhf;
@@ -566,10 +605,13 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
   erep = alloc<double>(tot_n);
   perm = alloc<int>(tot_nao + tot_n + nmol);
   eeq_failed = alloc<int>(nmol);
+  d_sub = alloc<MolDesc>(nmol);
+  phase_act = alloc<int>(nmol);
   ns_c = alloc<double>(nmol);
   Sm = alloc<double>(tot_mm);
   G = alloc<double>(tot_mm);
   oa_act = alloc<int>(nmol);
+  oa_ok = alloc<int>(nmol);
   oa_stat = alloc<unsigned long long>(4LL * nmol);
   ns_res = alloc<unsigned long long>(nmol);
   ns_stage = alloc<int>(nmol);
@@ -581,6 +623,8 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
 }
 
 Batch::~Batch() {
+  for (auto& v : prof_ev_)
+    for (hipEvent_t e : v) (void)hipEventDestroy(e);
   for (void* p : allocs_) (void)hipFree(p);
   if (h_pinned) (void)hipHostFree(h_pinned);
 }
@@ -604,8 +648,11 @@ void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
       HIPCHECK(hipStreamSynchronize(stream));
       if (h_pinned[0] <= 0) break;
       for (int c = 0; c < js.max_rounds; ++c, ++r) {
+        prof_begin(2);
         hipLaunchKernelGGL(k_jac_pairs, dim3(js.npairs), dim3(256), 0, stream, js.d_probs, js.d_pairs, r,
-                           (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state);
+                           (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
+                           inner_sweeps, (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr);
+        prof_end(2, nullptr, 0);
         hipLaunchKernelGGL(k_jac_apply_a, dim3(js.nta), dim3(256), 0, stream, js.d_probs, js.d_ta, r, Am,
                            (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
         hipLaunchKernelGGL(k_jac_apply_w, dim3(js.ntw), dim3(256), 0, stream, js.d_probs, js.d_tw, r, Wm,
@@ -624,6 +671,7 @@ void Batch::refine(double* Am, double* Wm) {
   // refining molecules = SCC-active ones
   hipLaunchKernelGGL(k_copy_state_active, dim3(grid1(nmol, 64)), dim3(64), 0, stream, (const MolState*)d_state, nmol,
                      oa_act);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nmol, 64)), dim3(64), 0, stream, oa_ok, (long long)nmol, 0);
   hipLaunchKernelGGL(k_fill, dim3(grid1(4LL * nmol, 256)), dim3(256), 0, stream, (double*)oa_stat, 4LL * nmol, 0.0);
   hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_desc, (const int*)oa_act, (const double*)Am, Sm);
   for (int step = 0; step <= oa_max_steps; ++step) {
@@ -634,8 +682,8 @@ void Batch::refine(double* Am, double* Wm) {
     }
     hipLaunchKernelGGL(k_oa_e, gmm, dim3(256), 0, stream, d_desc, (const int*)oa_act, (const double*)Sm,
                        step > 0 ? (const double*)G : (const double*)nullptr, H, oa_stat);
-    hipLaunchKernelGGL(k_oa_mark, dim3(1), dim3(256), 0, stream, nmol, (const MolState*)d_state, oa_act, oa_stat, step,
-                       oa_max_steps, jac_nactive);
+    hipLaunchKernelGGL(k_oa_mark, dim3(1), dim3(256), 0, stream, nmol, (const MolDesc*)d_desc, oa_act, oa_ok, oa_stat,
+                       step, oa_max_steps, jac_nactive);
     HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     if (h_pinned[0] <= 0) break;
@@ -647,10 +695,8 @@ void Batch::refine(double* Am, double* Wm) {
     }
     oa_steps_ += 1;
   }
-  // A <- X^T A X for the refined molecules (the Jacobi sweeps continue from there)
-  hipLaunchKernelGGL(k_copy_state_active, dim3(grid1(nmol, 64)), dim3(64), 0, stream, (const MolState*)d_state, nmol,
-                     oa_act);
-  hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_desc, (const int*)oa_act, (const double*)Sm, Am);
+  hipLaunchKernelGGL(k_oa_finish, gmm, dim3(256), 0, stream, d_desc, (const MolState*)d_state, (const int*)oa_ok,
+                     (const double*)Sm, Am, Wm);
 }
 
 void Batch::set_guess(int m, const double* qsh_in, const double* dipm_in, const double* qp_in) {
@@ -675,39 +721,112 @@ void Batch::gemm(int kind, const double* a, const double* b, double* c, const do
     m = &d_state[0].active;
     ms = (int)(sizeof(MolState) / sizeof(int));
   }
+  prof_begin(1);
+#define XTBAMD_GEMM(K, TA, TB, SC, PM)                                                                              \
+  case K:                                                                                                          \
+    if (use_mfma)                                                                                                  \
+      hipLaunchKernelGGL((k_gemm_mfma<TA, TB, SC, PM>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b, \
+                         c, scale_, perm_, alpha, m, ms);                                                           \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_gemm<TA, TB, SC, PM>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b, c,   \
+                         scale_, perm_, alpha, m, ms);                                                              \
+    break;
   switch (kind) {
-    case 0:
-      hipLaunchKernelGGL((k_gemm<false, false, false, false>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
-                         c, scale_, perm_, alpha, m, ms);
-      break;
-    case 1:
-      hipLaunchKernelGGL((k_gemm<true, false, false, false>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
-                         c, scale_, perm_, alpha, m, ms);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_gemm<false, true, true, false>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
-                         c, scale_, perm_, alpha, m, ms);
-      break;
+    XTBAMD_GEMM(0, false, false, false, false)
+    XTBAMD_GEMM(1, true, false, false, false)
+    XTBAMD_GEMM(2, false, true, true, false)
     default:
-      hipLaunchKernelGGL((k_gemm<false, false, false, true>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b,
-                         c, scale_, perm_, alpha, m, ms);
-      break;
+    XTBAMD_GEMM(3, false, false, false, true)
+  }
+#undef XTBAMD_GEMM
+  prof_end(1, m, ms);
+}
+
+void Batch::prof_enable(int mask) {
+  prof_mask = mask;
+  for (int k = 0; k < 3; ++k) prof_used_[k] = 0;
+  if (!prof_work_) prof_work_ = alloc<unsigned long long>(3);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, (double*)prof_work_, 3LL, 0.0);
+  HIPCHECK(hipStreamSynchronize(stream));
+}
+
+void Batch::prof_begin(int kind) {
+  if (!(prof_mask & (1 << kind))) return;
+  std::vector<hipEvent_t>& ev = prof_ev_[kind];
+  if (prof_used_[kind] + 2 > (int)ev.size()) {
+    const size_t old = ev.size();
+    ev.resize(old + 1024);
+    for (size_t i = old; i < ev.size(); ++i) HIPCHECK(hipEventCreate(&ev[i]));
+  }
+  HIPCHECK(hipEventRecord(ev[prof_used_[kind]], stream));
+}
+
+void Batch::prof_end(int kind, const int* mask, int mstride) {
+  if (!(prof_mask & (1 << kind))) return;
+  HIPCHECK(hipEventRecord(prof_ev_[kind][prof_used_[kind] + 1], stream));
+  prof_used_[kind] += 2;
+  if (kind == 1)
+    hipLaunchKernelGGL(k_count_gemm, dim3(grid1(nmol, 256)), dim3(256), 0, stream, d_desc, nmol, mask, mstride,
+                       prof_work_ + 1);
+}
+
+int Batch::prof_result(int kind, int* launches, double* total_ms, double* work) {
+  if (kind < 1 || kind > 2) throw std::runtime_error("unknown kernel family");
+  HIPCHECK(hipStreamSynchronize(stream));
+  double t = 0.0;
+  for (int i = 0; i + 1 < prof_used_[kind]; i += 2) {
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, prof_ev_[kind][i], prof_ev_[kind][i + 1]));
+    t += ms;
+  }
+  unsigned long long w = 0;
+  if (prof_work_) HIPCHECK(hipMemcpy(&w, prof_work_ + kind, sizeof(w), hipMemcpyDeviceToHost));
+  if (launches) *launches = prof_used_[kind] / 2;
+  if (total_ms) *total_ms = t;
+  if (work) *work = (double)w;
+  return 0;
+}
+
+void Batch::select(const std::vector<int>& mols) {
+  sel_ = mols;
+  nsub = (int)mols.size();
+  std::vector<MolDesc> sub(nsub);
+  std::vector<int> flag(nmol, 0);
+  for (int k = 0; k < nsub; ++k) {
+    if (mols[k] < 0 || mols[k] >= nmol) throw std::runtime_error("invalid molecule index");
+    sub[k] = desc[mols[k]];
+    flag[mols[k]] = 1;
+  }
+  if (nsub) HIPCHECK(hipMemcpyAsync(d_sub, sub.data(), nsub * sizeof(MolDesc), hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipMemcpyAsync(phase_act, flag.data(), nmol * sizeof(int), hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+  sub_max_n = sub_max_nsh = sub_max_nao = sub_max_nd4 = 0;
+  for (const MolDesc& d : sub) {
+    sub_max_n = std::max(sub_max_n, d.n);
+    sub_max_nsh = std::max(sub_max_nsh, d.nsh);
+    sub_max_nao = std::max(sub_max_nao, d.nao);
+    sub_max_nd4 = std::max(sub_max_nd4, d.nd4);
   }
 }
 
+void Batch::select_all() {
+  std::vector<int> all(nmol);
+  for (int m = 0; m < nmol; ++m) all[m] = m;
+  select(all);
+}
+
 void Batch::orthogonalizer() {
-  const dim3 gmm(grid1((long long)max_nao * max_nao, 256), nmol);
-  hipLaunchKernelGGL(k_ns_bound, dim3(nmol), dim3(256), 0, stream, d_desc, (const double*)S, ns_c);
-  hipLaunchKernelGGL(k_ns_init, gmm, dim3(256), 0, stream, d_desc, (const double*)S, (const double*)ns_c, A, W);
-  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nmol, 64)), dim3(64), 0, stream, ns_stage, (long long)nmol, 0);
-  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nmol, 64)), dim3(64), 0, stream, ns_act, (long long)nmol, 1);
-  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nmol, 64)), dim3(64), 0, stream, ns_failed, (long long)nmol, 0);
-  hipLaunchKernelGGL(k_fill, dim3(grid1(2 * nmol, 64)), dim3(64), 0, stream, (double*)ns_res, (long long)nmol, 0.0);
+  // C = S^(-1/2) for the selected molecules
+  const dim3 gmm(grid1((long long)sub_max_nao * sub_max_nao, 256), nsub);
+  hipLaunchKernelGGL(k_ns_bound, dim3(nsub), dim3(256), 0, stream, d_sub, (const double*)S, ns_c);
+  hipLaunchKernelGGL(k_ns_init, gmm, dim3(256), 0, stream, d_sub, (const double*)S, (const double*)ns_c, A, W);
+  hipLaunchKernelGGL(k_ns_begin, dim3(grid1(nmol, 64)), dim3(64), 0, stream, nmol, (const int*)phase_act, ns_stage,
+                     ns_act, ns_failed, ns_res);
   const int maxit = 60;
   for (int it = 0; it <= maxit; ++it) {
     // Y in A, Z in W
     gemm(0, W, A, P, nullptr, nullptr, false, 1.0, ns_act);           // P = Z Y
-    hipLaunchKernelGGL(k_ns_form, gmm, dim3(256), 0, stream, d_desc, (const int*)ns_act, P, ns_res);
+    hipLaunchKernelGGL(k_ns_form, gmm, dim3(256), 0, stream, d_sub, (const int*)ns_act, P, ns_res);
     hipLaunchKernelGGL(k_ns_mark, dim3(1), dim3(256), 0, stream, nmol, ns_stage, ns_act, ns_res, it, maxit, ns_failed,
                        jac_nactive);
     HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -715,53 +834,54 @@ void Batch::orthogonalizer() {
     if (h_pinned[0] <= 0) break;
     gemm(0, A, P, H, nullptr, nullptr, false, 1.0, ns_act);           // Y T
     gemm(0, P, W, T, nullptr, nullptr, false, 1.0, ns_act);           // T Z
-    hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_desc, (const int*)ns_act, (const double*)H, A);
-    hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_desc, (const int*)ns_act, (const double*)T, W);
+    hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_sub, (const int*)ns_act, (const double*)H, A);
+    hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_sub, (const int*)ns_act, (const double*)T, W);
   }
-  hipLaunchKernelGGL(k_ns_final, gmm, dim3(256), 0, stream, d_desc, (const double*)W, (const double*)ns_c, C);
+  hipLaunchKernelGGL(k_ns_final, gmm, dim3(256), 0, stream, d_sub, (const double*)W, (const double*)ns_c, C);
 }
 
 void Batch::setup() {
+  if (nsub == 0) return;
   const Element* el = elements_dev;
+  const int ns = nsub;
   // every single point starts from the initial SCC state (iteration 0, active)
-  HIPCHECK(hipMemcpyAsync(d_state, h_state0_.data(), nmol * sizeof(MolState), hipMemcpyHostToDevice, stream));
+  for (int m : sel_)
+    HIPCHECK(hipMemcpyAsync(d_state + m, &h_state0_[m], sizeof(MolState), hipMemcpyHostToDevice, stream));
   HIPCHECK(hipStreamSynchronize(stream));
   iterations_ = 0;
   jacobi_rounds_ = 0;
   oa_steps_ = 0;
-  const dim3 gat(max_n, nmol);
-  hipLaunchKernelGGL(k_cn, gat, dim3(64), 0, stream, d_desc, el, at, xyz, cn, cn_cov, cn_d4, cn_eeq);
-  hipLaunchKernelGGL(k_shell_setup, dim3(grid1(max_nsh, 64), nmol), dim3(64), 0, stream, d_desc, el, at, sh_at, sh_m,
-                     cn, selfe, dsedcn, shgam, zsh);
-  const long long maxpairs = (long long)max_n * (max_n - 1) / 2;
+  const dim3 gat(sub_max_n, ns);
+  hipLaunchKernelGGL(k_cn, gat, dim3(64), 0, stream, d_sub, el, at, xyz, cn, cn_cov, cn_d4, cn_eeq);
+  hipLaunchKernelGGL(k_shell_setup, dim3(grid1(sub_max_nsh, 64), ns), dim3(64), 0, stream, d_sub, el, at, sh_at,
+                     sh_m, cn, selfe, dsedcn, shgam, zsh);
+  const long long maxpairs = (long long)sub_max_n * (sub_max_n - 1) / 2;
   if (maxpairs > 0)
-    hipLaunchKernelGGL(k_sdqh0, dim3(grid1(maxpairs, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0,
+    hipLaunchKernelGGL(k_sdqh0, dim3(grid1(maxpairs, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
                        at_ao0, selfe, intcut, S, H0, H0n, D, Q);
-  hipLaunchKernelGGL(k_sdq_onsite, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0,
+  hipLaunchKernelGGL(k_sdq_onsite, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
                      at_ao0, selfe, intcut, S, H0, H0n, D, Q);
-  hipLaunchKernelGGL(k_coulomb, dim3(grid1((long long)max_nsh * max_nsh, 256), nmol), dim3(256), 0, stream, d_desc,
-                     el, at, xyz, sh_at, sh_m, J);
-  hipLaunchKernelGGL(k_d4_weights, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, cn_d4, gw);
-  hipLaunchKernelGGL(k_dispmat, dim3(grid1((long long)max_nd4 * max_nd4, 256), nmol), dim3(256), 0, stream, d_desc,
-                     el, at, xyz, d4_at, d4_ref, gw, c6ref, dispmat);
-  hipLaunchKernelGGL(k_aes_setup, dim3(grid1((long long)max_n * max_n, 256), nmol), dim3(256), 0, stream, d_desc, el,
-                     at, xyz, cn, radcn, gab3, gab5);
+  hipLaunchKernelGGL(k_coulomb, dim3(grid1((long long)sub_max_nsh * sub_max_nsh, 256), ns), dim3(256), 0, stream,
+                     d_sub, el, at, xyz, sh_at, sh_m, J);
+  hipLaunchKernelGGL(k_d4_weights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, cn_d4, gw);
+  hipLaunchKernelGGL(k_dispmat, dim3(grid1((long long)sub_max_nd4 * sub_max_nd4, 256), ns), dim3(256), 0, stream,
+                     d_sub, el, at, xyz, d4_at, d4_ref, gw, c6ref, dispmat);
+  hipLaunchKernelGGL(k_aes_setup, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub,
+                     el, at, xyz, cn, radcn, gab3, gab5);
   // initial charges: EEQ (default) or SAD
-  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nmol, 64)), dim3(64), 0, stream, eeq_failed, (long long)nmol, 0);
   if (opt.guess == 0) {
-    const long long ne = max_n + 1;
-    hipLaunchKernelGGL(k_eeq_matrix, dim3(grid1(ne * ne, 256), nmol), dim3(256), 0, stream, d_desc, el, at, xyz,
-                       cn_eeq, E, xE);
-    hipLaunchKernelGGL(k_eeq_gesv, dim3(nmol), dim3(256), 0, stream, d_desc, el, at, at_sh0, E, xE, yE, q, qsh,
+    const long long ne = sub_max_n + 1;
+    hipLaunchKernelGGL(k_eeq_matrix, dim3(grid1(ne * ne, 256), ns), dim3(256), 0, stream, d_sub, el, at, xyz, cn_eeq,
+                       E, xE);
+    hipLaunchKernelGGL(k_eeq_gesv, dim3(ns), dim3(256), 0, stream, d_sub, el, at, at_sh0, E, xE, yE, q, qsh,
                        (const double*)zsh, eeq_failed);
   } else {
-    hipLaunchKernelGGL(k_sad_guess, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, at_sh0, q,
+    hipLaunchKernelGGL(k_sad_guess, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, at_sh0, q,
                        qsh, (const double*)zsh);
   }
-  hipLaunchKernelGGL(k_fill, dim3(grid1(3 * tot_n, 256)), dim3(256), 0, stream, dipm, 3LL * tot_n, 0.0);
-  hipLaunchKernelGGL(k_fill, dim3(grid1(6 * tot_n, 256)), dim3(256), 0, stream, qp, 6LL * tot_n, 0.0);
+  hipLaunchKernelGGL(k_zero_atoms, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, dipm, qp);
   // restart guesses replace the initial charges (src/api/interface.f90:83-101, src/scf_module.F90:292)
-  for (int m = 0; m < nmol; ++m) {
+  for (int m : sel_) {
     if (guess_qsh_.empty() || guess_qsh_[m].empty()) continue;
     const MolDesc& d = desc[m];
     std::vector<double> qa(d.n, 0.0);
@@ -841,43 +961,47 @@ bool Batch::scc_iteration() {
 }
 
 void Batch::gradient() {
+  if (nsub == 0) return;
   const Element* el = elements_dev;
+  const int ns = nsub;
   // energy-weighted density into T
-  hipLaunchKernelGGL(k_pew_scale, dim3(grid1(max_nao, 64), nmol), dim3(64), 0, stream, d_desc, (const double*)focc,
+  hipLaunchKernelGGL(k_pew_scale, dim3(grid1(sub_max_nao, 64), ns), dim3(64), 0, stream, d_sub, (const double*)focc,
                      (const double*)emo, row_pd);
-  gemm(2, C, C, T, row_pd, nullptr, false);
-  hipLaunchKernelGGL(k_aes_pot, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, q, dipm, qp,
+  gemm(2, C, C, T, row_pd, nullptr, false, 1.0, phase_act);
+  hipLaunchKernelGGL(k_aes_pot, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, q, dipm, qp,
                      gab3, gab5, 0, vs, vd, vq, (const MolState*)nullptr);
-  hipLaunchKernelGGL(k_fill, dim3(grid1(3 * tot_n, 256)), dim3(256), 0, stream, grad, 3LL * tot_n, 0.0);
-  const dim3 gat(max_n, nmol);
-  hipLaunchKernelGGL(k_grad_h0, gat, dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0, at_ao0, (const double*)selfe,
+  hipLaunchKernelGGL(k_zero_atoms, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, grad,
+                     (double*)nullptr);
+  const dim3 gat(sub_max_n, ns);
+  hipLaunchKernelGGL(k_grad_h0, gat, dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)selfe,
                      (const double*)dsedcn, (const double*)P, (const double*)T, (const double*)H0n, (const double*)S,
                      (const double*)shift, (const double*)vs, (const double*)vd, (const double*)vq, intcut, grad,
                      dhdcn);
-  hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_desc, xyz, q, dipm, qp, radcn, gab3, gab5, grad, tmp2);
-  hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, cn_cov, q,
+  hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_sub, xyz, q, dipm, qp, radcn, gab3, gab5, grad, tmp2);
+  hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, cn_cov, q,
                      zetavec, zerovec, zetadcn, zerodcn);
-  hipLaunchKernelGGL(k_d4_c6, dim3(grid1((long long)max_n * max_n, 256), nmol), dim3(256), 0, stream, d_desc, el, at,
-                     c6ref, zetavec, zerovec, zetadcn, zerodcn, c6, dc6, c60, dc60);
-  hipLaunchKernelGGL(k_d4_grad2, gat, dim3(64), 0, stream, d_desc, el, at, xyz, (const double*)c6, (const double*)dc6,
+  hipLaunchKernelGGL(k_d4_c6, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub, el,
+                     at, c6ref, zetavec, zerovec, zetadcn, zerodcn, c6, dc6, c60, dc60);
+  hipLaunchKernelGGL(k_d4_grad2, gat, dim3(64), 0, stream, d_sub, el, at, xyz, (const double*)c6, (const double*)dc6,
                      grad, dedcn);
-  hipLaunchKernelGGL(k_d4_atm, gat, dim3(256), 0, stream, d_desc, el, at, xyz, (const double*)c60, (const double*)dc60,
+  hipLaunchKernelGGL(k_d4_atm, gat, dim3(256), 0, stream, d_sub, el, at, xyz, (const double*)c60, (const double*)dc60,
                      grad, dedcn, eat);
-  hipLaunchKernelGGL(k_grad_pairs, gat, dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0, (const double*)qsh, grad,
+  hipLaunchKernelGGL(k_grad_pairs, gat, dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0, (const double*)qsh, grad,
                      erep);
-  hipLaunchKernelGGL(k_cn_chain, gat, dim3(64), 0, stream, d_desc, el, at, xyz, (const double*)cn,
+  hipLaunchKernelGGL(k_cn_chain, gat, dim3(64), 0, stream, d_sub, el, at, xyz, (const double*)cn,
                      (const double*)dhdcn, (const double*)tmp2, (const double*)dedcn, grad);
   // bond orders from P S
-  gemm(0, P, S, A, nullptr, nullptr, false);
-  hipLaunchKernelGGL(k_wbo, dim3(grid1((long long)max_n * max_n, 256), nmol), dim3(256), 0, stream, d_desc, xyz,
+  gemm(0, P, S, A, nullptr, nullptr, false, 1.0, phase_act);
+  hipLaunchKernelGGL(k_wbo, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub, xyz,
                      at_ao0, at, el, (const double*)A, wbo);
-  hipLaunchKernelGGL(k_final, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, el, at, xyz, (const double*)grad,
+  hipLaunchKernelGGL(k_final, dim3(ns), dim3(256), 0, stream, d_sub, d_state, el, at, xyz, (const double*)grad,
                      (const double*)erep, (const double*)eat, (const double*)row_d);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(stream));
 }
 
 void Batch::run() {
+  select_all();
   setup();
   int guard = 0;
   while (scc_iteration()) {
@@ -1018,6 +1142,21 @@ int xtbamd_batch_counters(xtbamd_batch b, int* scc_steps, long long* jacobi_roun
   if (scc_steps) *scc_steps = b->batch->iterations_;
   if (jacobi_rounds) *jacobi_rounds = b->batch->jacobi_rounds_;
   return 0;
+}
+
+int xtbamd_batch_profile(xtbamd_batch b, int kernels) {
+  try {
+    if (!b || kernels < 0 || kernels > 3) throw std::runtime_error("unknown kernel family");
+    b->batch->prof_enable(kernels << 1);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_profile_result(xtbamd_batch b, int kernel, int* launches, double* total_ms, double* work) {
+  try {
+    if (!b) throw std::runtime_error("invalid batch");
+    return b->batch->prof_result(kernel, launches, total_ms, work);
+  } catch (const std::exception& e) { return fail(e); }
 }
 
 int xtbamd_batch_size(xtbamd_batch b) { return b ? b->batch->size() : 0; }

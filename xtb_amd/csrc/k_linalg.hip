@@ -88,6 +88,87 @@ __global__ void __launch_bounds__(256) k_gemm(const int4* __restrict__ tiles, co
   }
 }
 
+// The same batched product on the FP64 matrix cores: each wave owns a 32x32
+// quarter of the 64x64 tile as 2x2 blocks of v_mfma_f64_16x16x4f64.
+// Operand lanes: A(16x4) row = lane%16, k = lane/16; B(4x16) k = lane/16,
+// column = lane%16; result register i of a lane holds row lane/16 + 4 i,
+// column lane%16.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <bool TA, bool TB, bool SCALE, bool PERM>
+__global__ void __launch_bounds__(256) k_gemm_mfma(const int4* __restrict__ tiles, const MolDesc* __restrict__ desc,
+                                                   const double* __restrict__ A, const double* __restrict__ B,
+                                                   double* C, const double* __restrict__ scale,
+                                                   const int* __restrict__ perm, double alpha,
+                                                   const int* __restrict__ mask, int mstride) {
+  __shared__ double As[kBK][kTile + 1];
+  __shared__ double Bs[kBK][kTile + 1];
+  const int4 tl = tiles[blockIdx.x];
+  if (mask && !mask[(long long)tl.x * mstride]) return;
+  const MolDesc md = desc[tl.x];
+  const int N = md.nao;
+  const double* Am = A + md.m0;
+  const double* Bm = B + md.m0;
+  double* Cm = C + md.m0;
+  const double* sc = SCALE ? scale + md.o0 : nullptr;
+  const int* pm = PERM ? perm + md.o0 : nullptr;
+  const int row0 = tl.y * kTile, col0 = tl.z * kTile;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1, i16 = lane & 15, k4 = lane >> 4;
+  v4d acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[r][c] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int kb = 0; kb < N; kb += kBK) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int ii, kk;
+      if (TA) { kk = e >> 6; ii = e & 63; } else { ii = e >> 4; kk = e & 15; }
+      const int gi = row0 + ii, gk = kb + kk;
+      double v = 0.0;
+      if (gi < N && gk < N) v = TA ? Am[(long long)gk * N + gi] : Am[(long long)gi * N + gk];
+      As[kk][ii] = v;
+      int jj, k2;
+      if (TB) { jj = e >> 4; k2 = e & 15; } else { k2 = e >> 6; jj = e & 63; }
+      const int gj = col0 + jj, gk2 = kb + k2;
+      double x = 0.0;
+      if (gj < N && gk2 < N) {
+        const int cj = PERM ? pm[gj] : gj;
+        x = TB ? Bm[(long long)cj * N + gk2] : Bm[(long long)gk2 * N + cj];
+        if (SCALE) x *= sc[gk2];
+      }
+      Bs[k2][jj] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 4) {
+      double a[2], b[2];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) a[r] = As[kk + k4][32 * wr + 16 * r + i16];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) b[c] = Bs[kk + k4][32 * wc + 16 * c + i16];
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int gj = col0 + 32 * wc + 16 * c + i16;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int gi = row0 + 32 * wr + 16 * r + k4 + 4 * i;
+        if (gi < N && gj < N) Cm[(long long)gi * N + gj] = alpha * acc[r][c][i];
+      }
+    }
+}
+
 // instantiations used by the engine
 template __global__ void k_gemm<false, false, false, false>(const int4*, const MolDesc*, const double*, const double*,
                                                             double*, const double*, const int*, double, const int*, int);
@@ -97,6 +178,18 @@ template __global__ void k_gemm<false, true, true, false>(const int4*, const Mol
                                                           double*, const double*, const int*, double, const int*, int);
 template __global__ void k_gemm<false, false, false, true>(const int4*, const MolDesc*, const double*, const double*,
                                                            double*, const double*, const int*, double, const int*, int);
+template __global__ void k_gemm_mfma<false, false, false, false>(const int4*, const MolDesc*, const double*,
+                                                                 const double*, double*, const double*, const int*,
+                                                                 double, const int*, int);
+template __global__ void k_gemm_mfma<true, false, false, false>(const int4*, const MolDesc*, const double*,
+                                                                const double*, double*, const double*, const int*,
+                                                                double, const int*, int);
+template __global__ void k_gemm_mfma<false, true, true, false>(const int4*, const MolDesc*, const double*,
+                                                               const double*, double*, const double*, const int*,
+                                                               double, const int*, int);
+template __global__ void k_gemm_mfma<false, false, false, true>(const int4*, const MolDesc*, const double*,
+                                                                const double*, double*, const double*, const int*,
+                                                                double, const int*, int);
 
 // ---------------------------------------------------------------------------
 // Cyclic parallel Jacobi on a 64x64 symmetric matrix held in LDS.
@@ -223,8 +316,6 @@ __device__ __forceinline__ void circle_pair(int nbp, int r, int k, int& bp, int&
 // inner rounds with rotations (dim <= 64)
 constexpr int kJacState = 8;
 constexpr int kMaxSweeps = 40;
-// inner sweeps per pivot subproblem of the block path
-constexpr int kInnerSweeps = 3;
 
 // problems of dimension <= 64: one workgroup solves to convergence in LDS
 __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict__ probs, const int* __restrict__ list,
@@ -282,10 +373,11 @@ __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict_
 __device__ __forceinline__ int sub_index(int bp, int bq, int u) { return u < 32 ? 32 * bp + u : 32 * bq + (u - 32); }
 
 // one workgroup per (problem, pair): diagonalise the 64x64 pivot subproblem
-// (at most kInnerSweeps sweeps)
+// (at most inner_sweeps sweeps)
 __global__ void __launch_bounds__(256) k_jac_pairs(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
                                                    int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
-                                                   const double* __restrict__ scale_in, const int* __restrict__ state) {
+                                                   const double* __restrict__ scale_in, const int* __restrict__ state,
+                                                   int inner_sweeps, unsigned long long* work) {
   __shared__ double a[64 * kLd];
   __shared__ double q[64 * kLd];
   __shared__ double cs[96];
@@ -308,12 +400,15 @@ __global__ void __launch_bounds__(256) k_jac_pairs(const JacobiProb* __restrict_
   }
   __syncthreads();
   const double tol = 1e-12 * scale_in[it.x];
-  const int nrot = jacobi64(a, q, tol, kInnerSweeps, cs, pq, flags, red);
+  const int nrot = jacobi64(a, q, tol, inner_sweeps, cs, pq, flags, red);
   __syncthreads();
   double* Qp = Qbuf + pr.q_off + (long long)it.y * 64 * 64;
   if (nrot > 0)
     for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) Qp[e] = q[(e >> 6) * kLd + (e & 63)];
-  if (threadIdx.x == 0) rotflag[(int)(pr.q_off / 4096) + it.y] = nrot > 0 ? 1 : 0;
+  if (threadIdx.x == 0) {
+    rotflag[(int)(pr.q_off / 4096) + it.y] = nrot > 0 ? 1 : 0;
+    if (work && nrot > 0) atomicAdd(work, 36864ull * (unsigned long long)nrot);
+  }
 }
 
 // A[I_k, I_l] <- Q_k^T A[I_k, I_l] Q_l for pair tiles k <= l
@@ -560,8 +655,9 @@ __global__ void __launch_bounds__(256) k_oa_e(const MolDesc* __restrict__ desc, 
                                               double* E, unsigned long long* stat) {
   __shared__ double sh[4];
   const MolDesc md = desc[blockIdx.y];
-  if (!act[blockIdx.y]) return;
+  if (!act[md.mol]) return;
   const long long N = md.nao;
+  if ((long long)blockIdx.x * blockDim.x >= N * N) return;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double soff = 0.0, rmax = 0.0, emax = 0.0, dmax = 0.0;
   if (t < N * N) {
@@ -598,7 +694,7 @@ __global__ void __launch_bounds__(256) k_oa_e(const MolDesc* __restrict__ desc, 
   emax = block_max<256>(emax, sh);
   dmax = block_max<256>(dmax, sh);
   if (threadIdx.x == 0) {
-    unsigned long long* st = stat + 4LL * blockIdx.y;
+    unsigned long long* st = stat + 4LL * md.mol;
     if (soff > 0.0) atomicMax(&st[0], (unsigned long long)__double_as_longlong(soff));
     if (rmax > 0.0) atomicMax(&st[1], (unsigned long long)__double_as_longlong(rmax));
     if (emax > 0.0) atomicMax(&st[2], (unsigned long long)__double_as_longlong(emax));
@@ -607,9 +703,11 @@ __global__ void __launch_bounds__(256) k_oa_e(const MolDesc* __restrict__ desc, 
 }
 
 // stop refining a molecule when its off-diagonal is at the Jacobi tolerance
-// and X is orthonormal, when no well-separated pair is left to correct, or
-// after maxstep steps; act = 1 keeps it for one more update X <- X + X E
-__global__ void __launch_bounds__(256) k_oa_mark(int nmol, const MolState* __restrict__ mstate, int* act,
+// and X is orthonormal (to 5e-14 N, the rounding level of X^T X), or when no
+// well-separated pair is left to correct (|e_ij| <= 1e-12); the result is then
+// accepted (ok = 1).  After maxstep steps it is rejected: the Jacobi sweeps
+// start from the unrefined matrix.  act = 1 keeps it for one more update.
+__global__ void __launch_bounds__(256) k_oa_mark(int nmol, const MolDesc* __restrict__ desc, int* act, int* ok,
                                                  unsigned long long* stat, int step, int maxstep, int* nactive) {
   __shared__ double sh[4];
   double c = 0.0;
@@ -620,9 +718,15 @@ __global__ void __launch_bounds__(256) k_oa_mark(int nmol, const MolState* __res
       const double rmax = __longlong_as_double((long long)st[1]);
       const double emax = __longlong_as_double((long long)st[2]);
       const double scale = fmax(__longlong_as_double((long long)st[3]), 1.0);
-      const bool done = (soff <= 1e-12 * scale && rmax <= 1e-12) || (emax <= 1e-15 && rmax <= 1e-12) ||
-                        step >= maxstep;
-      if (done) act[m] = 0;
+      const double rtol = 5e-14 * desc[m].nao;
+      const bool conv = rmax <= rtol && (soff <= 1e-12 * scale || emax <= 1e-12);
+      if (conv) {
+        act[m] = 0;
+        ok[m] = 1;
+      } else if (step >= maxstep) {
+        act[m] = 0;
+        ok[m] = 0;
+      }
     }
     for (int k = 0; k < 4; ++k) st[k] = 0ull;
     c += act[m];
@@ -631,11 +735,23 @@ __global__ void __launch_bounds__(256) k_oa_mark(int nmol, const MolState* __res
   if (threadIdx.x == 0) nactive[0] = (int)c;
 }
 
+// accepted refinement: A <- X^T A X (kept in Sm); rejected: W <- I
+__global__ void k_oa_finish(const MolDesc* __restrict__ desc, const MolState* __restrict__ mstate,
+                            const int* __restrict__ ok, const double* __restrict__ Sm, double* A, double* W) {
+  const MolDesc md = desc[blockIdx.y];
+  if (!mstate[md.mol].active) return;
+  const long long N = md.nao;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * N) return;
+  if (ok[md.mol]) A[md.m0 + t] = Sm[md.m0 + t];
+  else W[md.m0 + t] = (t / N == t % N) ? 1.0 : 0.0;
+}
+
 // X <- X + D (D = X E) for refining molecules; with identity = 1, X <- I + E
 __global__ void k_oa_update(const MolDesc* __restrict__ desc, const int* __restrict__ act,
                             const double* __restrict__ D, double* X, int identity) {
   const MolDesc md = desc[blockIdx.y];
-  if (!act[blockIdx.y]) return;
+  if (!act[md.mol]) return;
   const long long N = md.nao;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N * N) return;

@@ -50,6 +50,11 @@ def library():
     lib.xtbamd_batch_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_longlong)]
     lib.xtbamd_batch_counters.restype = ctypes.c_int
+    lib.xtbamd_batch_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.xtbamd_batch_profile.restype = ctypes.c_int
+    lib.xtbamd_batch_profile_result.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    lib.xtbamd_batch_profile_result.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -120,6 +125,22 @@ class Batch:
         a, b = ctypes.c_int(), ctypes.c_longlong()
         self._check(self._lib.xtbamd_batch_counters(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    KERNELS = {"gemm": 1, "jacobi": 2}
+
+    def profile(self, *kernels):
+        """Start HIP-event timing of kernel families ("gemm", "jacobi")."""
+        bits = 0
+        for k in kernels:
+            bits |= 1 << (self.KERNELS[k] - 1)
+        self._check(self._lib.xtbamd_batch_profile(self._h, bits))
+
+    def profile_result(self, kernel="gemm"):
+        """(launches, summed ms, summed algorithmic flops) since profile()."""
+        n, t, w = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        self._check(self._lib.xtbamd_batch_profile_result(self._h, self.KERNELS[kernel], ctypes.byref(n),
+                                                          ctypes.byref(t), ctypes.byref(w)))
+        return n.value, t.value, w.value
 
     def __len__(self):
         return len(self.natoms)
