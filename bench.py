@@ -52,6 +52,46 @@ def cpu_baseline(mols, budget_s, procs):
     return done, wall, energies
 
 
+def serve(args):
+    """Serving workload: Poisson arrivals of seeded water clusters / alkanes of
+    20-400 atoms (log-uniform), continuous batching under an HBM budget;
+    latency = finish - arrival."""
+    import xtb_amd
+    rng = np.random.default_rng(args.seed + 1)
+    reqs = workloads.mixed_batch(args.serve_requests, seed=args.seed + 2, lo=20, hi=400)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_qps, len(reqs)))
+    srv = xtb_amd.Server(hbm_gb=args.serve_hbm_gb, max_atoms=400, max_slots=1024)
+    # warm-up: a few requests through the server before the clock starts
+    for z, x in reqs[:4]:
+        srv.submit(z, x)
+    while srv.step() > 0:
+        pass
+    srv.finished()
+    ids = {}
+    fin = {}
+    t0 = time.perf_counter()
+    nsub = 0
+    while len(fin) < len(reqs):
+        now = time.perf_counter() - t0
+        while nsub < len(reqs) and arrivals[nsub] <= now:
+            z, x = reqs[nsub]
+            ids[srv.submit(z, x)] = nsub
+            nsub += 1
+        left = srv.step()
+        t = time.perf_counter() - t0
+        for rid in srv.finished():
+            fin[ids[rid]] = t
+        if left == 0 and nsub < len(reqs):
+            time.sleep(max(0.0, arrivals[nsub] - (time.perf_counter() - t0)))
+    lat = np.array([fin[i] - arrivals[i] for i in range(len(reqs))])
+    span = max(fin.values()) - arrivals[0]
+    srv.close()
+    return {"requests": len(reqs), "offered_qps": args.serve_qps, "value": len(reqs) / span,
+            "unit": "single-points/s", "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)), "hbm_gb": args.serve_hbm_gb,
+            "workload": "Poisson arrivals, seeded water clusters / alkanes, 20-400 atoms log-uniform"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +103,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-qps", type=float, default=40.0)
+    ap.add_argument("--serve-hbm-gb", type=float, default=16.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -141,7 +184,9 @@ def main():
         "cpu_baseline": cpu,
     }
     names = {"gemm": "k_gemm (batched FP64 GEMM on the matrix cores)",
-             "jacobi": "k_jac_pairs (64x64 pivot problems of the block-Jacobi eigensolver, in LDS)"}
+             "jacobi": os.environ.get("XTBAMD_EIG") == "ql" and
+             "k_tred_ql (tridiagonalisation + QL eigensolver, one workgroup per molecule)" or
+             "k_jac_pairs (64x64 pivot problems of the block-Jacobi eigensolver, in LDS)"}
     lines = {}
     for k, (n, tms, work) in prof.items():
         if not n:
@@ -159,6 +204,8 @@ def main():
     if out["plain"]["value"]:
         out["speedup_batched_vs_plain"] = out["value"] / out["plain"]["value"]
     b.close()
+    if args.serve_requests > 0:
+        out["serving"] = serve(args)
     print(json.dumps(out), flush=True)
 
 

@@ -67,6 +67,29 @@ void xtbamd_batch_free(xtbamd_batch b);
  * per inner round of a 64x64 pivot problem). */
 int xtbamd_batch_profile(xtbamd_batch b, int kernels);
 int xtbamd_batch_profile_result(xtbamd_batch b, int kernel, int* launches, double* total_ms, double* work);
+/* ---- continuous batching (serving) ----
+ * Requests are queued by xtbamd_server_submit and admitted into the running
+ * batch whenever its packed device arrays (sized from hbm_gb for molecules of
+ * up to max_atoms atoms, in up to max_slots slots) have room; each
+ * xtbamd_server_step admits, runs one SCC iteration of every resident
+ * molecule and finishes (gradient, results to the host) those that stopped
+ * iterating.  A request's result is bitwise the same alone and in any mix. */
+typedef struct xtbamd_server_s* xtbamd_server;
+xtbamd_server xtbamd_server_new(double hbm_gb, int max_atoms, int max_slots, double accuracy, double etemp,
+                                int maxiter);
+/* returns the request id (>= 0), -1 on invalid input */
+long long xtbamd_server_submit(xtbamd_server s, int natoms, const int* numbers, const double* positions,
+                               double charge, int uhf);
+/* returns the number of requests not yet finished (queued + running) */
+int xtbamd_server_step(xtbamd_server s);
+/* ids of requests finished since the last call (at most max_ids; the rest on the next call) */
+int xtbamd_server_finished(xtbamd_server s, long long* ids, int max_ids);
+/* copy a finished result; with all array pointers NULL only sizes and scalars
+ * are filled and the result is kept, otherwise it is released */
+int xtbamd_server_result(xtbamd_server s, long long id, xtbamd_result* out);
+int xtbamd_server_pending(xtbamd_server s, int* queued, int* running);
+void xtbamd_server_free(xtbamd_server s);
+
 /* device / runtime information: number of visible GPUs */
 int xtbamd_device_count(void);
 

@@ -157,3 +157,72 @@ def test_rerun_same_batch_repeats_full_single_point():
         assert r1["iterations"] == r2["iterations"] > 1
         assert r1["energy"] == r2["energy"]
         assert np.array_equal(r1["gradient"], r2["gradient"])
+
+
+def _same(ra, rb, tag):
+    assert ra["iterations"] == rb["iterations"], tag
+    assert ra["energy"] == rb["energy"], f"{tag}: {ra['energy']!r} vs {rb['energy']!r}"
+    assert np.array_equal(ra["gradient"], rb["gradient"]), tag
+    assert np.array_equal(ra["charges"], rb["charges"]), tag
+
+
+def test_block_jacobi_batch_same_alone_and_in_mix():
+    """Molecules above 64 AOs (block-Jacobi path) mixed with smaller and
+    larger ones: bitwise the alone results."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    mols = [w.water_cluster(12, 5), w.alkane(20, 6), w.water_cluster(3, 7), w.alkane(45, 8), w.water_cluster(30, 9)]
+    b = xa.Batch([(z, x, 0.0, 0) for z, x in mols]).run()
+    mix = b.results()
+    b.close()
+    for k, ((z, x), rm) in enumerate(zip(mols, mix)):
+        _same(xa.singlepoint(z, x), rm, f"mol{k} nao={rm['nao']}")
+
+
+def test_server_results_same_alone_and_in_mix():
+    """Continuous batching: requests submitted while others iterate give the
+    bitwise alone results; every request finishes."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    mols = w.mixed_batch(14, seed=21, lo=10, hi=150)
+    srv = xa.Server(hbm_gb=2.0, max_atoms=400, max_slots=64)
+    ids = []
+    waves = [mols[:5], mols[5:9], mols[9:]]
+    steps = 0
+    for wave in waves:
+        for z, x in wave:
+            ids.append(srv.submit(z, x))
+        for _ in range(3):
+            srv.step()
+            steps += 1
+    while srv.step() > 0:
+        steps += 1
+        assert steps < 2000
+    done = set()
+    for _ in range(3):
+        done.update(srv.finished())
+    assert done == set(ids)
+    for rid, (z, x) in zip(ids, mols):
+        rs = srv.result(rid)
+        assert rs["converged"]
+        _same(xa.singlepoint(z, x), rs, f"request {rid} n={len(z)}")
+    srv.close()
+
+
+def test_server_admits_by_budget():
+    """A budget that holds only a few molecules at a time still serves a
+    long queue, and the results agree with the oracle."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    mols = [w.water_cluster(k, 30 + k) for k in (4, 9, 6, 12, 5, 8)]
+    srv = xa.Server(hbm_gb=0.02, max_atoms=40, max_slots=8)
+    ids = [srv.submit(z, x) for z, x in mols]
+    peak = 0
+    while srv.step() > 0:
+        peak = max(peak, srv.pending()[1])
+    assert 1 <= peak < len(mols)
+    got = set(srv.finished())
+    assert got == set(ids)
+    for rid, (z, x) in zip(ids, mols):
+        _cmp(srv.result(rid), po.singlepoint(z, x), f"budget request {rid}")
+    srv.close()

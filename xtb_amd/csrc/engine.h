@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <deque>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -31,6 +33,7 @@ struct MolDesc {
   int nbr, nel, nopen, maxiter;
   double charge;
   int mol;                      // index of the molecule (slot) in the batch
+  long long y0;                 // offset of the EEQ vectors (n+1)
 };
 
 // Per-molecule SCC state machine (src/scc_core.f90:434-640).
@@ -62,6 +65,27 @@ struct MoleculeInput {
   int uhf = 0;
 };
 
+// Host-side index maps of one molecule (local indices).
+struct HostMol {
+  std::vector<int> at, sh_at, sh_l, sh_ao0, sh_m, at_sh0, at_ao0, ao_at, ao_sh, d4_at, d4_ref;
+  std::vector<double> xyz;
+};
+
+// First-fit allocator of ranges in [0, cap) (one per packed device array).
+struct Arena {
+  long long cap = 0;
+  std::vector<std::pair<long long, long long>> free_;   // sorted [begin, end)
+  void init(long long c);
+  long long take(long long n);                          // -1 when no range fits
+  void give(long long off, long long n);
+};
+
+// Capacities of the packed arrays (elements), and the number of slots.
+struct Capacity {
+  int slots = 0;
+  long long n = 0, nsh = 0, nao = 0, nd4 = 0, mm = 0, jj = 0, xx = 0, pp = 0, ee = 0, ne = 0, v = 0, h = 0, g = 0;
+};
+
 // Per-molecule results copied back to the host.
 struct MolResult {
   int n = 0, nsh = 0, nao = 0, iterations = 0, converged = 0, failed = 0, ihomo = 0;
@@ -89,6 +113,8 @@ struct JacSet {
   int2* d_pairs = nullptr;
   int4 *d_ta = nullptr, *d_tw = nullptr, *d_to = nullptr;
   int nto = 0;
+  int* d_big = nullptr;   // problems above 64 (tridiagonal QL path)
+  int nbig = 0, big_max_dim = 0;
   unsigned long long* offmax = nullptr;
   double* qbuf = nullptr;
   int* rot = nullptr;
@@ -100,7 +126,10 @@ struct JacSet {
 
 class Batch {
  public:
+  // a batch of fixed molecules (packed layout)
   Batch(const std::vector<MoleculeInput>& mols, const Options& opt, hipStream_t stream = nullptr);
+  // an empty batch of cap.slots slots for continuous batching (admit/release)
+  Batch(const Capacity& cap, const Options& opt, hipStream_t stream = nullptr);
   ~Batch();
   Batch(const Batch&) = delete;
   Batch& operator=(const Batch&) = delete;
@@ -117,15 +146,23 @@ class Batch {
   // quadrupoles [n][6] of a previous wavefunction (used by the next setup)
   void set_guess(int m, const double* qsh_in, const double* dipm_in, const double* qp_in);
   MolResult result(int m, bool matrices = false) const;
-  int size() const { return (int)mols_.size(); }
+  int size() const { return nmol; }
+  // sizes and index maps of one molecule; throws for invalid input
+  static void describe(const MoleculeInput& mi, int maxiter, MolDesc& d, HostMol& hm, MolState& st);
+  // footprint of one molecule in each packed array
+  static Capacity footprint(const MolDesc& d);
+  // place a molecule into a free slot (device arrays, index maps); -1 when
+  // a packed array has no room.  rebuild() must follow before the next step.
+  int admit(const MoleculeInput& mi);
+  void release(int m);
+  void rebuild();
+  bool occupied(int m) const { return occupied_[m] != 0; }
   const std::string& error() const { return error_; }
 
   // --- layout (public for the kernel launchers) ---
   Options opt;
   hipStream_t stream;
   int nmol = 0, max_n = 0, max_nsh = 0, max_nao = 0, max_nd4 = 0, max_nbr = 0;
-  int tot_n = 0, tot_nsh = 0, tot_nao = 0, tot_nd4 = 0;
-  long long tot_mm = 0, tot_jj = 0, tot_xx = 0, tot_pp = 0, tot_ee = 0, tot_v = 0, tot_h = 0, tot_g = 0;
   double intcut = 25.0, neglect = 1e-8, scfconv = 1e-6, qconv = 1e-4;
   std::vector<MolDesc> desc;
   MolDesc* d_desc = nullptr;
@@ -135,7 +172,16 @@ class Batch {
   int* phase_act = nullptr;     // 1 for selected molecules (per slot)
   std::vector<int> sel_;
   MolState* d_state = nullptr;
-  std::vector<MoleculeInput> mols_;
+  Capacity cap_;
+  Arena ar_[13];
+  std::vector<char> occupied_;
+  std::vector<HostMol> hmol_;
+  std::vector<char> elem_present_;
+  long long tiles_cap_ = 0, small_cap_ = 0, pairs_cap_ = 0, ta_cap_ = 0, tw_cap_ = 0, to_cap_ = 0, big_cap_ = 0;
+  void allocate(const Capacity& c);
+  void init_common();
+  void place(int m, const MolDesc& d, const HostMol& hm, const MolState& st, bool upload);
+  void update_c6ref();
   std::string error_;
 
   // atom vectors
@@ -164,7 +210,7 @@ class Batch {
   // other matrices
   double *J = nullptr, *dispmat = nullptr, *gab3 = nullptr, *gab5 = nullptr;
   double *c6 = nullptr, *dc6 = nullptr, *c60 = nullptr, *dc60 = nullptr;
-  double *E = nullptr, *evE = nullptr, *xE = nullptr;
+  double *E = nullptr, *xE = nullptr;
   // Broyden
   double *q_in = nullptr, *dq = nullptr, *qlast = nullptr, *dqlast = nullptr;
   double *bdf = nullptr, *bu = nullptr, *ba = nullptr, *bomega = nullptr;
@@ -184,6 +230,10 @@ class Batch {
   int ntiles = 0;
   int4* d_tiles = nullptr;
   void jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w = false);
+  // eigenvectors as rows of W: LDS Jacobi up to 64, tridiagonal QL above
+  void eigh_rows(JacSet& js, double* Am, double* Wm, double* evout);
+  bool use_ql = false;     // tridiagonal QL (rows of W) instead of block Jacobi (columns of W)
+  int* eig_failed = nullptr;
   // refine W (init I) towards the eigenvectors of A; leaves A <- W^T A W
   void refine(double* Am, double* Wm);
   int oa_max_steps = 0;     // refinement steps before the Jacobi sweeps (0 = off)
@@ -213,7 +263,6 @@ class Batch {
   int prof_result(int kind, int* launches, double* total_ms, double* work);
   void prof_enable(int mask);
   std::vector<void*> allocs_;
-  std::vector<int> h_sh_at_;
   std::vector<MolState> h_state0_;   // initial SCC state, restored by every setup()
   std::vector<std::vector<double>> guess_qsh_, guess_dipm_, guess_qp_;
   bool setup_done_ = false;
@@ -221,5 +270,37 @@ class Batch {
   template <class T_>
   T_* alloc(size_t n);
 };
+
+// Continuous batching: requests are admitted into batch slots as packed-array
+// room (the HBM budget) frees up; each step runs one SCC iteration of every
+// resident molecule, whatever its iteration; finished molecules get their
+// gradient, leave their results on the host and free their slots.
+class Server {
+ public:
+  Server(const Capacity& cap, const Options& opt);
+  long long submit(const MoleculeInput& mi);
+  // admission + one SCC iteration + completion; returns requests not finished
+  int step();
+  // ids finished since the last call
+  std::vector<long long> take_finished();
+  bool result(long long id, MolResult& out, bool erase = true);
+  int running() const;
+  int queued() const { return (int)queue_.size(); }
+  Batch batch;
+
+ private:
+  std::deque<std::pair<long long, MoleculeInput>> queue_;
+  std::vector<long long> slot_req_;
+  std::map<long long, MolResult> done_;
+  std::vector<long long> finished_;
+  std::vector<int> active_host_;
+  int* d_active_ = nullptr;
+  long long next_id_ = 0;
+};
+
+// capacity for continuous batching from an HBM budget: the packed arrays hold
+// hbm_bytes worth of molecules of up to max_atoms atoms (at most 4 AOs,
+// 2 shells and 7 D4 references per atom), in up to max_slots slots
+Capacity server_capacity(double hbm_bytes, int max_atoms, int max_slots, int maxiter);
 
 }  // namespace xtbamd

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "engine.h"
@@ -59,8 +60,8 @@ __global__ void __launch_bounds__(256) k_eeq_gesv(const MolDesc* __restrict__ de
   const MolDesc md = desc[blockIdx.x];
   const int ne = md.ne, n = md.n;
   double* M = E + md.e0;
-  double* b = xE + md.a0 + md.mol;
-  double* y = yE + md.a0 + md.mol;
+  double* b = xE + md.y0;
+  double* y = yE + md.y0;
   const int tid = threadIdx.x;
   bool ok = true;
   for (int k = 0; k < ne; ++k) {
@@ -365,251 +366,200 @@ T_* Batch::alloc(size_t n) {
   return (T_*)p;
 }
 
-Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream_t s)
-    : opt(o), stream(s), mols_(mols) {
-  // XTBAMD_GEMM=mfma selects the FP64 matrix-core GEMM instead of VALU FMA
-  if (const char* g = std::getenv("XTBAMD_GEMM")) use_mfma = std::string(g) == "mfma";
+// --------------------------------------------------------------------------
+// packed-array allocator
+void Arena::init(long long c) {
+  cap = c;
+  free_.clear();
+  if (c > 0) free_.push_back({0, c});
+}
+
+long long Arena::take(long long n) {
+  if (n <= 0) return 0;
+  for (size_t i = 0; i < free_.size(); ++i) {
+    const long long b = free_[i].first, e = free_[i].second;
+    if (e - b >= n) {
+      if (e - b == n) free_.erase(free_.begin() + i);
+      else free_[i].first = b + n;
+      return b;
+    }
+  }
+  return -1;
+}
+
+void Arena::give(long long off, long long n) {
+  if (n <= 0) return;
+  auto it = free_.begin();
+  while (it != free_.end() && it->first < off) ++it;
+  it = free_.insert(it, {off, off + n});
+  // merge with the next and the previous range
+  if (it + 1 != free_.end() && it->second == (it + 1)->first) {
+    it->second = (it + 1)->second;
+    free_.erase(it + 1);
+  }
+  if (it != free_.begin() && (it - 1)->second == it->first) {
+    (it - 1)->second = it->second;
+    free_.erase(it);
+  }
+}
+
+// --------------------------------------------------------------------------
+void Batch::describe(const MoleculeInput& mi, int maxiter, MolDesc& d, HostMol& hm, MolState& st) {
+  const Element* el = host_elements();
+  std::memset(&d, 0, sizeof(d));
+  hm = HostMol();
+  d.n = (int)mi.numbers.size();
+  if (d.n < 1) throw std::runtime_error("molecule without atoms");
+  if ((int)mi.xyz.size() != 3 * d.n) throw std::runtime_error("coordinate array has the wrong size");
+  double zsum = 0.0;
+  for (int i = 0; i < d.n; ++i) {
+    const int z = mi.numbers[i];
+    if (z < 1 || z > XTB_MAX_Z) throw std::runtime_error("element not parametrised in GFN2-xTB");
+    const Element& e = el[z];
+    hm.at.push_back(z);
+    hm.at_sh0.push_back(d.nsh);
+    hm.at_ao0.push_back(d.nao);
+    for (int s = 0; s < e.nsh; ++s) {
+      hm.sh_at.push_back(i);
+      hm.sh_l.push_back(e.l[s]);
+      hm.sh_ao0.push_back(d.nao);
+      hm.sh_m.push_back(s);
+      for (int k = 0; k < 2 * e.l[s] + 1; ++k) {
+        hm.ao_at.push_back(i);
+        hm.ao_sh.push_back(d.nsh);
+      }
+      d.nsh += 1;
+      d.nao += 2 * e.l[s] + 1;
+    }
+    for (int r = 0; r < e.nref; ++r) {
+      hm.d4_at.push_back(i);
+      hm.d4_ref.push_back(r);
+    }
+    d.nd4 += e.nref;
+    zsum += e.zval;
+    for (int k = 0; k < 3; ++k) hm.xyz.push_back(mi.xyz[3 * i + k]);
+  }
+  d.ne = d.n + 1;
+  d.charge = mi.charge;
+  d.nel = (int)std::lround(zsum - mi.charge);
+  d.nopen = mi.uhf;
+  if (d.nopen == 0 && d.nel % 2 != 0) d.nopen = 1;
+  if (d.nel > 2 * d.nao) throw std::runtime_error("Not enough basis functions for filling orbitals");
+  d.nbr = d.nsh + 9 * d.n;
+  d.maxiter = maxiter;
+  std::memset(&st, 0, sizeof(st));
+  st.active = 1;
+  st.damp = 0.20;
+  host_occu(d.nao, d.nel, d.nopen, st.ihomoa, st.ihomob);
+  st.ihomo = st.ihomoa;
+}
+
+Capacity Batch::footprint(const MolDesc& d) {
+  Capacity c;
+  const long long mx = d.maxiter;
+  c.slots = 1;
+  c.n = d.n;
+  c.nsh = d.nsh;
+  c.nao = d.nao;
+  c.nd4 = d.nd4;
+  c.mm = (long long)d.nao * d.nao;
+  c.jj = (long long)d.nsh * d.nsh;
+  c.xx = (long long)d.nd4 * d.nd4;
+  c.pp = (long long)d.n * d.n;
+  c.ee = (long long)d.ne * d.ne;
+  c.ne = d.ne;
+  c.v = d.nbr;
+  c.h = mx * d.nbr;
+  c.g = 3 * mx * mx + 4 * mx;
+  return c;
+}
+
+static long long* cap_field(Capacity& c, int k) {
+  long long* f[13] = {&c.n, &c.nsh, &c.nao, &c.nd4, &c.mm, &c.jj, &c.xx, &c.pp, &c.ee, &c.ne, &c.v, &c.h, &c.g};
+  return f[k];
+}
+
+void Batch::init_common() {
+  if (const char* g = std::getenv("XTBAMD_GEMM")) use_mfma = std::string(g) != "valu";
   if (const char* r = std::getenv("XTBAMD_REFINE_STEPS")) oa_max_steps = std::atoi(r);
   if (const char* r = std::getenv("XTBAMD_INNER_SWEEPS")) inner_sweeps = std::max(1, std::atoi(r));
-  const Element* el = host_elements();
-  nmol = (int)mols.size();
-  if (nmol == 0) throw std::runtime_error("empty batch");
+  // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
+  if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
   intcut = std::max(20.0, 25.0 - 10.0 * std::log10(opt.accuracy));
   neglect = 10.0e-9 * opt.accuracy;
   scfconv = 1.e-6 * opt.accuracy;
   qconv = 1.e-4 * opt.accuracy;
-  desc.resize(nmol);
-  std::vector<int> h_at, h_sh_at, h_sh_l, h_sh_ao0, h_sh_m, h_at_sh0, h_at_ao0, h_ao_at, h_ao_sh, h_d4_at, h_d4_ref;
-  std::vector<double> h_xyz;
-  std::vector<MolState> h_state(nmol);
+}
+
+void Batch::allocate(const Capacity& c) {
+  cap_ = c;
+  nmol = c.slots;
+  for (int k = 0; k < 13; ++k) ar_[k].init(*cap_field(cap_, k));
+  const long long tn = std::max(1LL, c.n), tsh = std::max(1LL, c.nsh), tao = std::max(1LL, c.nao);
+  const long long tmm = std::max(1LL, c.mm);
   const int mx = opt.maxiter;
+  desc.assign(nmol, MolDesc());
   for (int m = 0; m < nmol; ++m) {
-    const MoleculeInput& mi = mols[m];
-    MolDesc& d = desc[m];
-    std::memset(&d, 0, sizeof(d));
-    d.n = (int)mi.numbers.size();
-    if (d.n < 1) throw std::runtime_error("molecule without atoms");
-    if ((int)mi.xyz.size() != 3 * d.n) throw std::runtime_error("coordinate array has the wrong size");
-    d.a0 = tot_n;
-    d.s0 = tot_nsh;
-    d.o0 = tot_nao;
-    d.d0 = tot_nd4;
-    double zsum = 0.0;
-    for (int i = 0; i < d.n; ++i) {
-      const int z = mi.numbers[i];
-      if (z < 1 || z > XTB_MAX_Z) throw std::runtime_error("element not parametrised in GFN2-xTB");
-      const Element& e = el[z];
-      h_at.push_back(z);
-      h_at_sh0.push_back(d.nsh);
-      h_at_ao0.push_back(d.nao);
-      for (int s = 0; s < e.nsh; ++s) {
-        h_sh_at.push_back(i);
-        h_sh_l.push_back(e.l[s]);
-        h_sh_ao0.push_back(d.nao);
-        h_sh_m.push_back(s);
-        for (int k = 0; k < 2 * e.l[s] + 1; ++k) {
-          h_ao_at.push_back(i);
-          h_ao_sh.push_back(d.nsh);
-        }
-        d.nsh += 1;
-        d.nao += 2 * e.l[s] + 1;
-      }
-      for (int r = 0; r < e.nref; ++r) {
-        h_d4_at.push_back(i);
-        h_d4_ref.push_back(r);
-      }
-      d.nd4 += e.nref;
-      zsum += e.zval;
-      for (int k = 0; k < 3; ++k) h_xyz.push_back(mi.xyz[3 * i + k]);
-    }
-    d.ne = d.n + 1;
-    d.mol = m;
-    d.charge = mi.charge;
-    d.nel = (int)std::lround(zsum - mi.charge);
-    d.nopen = mi.uhf;
-    if (d.nopen == 0 && d.nel % 2 != 0) d.nopen = 1;
-    if (d.nel > 2 * d.nao) throw std::runtime_error("Not enough basis functions for filling orbitals");
-    d.nbr = d.nsh + 9 * d.n;
-    d.maxiter = mx;
-    d.m0 = tot_mm;
-    d.j0 = tot_jj;
-    d.x0 = tot_xx;
-    d.p0 = tot_pp;
-    d.e0 = tot_ee;
-    d.v0 = tot_v;
-    d.h0 = tot_h;
-    d.g0 = tot_g;
-    tot_n += d.n;
-    tot_nsh += d.nsh;
-    tot_nao += d.nao;
-    tot_nd4 += d.nd4;
-    tot_mm += (long long)d.nao * d.nao;
-    tot_jj += (long long)d.nsh * d.nsh;
-    tot_xx += (long long)d.nd4 * d.nd4;
-    tot_pp += (long long)d.n * d.n;
-    tot_ee += (long long)d.ne * d.ne;
-    tot_v += d.nbr;
-    tot_h += (long long)mx * d.nbr;
-    tot_g += 3LL * mx * mx + 4LL * mx;
-    max_n = std::max(max_n, d.n);
-    max_nsh = std::max(max_nsh, d.nsh);
-    max_nao = std::max(max_nao, d.nao);
-    max_nd4 = std::max(max_nd4, d.nd4);
-    max_nbr = std::max(max_nbr, d.nbr);
-    MolState& st = h_state[m];
-    std::memset(&st, 0, sizeof(st));
-    st.active = 1;
-    st.damp = 0.20;
-    host_occu(d.nao, d.nel, d.nopen, st.ihomoa, st.ihomob);
-    st.ihomo = st.ihomoa;
+    std::memset(&desc[m], 0, sizeof(MolDesc));
+    desc[m].mol = m;
   }
-  // device allocations
+  occupied_.assign(nmol, 0);
+  hmol_.assign(nmol, HostMol());
+  elem_present_.assign(XTB_MAX_Z + 1, 0);
+  h_state0_.assign(nmol, MolState());
+  for (auto& st : h_state0_) std::memset(&st, 0, sizeof(st));
   d_desc = alloc<MolDesc>(nmol);
   d_state = alloc<MolState>(nmol);
-  at = alloc<int>(tot_n);
-  xyz = alloc<double>(3 * tot_n);
-  cn = alloc<double>(tot_n); cn_d4 = alloc<double>(tot_n); cn_eeq = alloc<double>(tot_n); cn_cov = alloc<double>(tot_n);
-  q = alloc<double>(tot_n); dipm = alloc<double>(3 * tot_n); qp = alloc<double>(6 * tot_n); radcn = alloc<double>(tot_n);
-  vs = alloc<double>(tot_n); vd = alloc<double>(3 * tot_n); vq = alloc<double>(6 * tot_n); ashift = alloc<double>(tot_n);
-  grad = alloc<double>(3 * tot_n); dhdcn = alloc<double>(tot_n); dedcn = alloc<double>(tot_n);
-  tmp2 = alloc<double>(tot_n); eat = alloc<double>(tot_n); ae01 = alloc<double>(4 * tot_n);
-  sh_at = alloc<int>(tot_nsh); sh_l = alloc<int>(tot_nsh); sh_ao0 = alloc<int>(tot_nsh); sh_m = alloc<int>(tot_nsh);
-  at_sh0 = alloc<int>(tot_n); at_ao0 = alloc<int>(tot_n);
-  qsh = alloc<double>(tot_nsh); zsh = alloc<double>(tot_nsh); shift = alloc<double>(tot_nsh);
-  shgam = alloc<double>(tot_nsh); selfe = alloc<double>(tot_nsh); dsedcn = alloc<double>(tot_nsh);
-  jq = alloc<double>(tot_nsh);
-  ao_at = alloc<int>(tot_nao); ao_sh = alloc<int>(tot_nao);
-  emo = alloc<double>(tot_nao); focc = alloc<double>(tot_nao); focca = alloc<double>(tot_nao);
-  foccb = alloc<double>(tot_nao); ev = alloc<double>(tot_nao);
-  row_q = alloc<double>(tot_nao); row_e = alloc<double>(tot_nao); row_d = alloc<double>(3 * tot_nao);
-  row_qp = alloc<double>(6 * tot_nao); row_pd = alloc<double>(tot_nao);
-  d4_at = alloc<int>(tot_nd4); d4_ref = alloc<int>(tot_nd4);
-  gw = alloc<double>(tot_nd4); zv = alloc<double>(tot_nd4); dzv = alloc<double>(tot_nd4); d4dum = alloc<double>(tot_nd4);
+  at = alloc<int>(tn);
+  xyz = alloc<double>(3 * tn);
+  cn = alloc<double>(tn); cn_d4 = alloc<double>(tn); cn_eeq = alloc<double>(tn); cn_cov = alloc<double>(tn);
+  q = alloc<double>(tn); dipm = alloc<double>(3 * tn); qp = alloc<double>(6 * tn); radcn = alloc<double>(tn);
+  vs = alloc<double>(tn); vd = alloc<double>(3 * tn); vq = alloc<double>(6 * tn); ashift = alloc<double>(tn);
+  grad = alloc<double>(3 * tn); dhdcn = alloc<double>(tn); dedcn = alloc<double>(tn);
+  tmp2 = alloc<double>(tn); eat = alloc<double>(tn); ae01 = alloc<double>(4 * tn);
+  sh_at = alloc<int>(tsh); sh_l = alloc<int>(tsh); sh_ao0 = alloc<int>(tsh); sh_m = alloc<int>(tsh);
+  at_sh0 = alloc<int>(tn); at_ao0 = alloc<int>(tn);
+  qsh = alloc<double>(tsh); zsh = alloc<double>(tsh); shift = alloc<double>(tsh);
+  shgam = alloc<double>(tsh); selfe = alloc<double>(tsh); dsedcn = alloc<double>(tsh);
+  jq = alloc<double>(tsh);
+  ao_at = alloc<int>(tao); ao_sh = alloc<int>(tao);
+  emo = alloc<double>(tao); focc = alloc<double>(tao); focca = alloc<double>(tao);
+  foccb = alloc<double>(tao); ev = alloc<double>(tao);
+  row_q = alloc<double>(tao); row_e = alloc<double>(tao); row_d = alloc<double>(3 * tao);
+  row_qp = alloc<double>(6 * tao); row_pd = alloc<double>(tao);
+  d4_at = alloc<int>(std::max(1LL, c.nd4)); d4_ref = alloc<int>(std::max(1LL, c.nd4));
+  gw = alloc<double>(std::max(1LL, c.nd4)); zv = alloc<double>(std::max(1LL, c.nd4));
+  dzv = alloc<double>(std::max(1LL, c.nd4)); d4dum = alloc<double>(std::max(1LL, c.nd4));
   c6ref = alloc<double>((size_t)(XTB_MAX_Z + 1) * (XTB_MAX_Z + 1) * 49);
-  zetavec = alloc<double>(XTB_MAX_REF * tot_n); zerovec = alloc<double>(XTB_MAX_REF * tot_n);
-  zetadcn = alloc<double>(XTB_MAX_REF * tot_n); zerodcn = alloc<double>(XTB_MAX_REF * tot_n);
-  S = alloc<double>(tot_mm); H0 = alloc<double>(tot_mm); H0n = alloc<double>(tot_mm);
-  D = alloc<double>(3 * tot_mm); Q = alloc<double>(6 * tot_mm);
-  H = alloc<double>(tot_mm); C = alloc<double>(tot_mm); T = alloc<double>(tot_mm);
-  A = alloc<double>(tot_mm); W = alloc<double>(tot_mm); P = alloc<double>(tot_mm);
-  J = alloc<double>(tot_jj); dispmat = alloc<double>(tot_xx);
-  gab3 = alloc<double>(tot_pp); gab5 = alloc<double>(tot_pp);
-  c6 = alloc<double>(tot_pp); dc6 = alloc<double>(tot_pp); c60 = alloc<double>(tot_pp); dc60 = alloc<double>(tot_pp);
-  E = alloc<double>(tot_ee);
-  evE = alloc<double>(tot_n + nmol); xE = alloc<double>(tot_n + nmol);
-  q_in = alloc<double>(tot_v); dq = alloc<double>(tot_v); qlast = alloc<double>(tot_v); dqlast = alloc<double>(tot_v);
-  bdf = alloc<double>(tot_h); bu = alloc<double>(tot_h); ba = alloc<double>(tot_g);
+  HIPCHECK(hipMemset(c6ref, 0, (size_t)(XTB_MAX_Z + 1) * (XTB_MAX_Z + 1) * 49 * sizeof(double)));
+  zetavec = alloc<double>(XTB_MAX_REF * tn); zerovec = alloc<double>(XTB_MAX_REF * tn);
+  zetadcn = alloc<double>(XTB_MAX_REF * tn); zerodcn = alloc<double>(XTB_MAX_REF * tn);
+  S = alloc<double>(tmm); H0 = alloc<double>(tmm); H0n = alloc<double>(tmm);
+  D = alloc<double>(3 * tmm); Q = alloc<double>(6 * tmm);
+  H = alloc<double>(tmm); C = alloc<double>(tmm); T = alloc<double>(tmm);
+  A = alloc<double>(tmm); W = alloc<double>(tmm); P = alloc<double>(tmm);
+  J = alloc<double>(std::max(1LL, c.jj)); dispmat = alloc<double>(std::max(1LL, c.xx));
+  const long long tpp = std::max(1LL, c.pp);
+  gab3 = alloc<double>(tpp); gab5 = alloc<double>(tpp);
+  c6 = alloc<double>(tpp); dc6 = alloc<double>(tpp); c60 = alloc<double>(tpp); dc60 = alloc<double>(tpp);
+  wbo = alloc<double>(tpp);
+  E = alloc<double>(std::max(1LL, c.ee));
+  xE = alloc<double>(std::max(1LL, c.ne)); yE = alloc<double>(std::max(1LL, c.ne));
+  const long long tv = std::max(1LL, c.v);
+  q_in = alloc<double>(tv); dq = alloc<double>(tv); qlast = alloc<double>(tv); dqlast = alloc<double>(tv);
+  bdf = alloc<double>(std::max(1LL, c.h)); bu = alloc<double>(std::max(1LL, c.h)); ba = alloc<double>(std::max(1LL, c.g));
   bomega = alloc<double>((long long)nmol * mx);
-  // reference C6 of all element pairs present
-  {
-    std::vector<int> present(XTB_MAX_Z + 1, 0);
-    for (int z : h_at) present[z] = 1;
-    std::vector<double> tab((size_t)(XTB_MAX_Z + 1) * (XTB_MAX_Z + 1) * 49, 0.0);
-    for (int za = 1; za <= XTB_MAX_Z; ++za) {
-      if (!present[za]) continue;
-      for (int zb = 1; zb <= XTB_MAX_Z; ++zb) {
-        if (!present[zb]) continue;
-        for (int r = 0; r < el[za].nref; ++r)
-          for (int s2 = 0; s2 < el[zb].nref; ++s2)
-            tab[((size_t)za * (XTB_MAX_Z + 1) + zb) * 49 + r * 7 + s2] = host_ref_c6(za, r, zb, s2);
-      }
-    }
-    HIPCHECK(hipMemcpy(c6ref, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-  }
-  auto up_i = [&](int* d_, const std::vector<int>& h) {
-    if (!h.empty()) HIPCHECK(hipMemcpy(d_, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
-  };
-  up_i(at, h_at); up_i(sh_at, h_sh_at); up_i(sh_l, h_sh_l); up_i(sh_ao0, h_sh_ao0); up_i(sh_m, h_sh_m);
-  up_i(at_sh0, h_at_sh0); up_i(at_ao0, h_at_ao0); up_i(ao_at, h_ao_at); up_i(ao_sh, h_ao_sh);
-  up_i(d4_at, h_d4_at); up_i(d4_ref, h_d4_ref);
-  h_sh_at_ = h_sh_at;
-  HIPCHECK(hipMemcpy(xyz, h_xyz.data(), h_xyz.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHECK(hipMemcpy(d_desc, desc.data(), nmol * sizeof(MolDesc), hipMemcpyHostToDevice));
-  h_state0_ = h_state;
-  Element* d_el = alloc<Element>(XTB_MAX_Z + 1);
-  HIPCHECK(hipMemcpy(d_el, el, (XTB_MAX_Z + 1) * sizeof(Element), hipMemcpyHostToDevice));
-  elements_dev = d_el;
-  // GEMM tiles over (molecule, 64x64 tile)
-  std::vector<int4> tl;
-  for (int m = 0; m < nmol; ++m) {
-    const int nt = (desc[m].nao + 63) / 64;
-    for (int a = 0; a < nt; ++a)
-      for (int b = 0; b < nt; ++b) tl.push_back(make_int4(m, a, b, 0));
-  }
-  ntiles = (int)tl.size();
-  d_tiles = alloc<int4>(ntiles);
-  HIPCHECK(hipMemcpy(d_tiles, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice));
-  // Jacobi problem sets: S and H (dim nao, matrices A/W at m0), EEQ (dim ne at e0)
-  auto make_set = [&](bool eeq, JacSet& js) {
-    long long qoff = 0;
-    for (int m = 0; m < nmol; ++m) {
-      JacobiProb p;
-      std::memset(&p, 0, sizeof(p));
-      p.dim = eeq ? desc[m].ne : desc[m].nao;
-      p.nb = (p.dim + 31) / 32;
-      p.nbp = p.nb + (p.nb & 1);
-      p.rounds = std::max(1, p.nbp - 1);
-      p.a_off = eeq ? desc[m].e0 : desc[m].m0;
-      p.w_off = p.a_off;
-      p.ev_off = eeq ? desc[m].a0 + m : desc[m].o0;
-      p.q_off = qoff;
-      if (p.dim > 64) qoff += (long long)(p.nbp / 2) * 4096;
-      js.probs.push_back(p);
-    }
-    js.qsize = std::max<long long>(qoff, 4096);
-    js.d_probs = alloc<JacobiProb>(nmol);
-    HIPCHECK(hipMemcpy(js.d_probs, js.probs.data(), nmol * sizeof(JacobiProb), hipMemcpyHostToDevice));
-    std::vector<int> small;
-    std::vector<int2> pairs;
-    std::vector<int4> ta, tw, to;
-    js.max_rounds = 0;
-    for (int m = 0; m < nmol; ++m) {
-      const JacobiProb& p = js.probs[m];
-      if (p.dim <= 64) { small.push_back(m); continue; }
-      js.max_rounds = std::max(js.max_rounds, p.rounds);
-      const int np = p.nbp / 2;
-      for (int k = 0; k < np; ++k) pairs.push_back(make_int2(m, k));
-      for (int k = 0; k < np; ++k)
-        for (int l = k; l < np; ++l) ta.push_back(make_int4(m, k, l, 0));
-      const int nrb = (p.dim + 63) / 64;
-      for (int rb = 0; rb < nrb; ++rb)
-        for (int l = 0; l < np; ++l) tw.push_back(make_int4(m, rb, l, 0));
-      for (int a = 0; a < nrb; ++a)
-        for (int b = a; b < nrb; ++b) to.push_back(make_int4(m, a, b, 0));
-    }
-    js.nsmall = (int)small.size();
-    js.npairs = (int)pairs.size();
-    js.nta = (int)ta.size();
-    js.ntw = (int)tw.size();
-    js.nto = (int)to.size();
-    js.d_small = alloc<int>(small.size());
-    js.d_pairs = alloc<int2>(pairs.size());
-    js.d_ta = alloc<int4>(ta.size());
-    js.d_tw = alloc<int4>(tw.size());
-    js.d_to = alloc<int4>(to.size());
-    if (!small.empty()) HIPCHECK(hipMemcpy(js.d_small, small.data(), small.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (!pairs.empty()) HIPCHECK(hipMemcpy(js.d_pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
-    if (!ta.empty()) HIPCHECK(hipMemcpy(js.d_ta, ta.data(), ta.size() * sizeof(int4), hipMemcpyHostToDevice));
-    if (!tw.empty()) HIPCHECK(hipMemcpy(js.d_tw, tw.data(), tw.size() * sizeof(int4), hipMemcpyHostToDevice));
-    if (!to.empty()) HIPCHECK(hipMemcpy(js.d_to, to.data(), to.size() * sizeof(int4), hipMemcpyHostToDevice));
-    js.offmax = alloc<unsigned long long>(nmol);
-    js.qbuf = alloc<double>(js.qsize);
-    js.rot = alloc<int>(js.qsize / 4096 + 1);
-    js.state = alloc<int>(nmol * kJacState);
-    js.scale = alloc<double>(nmol);
-    js.perm = nullptr;
-  };
-  make_set(false, jac_n);
+  erep = alloc<double>(tn);
+  perm = alloc<int>(tao);
+  Sm = alloc<double>(tmm);
+  G = alloc<double>(tmm);
   jac_nactive = alloc<int>(1);
-  yE = alloc<double>(tot_n + nmol);
-  erep = alloc<double>(tot_n);
-  perm = alloc<int>(tot_nao + tot_n + nmol);
   eeq_failed = alloc<int>(nmol);
+  eig_failed = alloc<int>(nmol);
+  HIPCHECK(hipMemset(eig_failed, 0, nmol * sizeof(int)));
   d_sub = alloc<MolDesc>(nmol);
   phase_act = alloc<int>(nmol);
   ns_c = alloc<double>(nmol);
-  Sm = alloc<double>(tot_mm);
-  G = alloc<double>(tot_mm);
   oa_act = alloc<int>(nmol);
   oa_ok = alloc<int>(nmol);
   oa_stat = alloc<unsigned long long>(4LL * nmol);
@@ -617,9 +567,276 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
   ns_stage = alloc<int>(nmol);
   ns_act = alloc<int>(nmol);
   ns_failed = alloc<int>(nmol);
-  wbo = alloc<double>(tot_pp);
-  (void)evE;
+  // Jacobi set: per-slot problem table and state; work lists grow on demand
+  JacSet& js = jac_n;
+  js.probs.assign(nmol, JacobiProb());
+  js.d_probs = alloc<JacobiProb>(nmol);
+  js.offmax = alloc<unsigned long long>(nmol);
+  js.state = alloc<int>((long long)nmol * kJacState);
+  js.scale = alloc<double>(nmol);
+  // pivot-factor buffer: at most (nao/64 + 1) pairs per molecule
+  js.qsize = std::max(4096LL, (c.nao / 64 + 2LL * nmol) * 4096LL);
+  js.qbuf = alloc<double>(js.qsize);
+  js.rot = alloc<int>(js.qsize / 4096 + 1);
+  js.perm = nullptr;
+  Element* d_el = alloc<Element>(XTB_MAX_Z + 1);
+  HIPCHECK(hipMemcpy(d_el, host_elements(), (XTB_MAX_Z + 1) * sizeof(Element), hipMemcpyHostToDevice));
+  elements_dev = d_el;
+  // every slot starts empty and inactive
+  HIPCHECK(hipMemset(d_state, 0, nmol * sizeof(MolState)));
   HIPCHECK(hipHostMalloc((void**)&h_pinned, 64 * sizeof(int), hipHostMallocDefault));
+}
+
+// reference C6 of all element pairs present (D4 reference tables)
+void Batch::update_c6ref() {
+  const Element* el = host_elements();
+  std::vector<double> tab((size_t)(XTB_MAX_Z + 1) * (XTB_MAX_Z + 1) * 49, 0.0);
+  for (int za = 1; za <= XTB_MAX_Z; ++za) {
+    if (!elem_present_[za]) continue;
+    for (int zb = 1; zb <= XTB_MAX_Z; ++zb) {
+      if (!elem_present_[zb]) continue;
+      for (int r = 0; r < el[za].nref; ++r)
+        for (int s2 = 0; s2 < el[zb].nref; ++s2)
+          tab[((size_t)za * (XTB_MAX_Z + 1) + zb) * 49 + r * 7 + s2] = host_ref_c6(za, r, zb, s2);
+    }
+  }
+  HIPCHECK(hipMemcpy(c6ref, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// slot m takes the molecule: offsets from the packed-array allocators
+void Batch::place(int m, const MolDesc& d0, const HostMol& hm, const MolState& st, bool upload) {
+  MolDesc d = d0;
+  Capacity f = footprint(d);
+  long long off[13];
+  for (int k = 0; k < 13; ++k) {
+    off[k] = ar_[k].take(*cap_field(f, k));
+    if (off[k] < 0) {
+      for (int j = 0; j < k; ++j) ar_[j].give(off[j], *cap_field(f, j));
+      throw std::runtime_error("no room");
+    }
+  }
+  d.a0 = (int)off[0];
+  d.s0 = (int)off[1];
+  d.o0 = (int)off[2];
+  d.d0 = (int)off[3];
+  d.m0 = off[4];
+  d.j0 = off[5];
+  d.x0 = off[6];
+  d.p0 = off[7];
+  d.e0 = off[8];
+  d.y0 = off[9];
+  d.v0 = off[10];
+  d.h0 = off[11];
+  d.g0 = off[12];
+  d.mol = m;
+  desc[m] = d;
+  hmol_[m] = hm;
+  h_state0_[m] = st;
+  occupied_[m] = 1;
+  if (upload) {
+    auto up_i = [&](int* base, long long o, const std::vector<int>& v) {
+      if (!v.empty())
+        HIPCHECK(hipMemcpyAsync(base + o, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    };
+    up_i(at, d.a0, hm.at); up_i(at_sh0, d.a0, hm.at_sh0); up_i(at_ao0, d.a0, hm.at_ao0);
+    up_i(sh_at, d.s0, hm.sh_at); up_i(sh_l, d.s0, hm.sh_l); up_i(sh_ao0, d.s0, hm.sh_ao0); up_i(sh_m, d.s0, hm.sh_m);
+    up_i(ao_at, d.o0, hm.ao_at); up_i(ao_sh, d.o0, hm.ao_sh);
+    up_i(d4_at, d.d0, hm.d4_at); up_i(d4_ref, d.d0, hm.d4_ref);
+    HIPCHECK(hipMemcpyAsync(xyz + 3LL * d.a0, hm.xyz.data(), hm.xyz.size() * sizeof(double), hipMemcpyHostToDevice,
+                            stream));
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
+  for (int z : hm.at) elem_present_[z] = 1;
+}
+
+int Batch::admit(const MoleculeInput& mi) {
+  MolDesc d;
+  HostMol hm;
+  MolState st;
+  describe(mi, opt.maxiter, d, hm, st);
+  int slot = -1;
+  for (int m = 0; m < nmol; ++m)
+    if (!occupied_[m]) { slot = m; break; }
+  if (slot < 0) return -1;
+  // all packed arrays must have room
+  Capacity f = footprint(d);
+  for (int k = 0; k < 13; ++k) {
+    const long long need = *cap_field(f, k);
+    bool fits = need <= 0;
+    for (auto& r : ar_[k].free_)
+      if (r.second - r.first >= need) { fits = true; break; }
+    if (!fits) return -1;
+  }
+  bool newel = false;
+  for (int z : hm.at) newel = newel || !elem_present_[z];
+  place(slot, d, hm, st, true);
+  if (newel) update_c6ref();
+  if (!guess_qsh_.empty()) {
+    guess_qsh_[slot].clear();
+    guess_dipm_[slot].clear();
+    guess_qp_[slot].clear();
+  }
+  return slot;
+}
+
+void Batch::release(int m) {
+  if (m < 0 || m >= nmol || !occupied_[m]) return;
+  Capacity f = footprint(desc[m]);
+  const MolDesc& d = desc[m];
+  const long long off[13] = {d.a0, d.s0, d.o0, d.d0, d.m0, d.j0, d.x0, d.p0, d.e0, d.y0, d.v0, d.h0, d.g0};
+  for (int k = 0; k < 13; ++k) ar_[k].give(off[k], *cap_field(f, k));
+  std::memset(&desc[m], 0, sizeof(MolDesc));
+  desc[m].mol = m;
+  occupied_[m] = 0;
+  hmol_[m] = HostMol();
+  MolState z;
+  std::memset(&z, 0, sizeof(z));
+  h_state0_[m] = z;
+  HIPCHECK(hipMemcpyAsync(d_state + m, &h_state0_[m], sizeof(MolState), hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+}
+
+// work lists, grid bounds and descriptor table of the occupied slots
+void Batch::rebuild() {
+  max_n = max_nsh = max_nao = max_nd4 = max_nbr = 0;
+  for (int m = 0; m < nmol; ++m) {
+    if (!occupied_[m]) continue;
+    const MolDesc& d = desc[m];
+    max_n = std::max(max_n, d.n);
+    max_nsh = std::max(max_nsh, d.nsh);
+    max_nao = std::max(max_nao, d.nao);
+    max_nd4 = std::max(max_nd4, d.nd4);
+    max_nbr = std::max(max_nbr, d.nbr);
+  }
+  HIPCHECK(hipMemcpyAsync(d_desc, desc.data(), nmol * sizeof(MolDesc), hipMemcpyHostToDevice, stream));
+  auto grow = [&](auto*& ptr, long long& capv, long long need) {
+    using T_ = typename std::remove_reference<decltype(*ptr)>::type;
+    if (need <= capv && ptr) return;
+    if (ptr) {
+      HIPCHECK(hipStreamSynchronize(stream));
+      auto it = std::find(allocs_.begin(), allocs_.end(), (void*)ptr);
+      if (it != allocs_.end()) allocs_.erase(it);
+      HIPCHECK(hipFree(ptr));
+    }
+    capv = std::max<long long>(need, 2 * capv);
+    ptr = alloc<T_>(capv);
+  };
+  // GEMM tiles over (molecule, 64x64 tile)
+  std::vector<int4> tl;
+  for (int m = 0; m < nmol; ++m) {
+    if (!occupied_[m]) continue;
+    const int nt = (desc[m].nao + 63) / 64;
+    for (int a = 0; a < nt; ++a)
+      for (int b = 0; b < nt; ++b) tl.push_back(make_int4(m, a, b, 0));
+  }
+  ntiles = (int)tl.size();
+  grow(d_tiles, tiles_cap_, std::max<long long>(1, ntiles));
+  if (ntiles)
+    HIPCHECK(hipMemcpyAsync(d_tiles, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice, stream));
+  // Jacobi problems: dim nao, matrices at m0, eigenvalues at o0
+  JacSet& js = jac_n;
+  long long qoff = 0;
+  std::vector<int> small, big;
+  std::vector<int2> pairs;
+  std::vector<int4> ta, tw, to;
+  js.max_rounds = 0;
+  js.big_max_dim = 0;
+  for (int m = 0; m < nmol; ++m) {
+    JacobiProb& p = js.probs[m];
+    std::memset(&p, 0, sizeof(p));
+    if (!occupied_[m]) continue;
+    p.dim = desc[m].nao;
+    p.nb = (p.dim + 31) / 32;
+    p.nbp = p.nb + (p.nb & 1);
+    p.rounds = std::max(1, p.nbp - 1);
+    p.a_off = desc[m].m0;
+    p.w_off = p.a_off;
+    p.ev_off = desc[m].o0;
+    p.q_off = qoff;
+    if (p.dim <= 64) {
+      small.push_back(m);
+      continue;
+    }
+    big.push_back(m);
+    js.big_max_dim = std::max(js.big_max_dim, p.dim);
+    qoff += (long long)(p.nbp / 2) * 4096;
+    js.max_rounds = std::max(js.max_rounds, p.rounds);
+    const int np = p.nbp / 2;
+    for (int k = 0; k < np; ++k) pairs.push_back(make_int2(m, k));
+    for (int k = 0; k < np; ++k)
+      for (int l = k; l < np; ++l) ta.push_back(make_int4(m, k, l, 0));
+    const int nrb = (p.dim + 63) / 64;
+    for (int rb = 0; rb < nrb; ++rb)
+      for (int l = 0; l < np; ++l) tw.push_back(make_int4(m, rb, l, 0));
+    for (int a = 0; a < nrb; ++a)
+      for (int b = a; b < nrb; ++b) to.push_back(make_int4(m, a, b, 0));
+  }
+  if (qoff > js.qsize) throw std::runtime_error("Jacobi pivot buffer too small");
+  HIPCHECK(hipMemcpyAsync(js.d_probs, js.probs.data(), nmol * sizeof(JacobiProb), hipMemcpyHostToDevice, stream));
+  js.nsmall = (int)small.size();
+  js.npairs = (int)pairs.size();
+  js.nta = (int)ta.size();
+  js.ntw = (int)tw.size();
+  js.nto = (int)to.size();
+  grow(js.d_small, small_cap_, std::max<long long>(1, js.nsmall));
+  grow(js.d_pairs, pairs_cap_, std::max<long long>(1, js.npairs));
+  grow(js.d_ta, ta_cap_, std::max<long long>(1, js.nta));
+  grow(js.d_tw, tw_cap_, std::max<long long>(1, js.ntw));
+  grow(js.d_to, to_cap_, std::max<long long>(1, js.nto));
+  js.nbig = (int)big.size();
+  grow(js.d_big, big_cap_, std::max<long long>(1, js.nbig));
+  if (!big.empty()) HIPCHECK(hipMemcpyAsync(js.d_big, big.data(), big.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+  if (!small.empty())
+    HIPCHECK(hipMemcpyAsync(js.d_small, small.data(), small.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+  if (!pairs.empty())
+    HIPCHECK(hipMemcpyAsync(js.d_pairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+  if (!ta.empty()) HIPCHECK(hipMemcpyAsync(js.d_ta, ta.data(), ta.size() * sizeof(int4), hipMemcpyHostToDevice, stream));
+  if (!tw.empty()) HIPCHECK(hipMemcpyAsync(js.d_tw, tw.data(), tw.size() * sizeof(int4), hipMemcpyHostToDevice, stream));
+  if (!to.empty()) HIPCHECK(hipMemcpyAsync(js.d_to, to.data(), to.size() * sizeof(int4), hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipStreamSynchronize(stream));
+}
+
+Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream_t s) : opt(o), stream(s) {
+  init_common();
+  if (mols.empty()) throw std::runtime_error("empty batch");
+  const int nm = (int)mols.size();
+  std::vector<MolDesc> ds(nm);
+  std::vector<HostMol> hms(nm);
+  std::vector<MolState> sts(nm);
+  Capacity tot;
+  tot.slots = nm;
+  for (int m = 0; m < nm; ++m) {
+    describe(mols[m], opt.maxiter, ds[m], hms[m], sts[m]);
+    Capacity f = footprint(ds[m]);
+    for (int k = 0; k < 13; ++k) *cap_field(tot, k) += *cap_field(f, k);
+  }
+  allocate(tot);
+  for (int m = 0; m < nm; ++m) place(m, ds[m], hms[m], sts[m], false);
+  // bulk upload: the packed layout is the concatenation of the molecules
+  auto cat_i = [&](std::vector<int> HostMol::*f) {
+    std::vector<int> v;
+    for (const HostMol& h : hms) v.insert(v.end(), (h.*f).begin(), (h.*f).end());
+    return v;
+  };
+  auto up_i = [&](int* d_, const std::vector<int>& h) {
+    if (!h.empty()) HIPCHECK(hipMemcpy(d_, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+  };
+  up_i(at, cat_i(&HostMol::at)); up_i(at_sh0, cat_i(&HostMol::at_sh0)); up_i(at_ao0, cat_i(&HostMol::at_ao0));
+  up_i(sh_at, cat_i(&HostMol::sh_at)); up_i(sh_l, cat_i(&HostMol::sh_l)); up_i(sh_ao0, cat_i(&HostMol::sh_ao0));
+  up_i(sh_m, cat_i(&HostMol::sh_m)); up_i(ao_at, cat_i(&HostMol::ao_at)); up_i(ao_sh, cat_i(&HostMol::ao_sh));
+  up_i(d4_at, cat_i(&HostMol::d4_at)); up_i(d4_ref, cat_i(&HostMol::d4_ref));
+  std::vector<double> xall;
+  for (const HostMol& h : hms) xall.insert(xall.end(), h.xyz.begin(), h.xyz.end());
+  HIPCHECK(hipMemcpy(xyz, xall.data(), xall.size() * sizeof(double), hipMemcpyHostToDevice));
+  update_c6ref();
+  rebuild();
+}
+
+Batch::Batch(const Capacity& cap, const Options& o, hipStream_t s) : opt(o), stream(s) {
+  init_common();
+  if (cap.slots <= 0) throw std::runtime_error("empty batch");
+  allocate(cap);
+  rebuild();
 }
 
 Batch::~Batch() {
@@ -636,34 +853,65 @@ void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
                      check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0);
   if (js.nsmall)
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
-                       (const double*)js.scale, js.state, keep_w ? 1 : 0);
+                       (const double*)js.scale, js.state, keep_w ? 1 : 0, 0);
   if (js.npairs) {
-    int r = 0;
-    for (int sweep = 0; sweep <= kMaxSweeps; ++sweep) {
+    // every round starts with the convergence test of the problems at a
+    // sweep boundary; the host reads the number still running once per
+    // max_rounds rounds
+    const int rmax = (kMaxSweeps + 1) * js.max_rounds;
+    for (int r = 0; r <= rmax; ++r) {
       hipLaunchKernelGGL(k_jac_offmax, dim3(js.nto), dim3(256), 0, stream, js.d_probs, js.d_to, (const double*)Am,
-                         (const int*)js.state, js.offmax);
+                         (const int*)js.state, js.offmax, r);
       hipLaunchKernelGGL(k_jac_mark, dim3(1), dim3(256), 0, stream, js.d_probs, nb, (const double*)js.scale,
-                         js.state, js.offmax, sweep, jac_nactive);
-      HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
-      HIPCHECK(hipStreamSynchronize(stream));
-      if (h_pinned[0] <= 0) break;
-      for (int c = 0; c < js.max_rounds; ++c, ++r) {
-        prof_begin(2);
-        hipLaunchKernelGGL(k_jac_pairs, dim3(js.npairs), dim3(256), 0, stream, js.d_probs, js.d_pairs, r,
-                           (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
-                           inner_sweeps, (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr);
-        prof_end(2, nullptr, 0);
+                         js.state, js.offmax, r, jac_nactive);
+      if (r % js.max_rounds == 0) {
+        HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        if (h_pinned[0] <= 0) break;
+      }
+      prof_begin(2);
+      hipLaunchKernelGGL(k_jac_pairs, dim3(js.npairs), dim3(256), 0, stream, js.d_probs, js.d_pairs, r,
+                         (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
+                         inner_sweeps, (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr);
+      prof_end(2, nullptr, 0);
+      if (use_mfma) {
+        hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(js.nta), dim3(256), 0, stream, js.d_probs, js.d_ta, r, Am,
+                           (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+        hipLaunchKernelGGL(k_jac_apply_w_mfma, dim3(js.ntw), dim3(256), 0, stream, js.d_probs, js.d_tw, r, Wm,
+                           (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+      } else {
         hipLaunchKernelGGL(k_jac_apply_a, dim3(js.nta), dim3(256), 0, stream, js.d_probs, js.d_ta, r, Am,
                            (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
         hipLaunchKernelGGL(k_jac_apply_w, dim3(js.ntw), dim3(256), 0, stream, js.d_probs, js.d_tw, r, Wm,
                            (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
       }
-      jacobi_rounds_ += js.max_rounds;
+      jacobi_rounds_ += 1;
     }
   }
   if (evout)
     hipLaunchKernelGGL(k_jac_sort, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm,
-                       check_active ? (const MolState*)d_state : (const MolState*)nullptr);
+                       check_active ? (const MolState*)d_state : (const MolState*)nullptr, 1 << 30);
+}
+
+void Batch::eigh_rows(JacSet& js, double* Am, double* Wm, double* evout) {
+  const MolState* st = d_state;
+  // state and scale of the small problems (W is written whole below)
+  hipLaunchKernelGGL(k_jac_init, dim3(nmol), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale,
+                     js.state, st, js.offmax, 1);
+  if (js.nsmall) {
+    hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
+                       (const double*)js.scale, js.state, 0, 1);
+    hipLaunchKernelGGL(k_jac_sort, dim3(nmol), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm, st,
+                       64);
+  }
+  if (js.nbig) {
+    const int cap = std::min(js.big_max_dim, 1600);
+    prof_begin(2);
+    hipLaunchKernelGGL(k_tred_ql, dim3(js.nbig), dim3(256), (size_t)5 * cap * sizeof(double), stream, js.d_probs,
+                       (const int*)js.d_big, Am, Wm, G, evout, perm, st, cap, eig_failed,
+                       (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr);
+    prof_end(2, nullptr, 0);
+  }
 }
 
 void Batch::refine(double* Am, double* Wm) {
@@ -735,6 +983,7 @@ void Batch::gemm(int kind, const double* a, const double* b, double* c, const do
     XTBAMD_GEMM(0, false, false, false, false)
     XTBAMD_GEMM(1, true, false, false, false)
     XTBAMD_GEMM(2, false, true, true, false)
+    XTBAMD_GEMM(4, false, true, false, true)
     default:
     XTBAMD_GEMM(3, false, false, false, true)
   }
@@ -885,7 +1134,7 @@ void Batch::setup() {
     if (guess_qsh_.empty() || guess_qsh_[m].empty()) continue;
     const MolDesc& d = desc[m];
     std::vector<double> qa(d.n, 0.0);
-    for (int s = 0; s < d.nsh; ++s) qa[h_sh_at_[d.s0 + s]] += guess_qsh_[m][s];
+    for (int s = 0; s < d.nsh; ++s) qa[hmol_[m].sh_at[s]] += guess_qsh_[m][s];
     HIPCHECK(hipMemcpyAsync(qsh + d.s0, guess_qsh_[m].data(), d.nsh * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemcpyAsync(q + d.a0, qa.data(), d.n * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemcpyAsync(dipm + 3 * d.a0, guess_dipm_[m].data(), 3 * d.n * sizeof(double), hipMemcpyHostToDevice,
@@ -926,8 +1175,13 @@ bool Batch::scc_iteration() {
                        d_desc, (const MolState*)d_state, W);
     refine(A, W);
   }
-  jacobi(jac_n, A, W, emo, true, warm);
-  gemm(3, C, W, T, nullptr, perm, true);      // T = C W(:, perm)
+  if (use_ql && !warm) {
+    eigh_rows(jac_n, A, W, emo);
+    gemm(4, C, W, T, nullptr, perm, true);    // T = C W(perm, :)^T
+  } else {
+    jacobi(jac_n, A, W, emo, true, warm);
+    gemm(3, C, W, T, nullptr, perm, true);    // T = C W(:, perm)
+  }
   hipLaunchKernelGGL(k_copy_active, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream,
                      d_desc, (const MolState*)d_state, (const double*)T, C);
   hipLaunchKernelGGL(k_occ_fermi, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, (const double*)emo, focca, foccb,
@@ -1052,6 +1306,123 @@ MolResult Batch::result(int m, bool matrices) const {
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// continuous batching
+__global__ void k_active_flags(const MolState* __restrict__ st, int nmol, int* out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < nmol) out[m] = st[m].active;
+}
+
+Capacity server_capacity(double hbm_bytes, int max_atoms, int max_slots, int maxiter) {
+  MolDesc d;
+  std::memset(&d, 0, sizeof(d));
+  d.n = max_atoms;
+  d.nsh = 2 * max_atoms;
+  d.nao = 4 * max_atoms;
+  d.nd4 = 7 * max_atoms;
+  d.ne = d.n + 1;
+  d.nbr = d.nsh + 9 * d.n;
+  d.maxiter = maxiter;
+  Capacity f = Batch::footprint(d);
+  // bytes per element of each packed array (the buffers allocate() places there)
+  const double per[13] = {8.0 * (3 + 4 + 1 + 3 + 6 + 1 + 1 + 3 + 6 + 1 + 3 + 3 + 2 + 4 + 4 * XTB_MAX_REF + 1) + 4.0 * 3,
+                          8.0 * 7 + 4.0 * 4, 8.0 * 16 + 4.0 * 3, 8.0 * 4 + 4.0 * 2, 8.0 * 22, 8.0, 8.0,
+                          8.0 * 7, 8.0, 16.0, 32.0, 16.0, 8.0};
+  double bytes = 0.0;
+  for (int k = 0; k < 13; ++k) bytes += per[k] * (double)*cap_field(f, k);
+  const long long copies = std::max(1LL, (long long)(hbm_bytes / bytes));
+  Capacity c;
+  for (int k = 0; k < 13; ++k) *cap_field(c, k) = copies * *cap_field(f, k);
+  c.slots = max_slots;
+  return c;
+}
+
+Server::Server(const Capacity& cap, const Options& opt) : batch(cap, opt) {
+  slot_req_.assign(batch.nmol, -1);
+  active_host_.assign(batch.nmol, 0);
+  batch.oa_max_steps = 0;   // per-batch warm-start refinement is not used here
+  HIPCHECK(hipMalloc((void**)&d_active_, batch.nmol * sizeof(int)));
+  batch.allocs_.push_back(d_active_);
+}
+
+long long Server::submit(const MoleculeInput& mi) {
+  // validate now so that a bad request fails its caller, not a later step
+  MolDesc d;
+  HostMol hm;
+  MolState st;
+  Batch::describe(mi, batch.opt.maxiter, d, hm, st);
+  const long long id = next_id_++;
+  queue_.emplace_back(id, mi);
+  return id;
+}
+
+int Server::running() const {
+  int c = 0;
+  for (long long r : slot_req_) c += r >= 0 ? 1 : 0;
+  return c;
+}
+
+int Server::step() {
+  // 1. admission: first fit over the queue (a large request does not block
+  //    smaller ones behind it; the queue order is kept otherwise)
+  std::vector<int> fresh;
+  for (auto it = queue_.begin(); it != queue_.end();) {
+    const int slot = batch.admit(it->second);
+    if (slot >= 0) {
+      slot_req_[slot] = it->first;
+      fresh.push_back(slot);
+      it = queue_.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  if (fresh.empty() && running() == 0 && !queue_.empty())
+    throw std::runtime_error("request larger than the HBM budget of the server");
+  if (!fresh.empty()) {
+    batch.rebuild();
+    batch.select(fresh);
+    batch.setup();
+  }
+  if (running() == 0) return queued();
+  // 2. one SCC iteration of every resident molecule
+  batch.scc_iteration();
+  // 3. completion: gradient and results of the molecules that stopped iterating
+  hipLaunchKernelGGL(k_active_flags, dim3((batch.nmol + 255) / 256), dim3(256), 0, batch.stream,
+                     (const MolState*)batch.d_state, batch.nmol, d_active_);
+  HIPCHECK(hipMemcpyAsync(active_host_.data(), d_active_, batch.nmol * sizeof(int), hipMemcpyDeviceToHost,
+                          batch.stream));
+  HIPCHECK(hipStreamSynchronize(batch.stream));
+  std::vector<int> fin;
+  for (int m = 0; m < batch.nmol; ++m)
+    if (slot_req_[m] >= 0 && !active_host_[m]) fin.push_back(m);
+  if (!fin.empty()) {
+    batch.select(fin);
+    batch.gradient();
+    for (int m : fin) {
+      done_[slot_req_[m]] = batch.result(m, false);
+      finished_.push_back(slot_req_[m]);
+      slot_req_[m] = -1;
+      batch.release(m);
+    }
+    batch.rebuild();
+  }
+  return running() + queued();
+}
+
+std::vector<long long> Server::take_finished() {
+  std::vector<long long> f;
+  f.swap(finished_);
+  return f;
+}
+
+bool Server::result(long long id, MolResult& out, bool erase) {
+  auto it = done_.find(id);
+  if (it == done_.end()) return false;
+  out = it->second;
+  if (erase) done_.erase(it);
+  return true;
+}
+
 }  // namespace xtbamd
 
 // ===========================================================================
@@ -1067,6 +1438,40 @@ thread_local std::string g_last_error;
 int fail(const std::exception& e) {
   g_last_error = e.what();
   return -1;
+}
+
+void copy_result(const xtbamd::MolResult& r, xtbamd_result* out) {
+  out->natoms = r.n;
+  out->nshells = r.nsh;
+  out->nao = r.nao;
+  out->iterations = r.iterations;
+  out->converged = r.converged;
+  out->failed = r.failed;
+  out->ihomo = r.ihomo;
+  out->energy = r.energy;
+  out->gnorm = r.gnorm;
+  out->hl_gap_ev = r.hl_gap_ev;
+  out->e_elec = r.e_elec;
+  out->e_es = r.e_es;
+  out->e_aes = r.e_aes;
+  out->e_axc = r.e_axc;
+  out->e_disp = r.e_disp;
+  out->e_rep = r.e_rep;
+  out->e_atm = r.e_atm;
+  for (int k = 0; k < 3; ++k) out->dipole[k] = r.dipole[k];
+  auto put = [](double* dst, const std::vector<double>& v) {
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  };
+  put(out->gradient, r.gradient);
+  put(out->charges, r.q);
+  put(out->qsh, r.qsh);
+  put(out->dipm, r.dipm);
+  put(out->qp, r.qp);
+  put(out->wbo, r.wbo);
+  put(out->emo, r.emo);
+  put(out->focc, r.focc);
+  put(out->coeff, r.coeff);
+  put(out->density, r.density);
 }
 }  // namespace
 
@@ -1159,44 +1564,98 @@ int xtbamd_batch_profile_result(xtbamd_batch b, int kernel, int* launches, doubl
   } catch (const std::exception& e) { return fail(e); }
 }
 
+struct xtbamd_server_s {
+  xtbamd::Server* server = nullptr;
+};
+
+xtbamd_server xtbamd_server_new(double hbm_gb, int max_atoms, int max_slots, double accuracy, double etemp,
+                                int maxiter) {
+  try {
+    if (hbm_gb <= 0 || max_atoms <= 0 || max_slots <= 0) throw std::runtime_error("invalid server arguments");
+    xtbamd::Options o;
+    o.accuracy = accuracy;
+    o.etemp = etemp;
+    o.maxiter = maxiter;
+    xtbamd::Capacity c = xtbamd::server_capacity(hbm_gb * 1073741824.0, max_atoms, max_slots, maxiter);
+    auto* h = new xtbamd_server_s;
+    h->server = new xtbamd::Server(c, o);
+    return h;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return nullptr;
+  }
+}
+
+long long xtbamd_server_submit(xtbamd_server s, int natoms, const int* numbers, const double* positions,
+                               double charge, int uhf) {
+  try {
+    if (!s || natoms <= 0 || !numbers || !positions) throw std::runtime_error("invalid request");
+    xtbamd::MoleculeInput mi;
+    mi.numbers.assign(numbers, numbers + natoms);
+    mi.xyz.assign(positions, positions + 3 * natoms);
+    mi.charge = charge;
+    mi.uhf = uhf;
+    return s->server->submit(mi);
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return -1;
+  }
+}
+
+int xtbamd_server_step(xtbamd_server s) {
+  try {
+    if (!s) throw std::runtime_error("invalid server");
+    return s->server->step();
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_server_finished(xtbamd_server s, long long* ids, int max_ids) {
+  try {
+    if (!s) throw std::runtime_error("invalid server");
+    static thread_local std::vector<long long> pending;
+    std::vector<long long> f = s->server->take_finished();
+    pending.insert(pending.end(), f.begin(), f.end());
+    const int n = std::min<int>(max_ids, (int)pending.size());
+    for (int i = 0; i < n; ++i) ids[i] = pending[i];
+    pending.erase(pending.begin(), pending.begin() + n);
+    return n;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_server_result(xtbamd_server s, long long id, xtbamd_result* out) {
+  try {
+    if (!s || !out) throw std::runtime_error("invalid result request");
+    xtbamd::MolResult r;
+    const bool sizes_only = !out->gradient && !out->charges && !out->qsh && !out->dipm && !out->qp && !out->wbo &&
+                            !out->emo && !out->focc;
+    if (!s->server->result(id, r, !sizes_only)) throw std::runtime_error("no finished result with this id");
+    copy_result(r, out);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_server_pending(xtbamd_server s, int* queued, int* running) {
+  if (!s) return -1;
+  if (queued) *queued = s->server->queued();
+  if (running) *running = s->server->running();
+  return 0;
+}
+
+void xtbamd_server_free(xtbamd_server s) {
+  if (!s) return;
+  delete s->server;
+  delete s;
+}
+
 int xtbamd_batch_size(xtbamd_batch b) { return b ? b->batch->size() : 0; }
 
 int xtbamd_batch_get(xtbamd_batch b, int m, xtbamd_result* out) {
   try {
-    if (!b || !out || m < 0 || m >= b->batch->size()) throw std::runtime_error("invalid result request");
+    if (!b || !out || m < 0 || m >= b->batch->size() || !b->batch->occupied(m))
+      throw std::runtime_error("invalid result request");
     const bool mats = out->coeff || out->density;
     xtbamd::MolResult r = b->batch->result(m, mats);
-    out->natoms = r.n;
-    out->nshells = r.nsh;
-    out->nao = r.nao;
-    out->iterations = r.iterations;
-    out->converged = r.converged;
-    out->failed = r.failed;
-    out->ihomo = r.ihomo;
-    out->energy = r.energy;
-    out->gnorm = r.gnorm;
-    out->hl_gap_ev = r.hl_gap_ev;
-    out->e_elec = r.e_elec;
-    out->e_es = r.e_es;
-    out->e_aes = r.e_aes;
-    out->e_axc = r.e_axc;
-    out->e_disp = r.e_disp;
-    out->e_rep = r.e_rep;
-    out->e_atm = r.e_atm;
-    for (int k = 0; k < 3; ++k) out->dipole[k] = r.dipole[k];
-    auto put = [](double* dst, const std::vector<double>& v) {
-      if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(double));
-    };
-    put(out->gradient, r.gradient);
-    put(out->charges, r.q);
-    put(out->qsh, r.qsh);
-    put(out->dipm, r.dipm);
-    put(out->qp, r.qp);
-    put(out->wbo, r.wbo);
-    put(out->emo, r.emo);
-    put(out->focc, r.focc);
-    put(out->coeff, r.coeff);
-    put(out->density, r.density);
+    copy_result(r, out);
     return 0;
   } catch (const std::exception& e) { return fail(e); }
 }

@@ -178,6 +178,11 @@ template __global__ void k_gemm<false, true, true, false>(const int4*, const Mol
                                                           double*, const double*, const int*, double, const int*, int);
 template __global__ void k_gemm<false, false, false, true>(const int4*, const MolDesc*, const double*, const double*,
                                                            double*, const double*, const int*, double, const int*, int);
+template __global__ void k_gemm<false, true, false, true>(const int4*, const MolDesc*, const double*, const double*,
+                                                          double*, const double*, const int*, double, const int*, int);
+template __global__ void k_gemm_mfma<false, true, false, true>(const int4*, const MolDesc*, const double*,
+                                                               const double*, double*, const double*, const int*,
+                                                               double, const int*, int);
 template __global__ void k_gemm_mfma<false, false, false, false>(const int4*, const MolDesc*, const double*,
                                                                  const double*, double*, const double*, const int*,
                                                                  double, const int*, int);
@@ -320,7 +325,7 @@ constexpr int kMaxSweeps = 40;
 // problems of dimension <= 64: one workgroup solves to convergence in LDS
 __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict__ probs, const int* __restrict__ list,
                                                    double* A, double* W, const double* __restrict__ scale_in,
-                                                   int* state, int keep_w) {
+                                                   int* state, int keep_w, int rows_out) {
   __shared__ double a[64 * kLd];
   __shared__ double q[64 * kLd];
   __shared__ double cs[96];
@@ -345,7 +350,7 @@ __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict_
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
     const int u = e / n, v = e % n;
     Ap[(long long)u * n + v] = a[u * kLd + v];
-    if (!keep_w) Wp[(long long)u * n + v] = q[u * kLd + v];
+    if (!keep_w) Wp[(long long)u * n + v] = rows_out ? q[v * kLd + u] : q[u * kLd + v];
   }
   if (keep_w && nrot > 0) {
     // W <- W Q, with the old W staged in LDS
@@ -555,6 +560,147 @@ __global__ void __launch_bounds__(256) k_jac_apply_w(const JacobiProb* __restric
   }
 }
 
+// ---- the two apply kernels on the FP64 matrix cores: 64x64x64 products
+// staged in LDS (two 64x65 tiles, two workgroups per CU); each wave owns a
+// 32x32 quadrant as 2x2 v_mfma_f64_16x16x4f64 blocks (lane layouts as in
+// k_gemm_mfma).  out(32x32 quadrant) = sum_k L[row][k] R[k][col] with the
+// left operand read as L[row][k] = lt ? Lm[k][row] : Lm[row][k].
+__device__ __forceinline__ void mfma_tile64(const double* Lm, bool lt, const double* Rm, v4d acc[2][2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = w >> 1, wc = w & 1, i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[r][c] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int kk = 0; kk < 64; kk += 4) {
+    double a[2], b[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int row = 32 * wr + 16 * r + i16, k = kk + k4;
+      a[r] = lt ? Lm[k * kLd + row] : Lm[row * kLd + k];
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) b[c] = Rm[(kk + k4) * kLd + 32 * wc + 16 * c + i16];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void store_quadrants(double* Tm, const v4d acc[2][2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = w >> 1, wc = w & 1, i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Tm[(32 * wr + 16 * r + k4 + 4 * i) * kLd + 32 * wc + 16 * c + i16] = acc[r][c][i];
+}
+
+// A[I_k, I_l] <- Q_k^T A[I_k, I_l] Q_l for pair tiles k <= l (matrix cores)
+__global__ void __launch_bounds__(256) k_jac_apply_a_mfma(const JacobiProb* __restrict__ probs,
+                                                          const int4* __restrict__ list, int r, double* A,
+                                                          const double* __restrict__ Qbuf,
+                                                          const int* __restrict__ rotflag,
+                                                          const int* __restrict__ state) {
+  __shared__ double t[64 * kLd];
+  __shared__ double qm[64 * kLd];
+  const int4 it = list[blockIdx.x];
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return;
+  const int k = it.y, l = it.z;
+  const int qb = (int)(pr.q_off / 4096);
+  const int rk = rotflag[qb + k], rl = rotflag[qb + l];
+  if (!rk && !rl) return;
+  const int n = pr.dim;
+  const int lr = r % pr.rounds;
+  int kp, kq, lp, lq;
+  circle_pair(pr.nbp, lr, k, kp, kq);
+  circle_pair(pr.nbp, lr, l, lp, lq);
+  double* Ap = A + pr.a_off;
+  const double* Qk = Qbuf + pr.q_off + (long long)k * 4096;
+  const double* Ql = Qbuf + pr.q_off + (long long)l * 4096;
+  for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+    const int u = e >> 6, v = e & 63;
+    const int gu = sub_index(kp, kq, u), gv = sub_index(lp, lq, v);
+    t[u * kLd + v] = (gu < n && gv < n) ? Ap[(long long)gu * n + gv] : 0.0;
+    qm[u * kLd + v] = rl ? Ql[e] : (u == v ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  v4d acc[2][2];
+  mfma_tile64(t, false, qm, acc);   // M = T Q_l
+  __syncthreads();
+  store_quadrants(t, acc);
+  for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+    const int u = e >> 6, v = e & 63;
+    qm[u * kLd + v] = rk ? Qk[e] : (u == v ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  mfma_tile64(qm, true, t, acc);    // R = Q_k^T M
+  __syncthreads();
+  store_quadrants(t, acc);
+  __syncthreads();
+  // R to [I_k, I_l] (row-coalesced) and R^T to [I_l, I_k] (column reads of
+  // the LDS tile, row-coalesced writes); a diagonal tile is written from its
+  // upper half to both positions, which keeps A exactly symmetric
+  for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+    const int u = e >> 6, v = e & 63;
+    const int gu = sub_index(kp, kq, u), gv = sub_index(lp, lq, v);
+    if (gu < n && gv < n) {
+      const double val = (k == l && u > v) ? t[v * kLd + u] : t[u * kLd + v];
+      Ap[(long long)gu * n + gv] = val;
+    }
+  }
+  if (k != l)
+    for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+      const int v = e >> 6, u = e & 63;
+      const int gu = sub_index(kp, kq, u), gv = sub_index(lp, lq, v);
+      if (gu < n && gv < n) Ap[(long long)gv * n + gu] = t[u * kLd + v];
+    }
+}
+
+// W[rows, I_l] <- W[rows, I_l] Q_l for 64-row blocks (matrix cores)
+__global__ void __launch_bounds__(256) k_jac_apply_w_mfma(const JacobiProb* __restrict__ probs,
+                                                          const int4* __restrict__ list, int r, double* W,
+                                                          const double* __restrict__ Qbuf,
+                                                          const int* __restrict__ rotflag,
+                                                          const int* __restrict__ state) {
+  __shared__ double t[64 * kLd];
+  __shared__ double qm[64 * kLd];
+  const int4 it = list[blockIdx.x];
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return;
+  const int l = it.z, rb = it.y;
+  const int qb = (int)(pr.q_off / 4096);
+  if (!rotflag[qb + l]) return;
+  const int n = pr.dim;
+  const int lr = r % pr.rounds;
+  int lp, lq;
+  circle_pair(pr.nbp, lr, l, lp, lq);
+  double* Wp = W + pr.w_off;
+  const double* Ql = Qbuf + pr.q_off + (long long)l * 4096;
+  for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+    const int u = e >> 6, v = e & 63;
+    const int gu = rb * 64 + u, gv = sub_index(lp, lq, v);
+    t[u * kLd + v] = (gu < n && gv < n) ? Wp[(long long)gu * n + gv] : 0.0;
+    qm[u * kLd + v] = Ql[e];
+  }
+  __syncthreads();
+  v4d acc[2][2];
+  mfma_tile64(t, false, qm, acc);
+  __syncthreads();
+  store_quadrants(t, acc);
+  __syncthreads();
+  for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+    const int u = e >> 6, v = e & 63;
+    const int gu = rb * 64 + u, gv = sub_index(lp, lq, v);
+    if (gu < n && gv < n) Wp[(long long)gu * n + gv] = t[u * kLd + v];
+  }
+}
+
 // initialise W = I and the per-problem scale max|A_ii| and state; molecules
 // that finished their SCC are marked done and skipped
 __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__ probs, const double* __restrict__ A,
@@ -587,16 +733,18 @@ __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__
   }
 }
 
-// largest off-diagonal magnitude of every unfinished block-path problem, one
-// workgroup per upper-triangular 64 x 64 tile; max over non-negative doubles
-// is taken on their bit patterns, which order the same way (order independent)
+// largest off-diagonal magnitude of the unfinished block-path problems that
+// start a sweep at global round r (r a multiple of the problem's rounds per
+// sweep), one workgroup per upper-triangular 64 x 64 tile; max over
+// non-negative doubles is taken on their bit patterns, which order the same
+// way (order independent)
 __global__ void __launch_bounds__(256) k_jac_offmax(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
                                                     const double* __restrict__ A, const int* __restrict__ state,
-                                                    unsigned long long* offmax) {
+                                                    unsigned long long* offmax, int r) {
   __shared__ double sh[4];
   const int4 it = list[blockIdx.x];
   const JacobiProb pr = probs[it.x];
-  if (state[it.x * kJacState]) return;
+  if (state[it.x * kJacState] || r % pr.rounds != 0) return;
   const int n = pr.dim;
   const double* Ap = A + pr.a_off;
   double m = 0.0;
@@ -608,30 +756,37 @@ __global__ void __launch_bounds__(256) k_jac_offmax(const JacobiProb* __restrict
   if (threadIdx.x == 0 && m > 0.0) atomicMax(&offmax[it.x], (unsigned long long)__double_as_longlong(m));
 }
 
-// convergence test at the start of an outer sweep: done when every
-// off-diagonal element is <= tol, failed after kMaxSweeps sweeps; counts the
-// problems still running into nactive and resets offmax
+// convergence test of the problems at a sweep boundary (per problem, so a
+// problem's rounds never depend on the other problems of the batch): done
+// when every off-diagonal element is <= tol, failed after kMaxSweeps sweeps;
+// counts the problems still running into nactive
 __global__ void __launch_bounds__(256) k_jac_mark(const JacobiProb* __restrict__ probs, int nprob,
                                                   const double* __restrict__ scale, int* state,
-                                                  unsigned long long* offmax, int sweep, int* nactive) {
+                                                  unsigned long long* offmax, int r, int* nactive) {
   __shared__ double sh[4];
   double c = 0.0;
   for (int p = threadIdx.x; p < nprob; p += blockDim.x) {
     int* st = state + p * kJacState;
     if (st[0]) continue;
-    if (probs[p].dim <= 64) continue;   // solved by k_jac_small
-    const double m = __longlong_as_double((long long)offmax[p]);
-    offmax[p] = 0ull;
-    if (m <= 1e-12 * scale[p]) {
-      st[0] = 1;
-      st[5] = sweep;
-    } else if (sweep >= kMaxSweeps) {
-      st[0] = 1;
-      st[1] = 1;
-      st[5] = sweep;
-    } else {
-      c += 1.0;
+    const JacobiProb pr = probs[p];
+    if (pr.dim <= 64) continue;   // solved by k_jac_small
+    if (r % pr.rounds == 0) {
+      const int sweep = r / pr.rounds;
+      const double m = __longlong_as_double((long long)offmax[p]);
+      offmax[p] = 0ull;
+      if (m <= 1e-12 * scale[p]) {
+        st[0] = 1;
+        st[5] = sweep;
+        continue;
+      }
+      if (sweep >= kMaxSweeps) {
+        st[0] = 1;
+        st[1] = 1;
+        st[5] = sweep;
+        continue;
+      }
     }
+    c += 1.0;
   }
   c = block_sum<256>(c, sh);
   if (threadIdx.x == 0) nactive[0] = (int)c;
@@ -759,11 +914,202 @@ __global__ void k_oa_update(const MolDesc* __restrict__ desc, const int* __restr
   X[md.m0 + t] = x + D[md.m0 + t];
 }
 
+// ---------------------------------------------------------------------------
+// Dense symmetric eigensolver for one problem per workgroup: Householder
+// tridiagonalisation (A = Q T Q^T, vectors kept in the rows of A), explicit
+// Q^T, implicit-shift QL on T with the rotations applied to the rows of
+// W^T = Q^T (Golub & Van Loan, "Matrix Computations", 4th ed., 8.3.1 and
+// 8.3.3; the same sequence as EISPACK tred2/tql2).  Row i of W (stored at
+// w_off) is the eigenvector of eigenvalue d_i; ev/perm receive the ascending
+// order.  Dynamic LDS: 5 * nmax doubles (d, e, tau and two work vectors).
+__global__ void __launch_bounds__(256) k_tred_ql(const JacobiProb* __restrict__ probs, const int* __restrict__ list,
+                                                 double* A, double* W, double* scratch, double* ev, int* perm,
+                                                 const MolState* __restrict__ mstate, int lds_cap, int* failed,
+                                                 unsigned long long* work) {
+  extern __shared__ double lds[];
+  __shared__ double red[4];
+  __shared__ double sc[4];
+  __shared__ int ctl[4];
+  const int pidx = list[blockIdx.x];
+  if (mstate && !mstate[pidx].active) return;
+  const JacobiProb pr = probs[pidx];
+  const int N = pr.dim;
+  // vectors in LDS when they fit, else in the problem's scratch matrix
+  double* base = (N <= lds_cap) ? lds : scratch + pr.a_off;
+  double* d = base;
+  double* e = base + N;
+  double* tau = base + 2 * N;
+  double* v = base + 3 * N;   // Householder vector, then rotation cosines
+  double* w = base + 4 * N;   // p / w vectors, then rotation sines
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double* M = A + pr.a_off;
+  double* Wt = W + pr.w_off;
+  // ---- tridiagonalisation
+  for (int k = 0; k + 2 < N; ++k) {
+    double s2 = 0.0;
+    for (int j = k + 2 + tid; j < N; j += nt) s2 += M[(long long)k * N + j] * M[(long long)k * N + j];
+    s2 = block_sum<256>(s2, red);
+    if (tid == 0) {
+      const double alpha = M[(long long)k * N + k + 1];
+      d[k] = M[(long long)k * N + k];
+      if (s2 == 0.0) {
+        tau[k] = 0.0;
+        e[k] = alpha;
+        sc[0] = 0.0;
+      } else {
+        const double beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+        tau[k] = (beta - alpha) / beta;
+        e[k] = beta;
+        sc[0] = 1.0 / (alpha - beta);
+      }
+    }
+    __syncthreads();
+    const double t = tau[k];
+    if (t == 0.0) continue;   // uniform
+    const double scale = sc[0];
+    for (int j = k + 1 + tid; j < N; j += nt) {
+      const double vj = (j == k + 1) ? 1.0 : M[(long long)k * N + j] * scale;
+      v[j] = vj;
+      M[(long long)k * N + j] = vj;   // kept for Q
+    }
+    __syncthreads();
+    // p = t A22 v (columns by thread, rows in the loop: coalesced)
+    for (int i = k + 1 + tid; i < N; i += nt) {
+      double acc = 0.0;
+      for (int j = k + 1; j < N; ++j) acc = fma(M[(long long)j * N + i], v[j], acc);
+      w[i] = t * acc;
+    }
+    __syncthreads();
+    double pv = 0.0;
+    for (int i = k + 1 + tid; i < N; i += nt) pv += w[i] * v[i];
+    pv = block_sum<256>(pv, red);
+    const double K = 0.5 * t * pv;
+    for (int i = k + 1 + tid; i < N; i += nt) w[i] -= K * v[i];
+    __syncthreads();
+    // A22 -= v w^T + w v^T
+    for (int i = k + 1 + tid; i < N; i += nt) {
+      const double vi = v[i], wi = w[i];
+      for (int j = k + 1; j < N; ++j) M[(long long)j * N + i] -= v[j] * wi + w[j] * vi;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (N >= 2) {
+      d[N - 2] = M[(long long)(N - 2) * N + N - 2];
+      e[N - 2] = M[(long long)(N - 2) * N + N - 1];
+    }
+    d[N - 1] = M[(long long)(N - 1) * N + N - 1];
+    e[N - 1] = 0.0;
+  }
+  // ---- W^T = Q^T = H_{N-3} ... H_0, built as W^T <- H_k W^T
+  for (long long x = tid; x < (long long)N * N; x += nt) Wt[x] = (x / N == x % N) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int k = 0; k + 2 < N; ++k) {
+    const double t = tau[k];
+    if (t == 0.0) continue;
+    for (int j = k + 1 + tid; j < N; j += nt) v[j] = M[(long long)k * N + j];
+    __syncthreads();
+    for (int c = tid; c < N; c += nt) {
+      double u = 0.0;
+      for (int r = k + 1; r < N; ++r) u = fma(v[r], Wt[(long long)r * N + c], u);
+      u *= t;
+      for (int r = k + 1; r < N; ++r) Wt[(long long)r * N + c] -= v[r] * u;
+    }
+    __syncthreads();
+  }
+  // ---- implicit QL; thread 0 runs the recurrence of a chase, all threads
+  // then rotate their columns of W^T by its (c, s) sequence
+  unsigned long long nrot = 0;
+  for (int l = 0; l < N; ++l) {
+    int iter = 0;
+    for (;;) {
+      if (tid == 0) {
+        int m = l;
+        for (; m < N - 1; ++m) {
+          const double dd = fabs(d[m]) + fabs(d[m + 1]);
+          if (fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+        }
+        ctl[0] = (m == l || iter >= 60) ? 1 : 0;
+        if (iter >= 60 && m != l) failed[pidx] = 1;
+        ctl[1] = m;
+        ctl[2] = l;   // lowest rotated index
+        if (m != l && iter < 60) {
+          double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+          double r = sqrt(g * g + 1.0);
+          g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+          double s = 1.0, c = 1.0, p = 0.0;
+          int i = m - 1;
+          bool early = false;
+          for (; i >= l; --i) {
+            const double f = s * e[i], b = c * e[i];
+            r = sqrt(f * f + g * g);
+            e[i + 1] = r;
+            if (r == 0.0) {
+              d[i + 1] -= p;
+              e[m] = 0.0;
+              early = true;
+              break;
+            }
+            s = f / r;
+            c = g / r;
+            g = d[i + 1] - p;
+            r = (d[i] - g) * s + 2.0 * c * b;
+            p = s * r;
+            d[i + 1] = g + p;
+            g = c * r - b;
+            v[i] = c;
+            w[i] = s;
+          }
+          ctl[2] = early ? i + 1 : l;
+          if (!early) {
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+          }
+        }
+      }
+      __syncthreads();
+      if (ctl[0]) break;   // uniform
+      const int m = ctl[1], lo = ctl[2];
+      nrot += (unsigned long long)(m - lo);
+      for (int col = tid; col < N; col += nt) {
+        for (int i = m - 1; i >= lo; --i) {
+          const double c = v[i], s = w[i];
+          const double zi = Wt[(long long)i * N + col], zj = Wt[(long long)(i + 1) * N + col];
+          Wt[(long long)(i + 1) * N + col] = s * zi + c * zj;
+          Wt[(long long)i * N + col] = c * zi - s * zj;
+        }
+      }
+      ++iter;
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  // ---- ascending order (stable rank sort)
+  for (int i = tid; i < N; i += nt) {
+    const double di = d[i];
+    int rank = 0;
+    for (int j = 0; j < N; ++j) {
+      const double dj = d[j];
+      rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+    }
+    ev[pr.ev_off + rank] = di;
+    perm[pr.ev_off + rank] = i;
+  }
+  // algorithmic flops: tridiagonalisation 4/3 N^3 and Q 4/3 N^3 (the
+  // LAPACK dsytrd/dorgtr counts), 6 N per applied rotation
+  if (work && tid == 0) {
+    const unsigned long long n3 = (unsigned long long)N * N * N;
+    atomicAdd(work, (8ull * n3) / 3ull + 6ull * (unsigned long long)N * nrot);
+  }
+}
+
 // eigenvalues ascending: ev[k] = A[perm[k]][perm[k]] (stable rank sort)
 __global__ void k_jac_sort(const JacobiProb* __restrict__ probs, const double* __restrict__ A, double* ev, int* perm,
-                           const MolState* __restrict__ mstate) {
+                           const MolState* __restrict__ mstate, int max_dim) {
   if (mstate && !mstate[blockIdx.x].active) return;
   const JacobiProb pr = probs[blockIdx.x];
+  if (pr.dim > max_dim) return;
   const int n = pr.dim;
   const double* Ap = A + pr.a_off;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {

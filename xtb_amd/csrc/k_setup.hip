@@ -414,9 +414,9 @@ __global__ void k_eeq_matrix(const MolDesc* __restrict__ desc, const Element* __
       const Element& ei = el[at[md.a0 + i]];
       const double c = cn_eeq[md.a0 + i];
       const double tmp = ei.eeq_kappa / (sqrt(c) + 1e-14);
-      xE[md.a0 + md.mol + i] = -ei.eeq_en + tmp * c;
+      xE[md.y0 + i] = -ei.eeq_en + tmp * c;
     } else {
-      xE[md.a0 + md.mol + n] = md.charge;
+      xE[md.y0 + n] = md.charge;
     }
   }
 }

@@ -47,6 +47,21 @@ def library():
     lib.xtbamd_batch_free.argtypes = [ctypes.c_void_p]
     lib.xtbamd_batch_free.restype = None
     lib.xtbamd_device_count.restype = ctypes.c_int
+    lib.xtbamd_server_new.restype = ctypes.c_void_p
+    lib.xtbamd_server_new.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int]
+    lib.xtbamd_server_submit.restype = ctypes.c_longlong
+    lib.xtbamd_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, _dp, ctypes.c_double, ctypes.c_int]
+    lib.xtbamd_server_step.restype = ctypes.c_int
+    lib.xtbamd_server_step.argtypes = [ctypes.c_void_p]
+    lib.xtbamd_server_finished.restype = ctypes.c_int
+    lib.xtbamd_server_finished.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+    lib.xtbamd_server_result.restype = ctypes.c_int
+    lib.xtbamd_server_result.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(Result)]
+    lib.xtbamd_server_pending.restype = ctypes.c_int
+    lib.xtbamd_server_pending.argtypes = [ctypes.c_void_p, ip, ip]
+    lib.xtbamd_server_free.restype = None
+    lib.xtbamd_server_free.argtypes = [ctypes.c_void_p]
     lib.xtbamd_batch_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_longlong)]
     lib.xtbamd_batch_counters.restype = ctypes.c_int
@@ -171,6 +186,78 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None):
             self._lib.xtbamd_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _fetch(lib, get, check):
+    """Sizes first (all array pointers NULL), then the arrays."""
+    r = Result()
+    check(get(ctypes.byref(r)))
+    n, nsh, nao = r.natoms, r.nshells, r.nao
+    out = {"gradient": np.zeros((n, 3)), "charges": np.zeros(n), "qsh": np.zeros(nsh),
+           "dipm": np.zeros((n, 3)), "qp": np.zeros((n, 6)), "wbo": np.zeros((n, n)),
+           "emo": np.zeros(nao), "focc": np.zeros(nao)}
+    for k, v in out.items():
+        setattr(r, k, _ptr(v))
+    check(get(ctypes.byref(r)))
+    for k in ("natoms", "nshells", "nao", "iterations", "converged", "failed", "ihomo", "energy", "gnorm",
+              "hl_gap_ev", "e_elec", "e_es", "e_aes", "e_axc", "e_disp", "e_rep", "e_atm"):
+        out[k] = getattr(r, k)
+    out["dipole"] = np.array(r.dipole[:])
+    return out
+
+
+class Server:
+    """Continuous batching of single points (xtbamd_server_* of include/xtb_amd_batch.h)."""
+
+    def __init__(self, hbm_gb=8.0, max_atoms=400, max_slots=1024, accuracy=1.0, etemp=300.0, maxiter=250):
+        lib = library()
+        self._lib = lib
+        h = lib.xtbamd_server_new(hbm_gb, max_atoms, max_slots, accuracy, etemp, maxiter)
+        if not h:
+            raise RuntimeError(_err(lib))
+        self._h = h
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RuntimeError(_err(self._lib))
+        return rc
+
+    def submit(self, numbers, positions, charge=0.0, uhf=0):
+        num = np.ascontiguousarray(numbers, dtype=np.int32)
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        rid = self._lib.xtbamd_server_submit(self._h, len(num), num.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                             _ptr(pos), float(charge), int(uhf))
+        if rid < 0:
+            raise RuntimeError(_err(self._lib))
+        return rid
+
+    def step(self):
+        """Admit, one SCC iteration of all resident molecules, finish; returns requests not finished."""
+        return self._check(self._lib.xtbamd_server_step(self._h))
+
+    def finished(self, max_ids=4096):
+        ids = (ctypes.c_longlong * max_ids)()
+        n = self._check(self._lib.xtbamd_server_finished(self._h, ids, max_ids))
+        return list(ids[:n])
+
+    def result(self, rid):
+        return _fetch(self._lib, lambda p: self._lib.xtbamd_server_result(self._h, rid, p), self._check)
+
+    def pending(self):
+        q, r = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.xtbamd_server_pending(self._h, ctypes.byref(q), ctypes.byref(r)))
+        return q.value, r.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.xtbamd_server_free(self._h)
             self._h = None
 
     def __del__(self):
