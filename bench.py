@@ -186,7 +186,7 @@ def main():
     names = {"gemm": "k_gemm (batched FP64 GEMM on the matrix cores)",
              "jacobi": os.environ.get("XTBAMD_EIG") == "ql" and
              "k_tred_ql (tridiagonalisation + QL eigensolver, one workgroup per molecule)" or
-             "k_jac_pairs (64x64 pivot problems of the block-Jacobi eigensolver, in LDS)"}
+             "k_jac_pairsP (32x32 pivot problems of the block-Jacobi eigensolver, in LDS)"}
     lines = {}
     for k, (n, tms, work) in prof.items():
         if not n:

@@ -483,6 +483,7 @@ void Batch::init_common() {
   if (const char* g = std::getenv("XTBAMD_GEMM")) use_mfma = std::string(g) != "valu";
   if (const char* r = std::getenv("XTBAMD_REFINE_STEPS")) oa_max_steps = std::atoi(r);
   if (const char* r = std::getenv("XTBAMD_INNER_SWEEPS")) inner_sweeps = std::max(1, std::atoi(r));
+  if (const char* r = std::getenv("XTBAMD_PIVOT")) jac_p = std::atoi(r) == 64 ? 64 : 32;
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
   if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
   intcut = std::max(20.0, 25.0 - 10.0 * std::log10(opt.accuracy));
@@ -577,7 +578,7 @@ void Batch::allocate(const Capacity& c) {
   // pivot-factor buffer: at most (nao/64 + 1) pairs per molecule
   js.qsize = std::max(4096LL, (c.nao / 64 + 2LL * nmol) * 4096LL);
   js.qbuf = alloc<double>(js.qsize);
-  js.rot = alloc<int>(js.qsize / 4096 + 1);
+  js.rot = alloc<int>(js.qsize / 1024 + 1);
   js.perm = nullptr;
   Element* d_el = alloc<Element>(XTB_MAX_Z + 1);
   HIPCHECK(hipMemcpy(d_el, host_elements(), (XTB_MAX_Z + 1) * sizeof(Element), hipMemcpyHostToDevice));
@@ -746,7 +747,7 @@ void Batch::rebuild() {
     std::memset(&p, 0, sizeof(p));
     if (!occupied_[m]) continue;
     p.dim = desc[m].nao;
-    p.nb = (p.dim + 31) / 32;
+    p.nb = (p.dim + jac_p / 2 - 1) / (jac_p / 2);
     p.nbp = p.nb + (p.nb & 1);
     p.rounds = std::max(1, p.nbp - 1);
     p.a_off = desc[m].m0;
@@ -759,15 +760,16 @@ void Batch::rebuild() {
     }
     big.push_back(m);
     js.big_max_dim = std::max(js.big_max_dim, p.dim);
-    qoff += (long long)(p.nbp / 2) * 4096;
+    qoff += (long long)(p.nbp / 2) * jac_p * jac_p;
     js.max_rounds = std::max(js.max_rounds, p.rounds);
     const int np = p.nbp / 2;
     for (int k = 0; k < np; ++k) pairs.push_back(make_int2(m, k));
     for (int k = 0; k < np; ++k)
       for (int l = k; l < np; ++l) ta.push_back(make_int4(m, k, l, 0));
-    const int nrb = (p.dim + 63) / 64;
-    for (int rb = 0; rb < nrb; ++rb)
+    const int nwb = (p.dim + jac_p - 1) / jac_p;
+    for (int rb = 0; rb < nwb; ++rb)
       for (int l = 0; l < np; ++l) tw.push_back(make_int4(m, rb, l, 0));
+    const int nrb = (p.dim + 63) / 64;
     for (int a = 0; a < nrb; ++a)
       for (int b = a; b < nrb; ++b) to.push_back(make_int4(m, a, b, 0));
   }
@@ -869,10 +871,24 @@ void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
         HIPCHECK(hipStreamSynchronize(stream));
         if (h_pinned[0] <= 0) break;
       }
+      unsigned long long* jw = (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr;
+      if (jac_p == 32) {
+        prof_begin(2);
+        hipLaunchKernelGGL((k_jac_pairsP<32, 64>), dim3(js.npairs), dim3(64), 0, stream, js.d_probs, js.d_pairs, r,
+                           (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
+                           inner_sweeps, jw);
+        prof_end(2, nullptr, 0);
+        hipLaunchKernelGGL((k_jac_apply_aP<32>), dim3(js.nta), dim3(64), 0, stream, js.d_probs, js.d_ta, r, Am,
+                           (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+        hipLaunchKernelGGL((k_jac_apply_wP<32>), dim3(js.ntw), dim3(64), 0, stream, js.d_probs, js.d_tw, r, Wm,
+                           (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+        jacobi_rounds_ += 1;
+        continue;
+      }
       prof_begin(2);
       hipLaunchKernelGGL(k_jac_pairs, dim3(js.npairs), dim3(256), 0, stream, js.d_probs, js.d_pairs, r,
                          (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
-                         inner_sweeps, (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr);
+                         inner_sweeps, jw);
       prof_end(2, nullptr, 0);
       if (use_mfma) {
         hipLaunchKernelGGL(k_jac_apply_a_mfma, dim3(js.nta), dim3(256), 0, stream, js.d_probs, js.d_ta, r, Am,

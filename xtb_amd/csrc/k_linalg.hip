@@ -701,6 +701,303 @@ __global__ void __launch_bounds__(256) k_jac_apply_w_mfma(const JacobiProb* __re
   }
 }
 
+// ---------------------------------------------------------------------------
+// Block-Jacobi kernels for a pivot size P (P/2-wide blocks): P = 64 with 256
+// threads (four waves) or P = 32 with 64 threads (one wave).  Same scheme as
+// above: cyclic parallel Jacobi of each P x P pivot problem in LDS, then
+// two-sided tile products on the matrix cores.
+template <int P, int NT>
+__device__ int jacobiP(double* a, double* q, double tol, int maxsweep, double* cs, int* pq, int* flags, double* red) {
+  constexpr int L = P + 1, H = P / 2, NB2 = H * (H + 1) / 2, NPB = (NB2 + NT - 1) / NT;
+  constexpr int RS = NT / H, RPT = P / RS;   // Q update: row stride and rows per thread
+  const int tid = threadIdx.x;
+  int kb1[NPB], kb2[NPB];
+#pragma unroll
+  for (int j = 0; j < NPB; ++j) {
+    const int b = tid + NT * j;
+    int k2 = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while (k2 * (k2 + 1) / 2 > b) --k2;
+    while ((k2 + 1) * (k2 + 2) / 2 <= b) ++k2;
+    kb2[j] = b < NB2 ? k2 : -1;
+    kb1[j] = b - k2 * (k2 + 1) / 2;
+  }
+  int total = 0;
+  for (int sweep = 0; sweep < maxsweep; ++sweep) {
+    double m = 0.0;
+    for (int e = tid; e < P * P; e += NT) {
+      const int u = e / P, v = e % P;
+      if (u < v) m = fmax(m, fabs(a[u * L + v]));
+    }
+    m = block_max<NT>(m, red);
+    if (m <= tol) break;
+    for (int r = 0; r < P - 1; ++r) {
+      if (tid < 64) {
+        bool rot = false;
+        if (tid < H) {
+          const int k = tid;
+          int p, qq;
+          if (k == 0) { p = 0; qq = 1 + r; }
+          else { p = 1 + (r + k) % (P - 1); qq = 1 + (r - k + (P - 1)) % (P - 1); }
+          if (p > qq) { const int t = p; p = qq; qq = t; }
+          const double apq = a[p * L + qq];
+          double c = 1.0, sn = 0.0, t = 0.0;
+          if (fabs(apq) > tol) {
+            const double app = a[p * L + p], aqq = a[qq * L + qq];
+            const double theta = (aqq - app) / (2.0 * apq);
+            t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            sn = t * c;
+            rot = true;
+          }
+          pq[2 * k] = p;
+          pq[2 * k + 1] = qq;
+          cs[3 * k] = c;
+          cs[3 * k + 1] = sn;
+          cs[3 * k + 2] = t;
+        }
+        const unsigned long long any = __ballot(rot);
+        if (tid == 0) flags[0] = any ? 1 : 0;
+      }
+      __syncthreads();
+      if (flags[0]) {
+        total += 1;
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) {
+          const int k1 = kb1[j], k2 = kb2[j];
+          if (k2 < 0) continue;
+          const int p1 = pq[2 * k1], q1 = pq[2 * k1 + 1], p2 = pq[2 * k2], q2 = pq[2 * k2 + 1];
+          const double c1 = cs[3 * k1], s1 = cs[3 * k1 + 1], c2 = cs[3 * k2], s2 = cs[3 * k2 + 1];
+          if (k1 == k2) {
+            if (s1 != 0.0) {
+              const double t1 = cs[3 * k1 + 2];
+              const double apq = a[p1 * L + q1];
+              a[p1 * L + p1] = a[p1 * L + p1] - t1 * apq;
+              a[q1 * L + q1] = a[q1 * L + q1] + t1 * apq;
+              a[p1 * L + q1] = 0.0;
+              a[q1 * L + p1] = 0.0;
+            }
+          } else if (s1 != 0.0 || s2 != 0.0) {
+            const double apr = a[p1 * L + p2], aps = a[p1 * L + q2];
+            const double aqr = a[q1 * L + p2], aqs = a[q1 * L + q2];
+            const double bpr = c2 * apr - s2 * aps, bps = s2 * apr + c2 * aps;
+            const double bqr = c2 * aqr - s2 * aqs, bqs = s2 * aqr + c2 * aqs;
+            const double npr = c1 * bpr - s1 * bqr, nqr = s1 * bpr + c1 * bqr;
+            const double nps = c1 * bps - s1 * bqs, nqs = s1 * bps + c1 * bqs;
+            a[p1 * L + p2] = npr; a[p2 * L + p1] = npr;
+            a[p1 * L + q2] = nps; a[q2 * L + p1] = nps;
+            a[q1 * L + p2] = nqr; a[p2 * L + q1] = nqr;
+            a[q1 * L + q2] = nqs; a[q2 * L + q1] = nqs;
+          }
+        }
+        {
+          const int k = tid % H;
+          const double sn = cs[3 * k + 1];
+          if (sn != 0.0) {
+            const double c = cs[3 * k];
+            const int p = pq[2 * k], qq = pq[2 * k + 1];
+            const int u0 = tid / H;
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+              const int u = u0 + RS * j;
+              const double up = q[u * L + p], uq = q[u * L + qq];
+              q[u * L + p] = c * up - sn * uq;
+              q[u * L + qq] = sn * up + c * uq;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  return total;
+}
+
+template <int P>
+__device__ __forceinline__ int sub_indexP(int bp, int bq, int u) {
+  return u < P / 2 ? (P / 2) * bp + u : (P / 2) * bq + (u - P / 2);
+}
+
+template <int P, int NT>
+__global__ void __launch_bounds__(NT) k_jac_pairsP(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
+                                                   int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
+                                                   const double* __restrict__ scale_in, const int* __restrict__ state,
+                                                   int inner_sweeps, unsigned long long* work) {
+  constexpr int L = P + 1;
+  __shared__ double a[P * L];
+  __shared__ double q[P * L];
+  __shared__ double cs[3 * (P / 2)];
+  __shared__ double red[4];
+  __shared__ int pq[P];
+  __shared__ int flags[2];
+  const int2 it = list[blockIdx.x];
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return;
+  const int n = pr.dim;
+  const int lr = r % pr.rounds;
+  int bp, bq;
+  circle_pair(pr.nbp, lr, it.y, bp, bq);
+  const double* Ap = A + pr.a_off;
+  for (int e = threadIdx.x; e < P * P; e += NT) {
+    const int u = e / P, v = e % P;
+    const int gu = sub_indexP<P>(bp, bq, u), gv = sub_indexP<P>(bp, bq, v);
+    a[u * L + v] = (gu < n && gv < n) ? Ap[(long long)gu * n + gv] : 0.0;
+    q[u * L + v] = (u == v) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const double tol = 1e-12 * scale_in[it.x];
+  const int nrot = jacobiP<P, NT>(a, q, tol, inner_sweeps, cs, pq, flags, red);
+  __syncthreads();
+  double* Qp = Qbuf + pr.q_off + (long long)it.y * P * P;
+  if (nrot > 0)
+    for (int e = threadIdx.x; e < P * P; e += NT) Qp[e] = q[(e / P) * L + (e % P)];
+  if (threadIdx.x == 0) {
+    rotflag[(int)(pr.q_off / (P * P)) + it.y] = nrot > 0 ? 1 : 0;
+    if (work && nrot > 0) atomicAdd(work, (unsigned long long)(9 * P * P) * (unsigned long long)nrot);
+  }
+}
+
+// out(P x P) = L R with L read transposed when lt; one 32x32 quadrant per wave
+template <int P>
+__device__ __forceinline__ void mfma_tileP(const double* Lm, bool lt, const double* Rm, v4d acc[2][2]) {
+  constexpr int L = P + 1, QW = P / 32;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = w / QW, wc = w % QW, i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[r][c] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int kk = 0; kk < P; kk += 4) {
+    double a[2], b[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int row = 32 * wr + 16 * r + i16, k = kk + k4;
+      a[r] = lt ? Lm[k * L + row] : Lm[row * L + k];
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) b[c] = Rm[(kk + k4) * L + 32 * wc + 16 * c + i16];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void store_quadrantsP(double* Tm, const v4d acc[2][2]) {
+  constexpr int L = P + 1, QW = P / 32;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = w / QW, wc = w % QW, i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Tm[(32 * wr + 16 * r + k4 + 4 * i) * L + 32 * wc + 16 * c + i16] = acc[r][c][i];
+}
+
+// A[I_k, I_l] <- Q_k^T A[I_k, I_l] Q_l for pair tiles k <= l
+template <int P>
+__global__ void __launch_bounds__((P / 32) * (P / 32) * 64) k_jac_apply_aP(
+    const JacobiProb* __restrict__ probs, const int4* __restrict__ list, int r, double* A,
+    const double* __restrict__ Qbuf, const int* __restrict__ rotflag, const int* __restrict__ state) {
+  constexpr int L = P + 1, NT = (P / 32) * (P / 32) * 64;
+  __shared__ double t[P * L];
+  __shared__ double qm[P * L];
+  const int4 it = list[blockIdx.x];
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return;
+  const int k = it.y, l = it.z;
+  const int qb = (int)(pr.q_off / (P * P));
+  const int rk = rotflag[qb + k], rl = rotflag[qb + l];
+  if (!rk && !rl) return;
+  const int n = pr.dim;
+  const int lr = r % pr.rounds;
+  int kp, kq, lp, lq;
+  circle_pair(pr.nbp, lr, k, kp, kq);
+  circle_pair(pr.nbp, lr, l, lp, lq);
+  double* Ap = A + pr.a_off;
+  const double* Qk = Qbuf + pr.q_off + (long long)k * P * P;
+  const double* Ql = Qbuf + pr.q_off + (long long)l * P * P;
+  for (int e = threadIdx.x; e < P * P; e += NT) {
+    const int u = e / P, v = e % P;
+    const int gu = sub_indexP<P>(kp, kq, u), gv = sub_indexP<P>(lp, lq, v);
+    t[u * L + v] = (gu < n && gv < n) ? Ap[(long long)gu * n + gv] : 0.0;
+    qm[u * L + v] = rl ? Ql[e] : (u == v ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  v4d acc[2][2];
+  mfma_tileP<P>(t, false, qm, acc);
+  __syncthreads();
+  store_quadrantsP<P>(t, acc);
+  for (int e = threadIdx.x; e < P * P; e += NT) {
+    const int u = e / P, v = e % P;
+    qm[u * L + v] = rk ? Qk[e] : (u == v ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  mfma_tileP<P>(qm, true, t, acc);
+  __syncthreads();
+  store_quadrantsP<P>(t, acc);
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * P; e += NT) {
+    const int u = e / P, v = e % P;
+    const int gu = sub_indexP<P>(kp, kq, u), gv = sub_indexP<P>(lp, lq, v);
+    if (gu < n && gv < n) Ap[(long long)gu * n + gv] = (k == l && u > v) ? t[v * L + u] : t[u * L + v];
+  }
+  if (k != l)
+    for (int e = threadIdx.x; e < P * P; e += NT) {
+      const int v = e / P, u = e % P;
+      const int gu = sub_indexP<P>(kp, kq, u), gv = sub_indexP<P>(lp, lq, v);
+      if (gu < n && gv < n) Ap[(long long)gv * n + gu] = t[u * L + v];
+    }
+}
+
+// W[P-row block, I_l] <- W[..., I_l] Q_l
+template <int P>
+__global__ void __launch_bounds__((P / 32) * (P / 32) * 64) k_jac_apply_wP(
+    const JacobiProb* __restrict__ probs, const int4* __restrict__ list, int r, double* W,
+    const double* __restrict__ Qbuf, const int* __restrict__ rotflag, const int* __restrict__ state) {
+  constexpr int L = P + 1, NT = (P / 32) * (P / 32) * 64;
+  __shared__ double t[P * L];
+  __shared__ double qm[P * L];
+  const int4 it = list[blockIdx.x];
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return;
+  const int l = it.z, rb = it.y;
+  const int qb = (int)(pr.q_off / (P * P));
+  if (!rotflag[qb + l]) return;
+  const int n = pr.dim;
+  const int lr = r % pr.rounds;
+  int lp, lq;
+  circle_pair(pr.nbp, lr, l, lp, lq);
+  double* Wp = W + pr.w_off;
+  const double* Ql = Qbuf + pr.q_off + (long long)l * P * P;
+  for (int e = threadIdx.x; e < P * P; e += NT) {
+    const int u = e / P, v = e % P;
+    const int gu = rb * P + u, gv = sub_indexP<P>(lp, lq, v);
+    t[u * L + v] = (gu < n && gv < n) ? Wp[(long long)gu * n + gv] : 0.0;
+    qm[u * L + v] = Ql[e];
+  }
+  __syncthreads();
+  v4d acc[2][2];
+  mfma_tileP<P>(t, false, qm, acc);
+  __syncthreads();
+  store_quadrantsP<P>(t, acc);
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * P; e += NT) {
+    const int u = e / P, v = e % P;
+    const int gu = rb * P + u, gv = sub_indexP<P>(lp, lq, v);
+    if (gu < n && gv < n) Wp[(long long)gu * n + gv] = t[u * L + v];
+  }
+}
+
+template __global__ void k_jac_pairsP<32, 64>(const JacobiProb*, const int2*, int, const double*, double*, int*,
+                                              const double*, const int*, int, unsigned long long*);
+template __global__ void k_jac_apply_aP<32>(const JacobiProb*, const int4*, int, double*, const double*, const int*,
+                                            const int*);
+template __global__ void k_jac_apply_wP<32>(const JacobiProb*, const int4*, int, double*, const double*, const int*,
+                                            const int*);
+
 // initialise W = I and the per-problem scale max|A_ii| and state; molecules
 // that finished their SCC are marked done and skipped
 __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__ probs, const double* __restrict__ A,
