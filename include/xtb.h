@@ -8,8 +8,9 @@
  * xtb_amd/fortran/xtb_capi.f90 on top of the MI355X engine.
  *
  * Supported: GFN2-xTB single points (energy + gradient) of molecules.
- * GFN0/GFN1/GFN-FF, solvation, external charges, CPCM-X and Hessians set an
- * environment error "... is not supported by this GFN2-xTB implementation".
+ * GFN0/GFN1/GFN-FF, solvation, external charges and CPCM-X set an
+ * environment error "... is not supported by this GFN2-xTB implementation";
+ * xtb_hessian runs its 6n displaced single points as one GPU batch.
  * Units: bohr, Hartree, Hartree/bohr, e; orbital energies in Hartree.
  */
 #pragma once
@@ -75,8 +76,15 @@ void xtb_setElectronicTemp(xtb_TEnvironment env, xtb_TCalculator calc, double te
 /* calculations: reference include/xtb.h:230-253 */
 void xtb_cpcmx_calc(xtb_TEnvironment env, xtb_TMolecule mol, xtb_TCalculator calc, xtb_TResults res);
 void xtb_singlepoint(xtb_TEnvironment env, xtb_TMolecule mol, xtb_TCalculator calc, xtb_TResults res);
+/* hessian [3n][3n] (symmetric), step_size in bohr (NULL: 0.005), atom_index_list
+ * 1-based (NULL: all atoms; else up to natoms entries, ended by an entry <= 0),
+ * dipole_gradient [3n][3], polarizability_gradient [3n][6] (zero for GFN2-xTB).
+ * The argument order after `hessian` is the reference implementation's
+ * (src/api/interface.f90:310-312: step before the list); its header lists them
+ * the other way round as int pointers (include/xtb.h:244-252). */
 void xtb_hessian(xtb_TEnvironment env, xtb_TMolecule mol, xtb_TCalculator calc, xtb_TResults res, double* hessian,
-                 int* atom_index_list, int* step_size, double* dipole_gradient, double* polarizability_gradient);
+                 const double* step_size, const int* atom_index_list, double* dipole_gradient,
+                 double* polarizability_gradient);
 
 /* results: reference include/xtb.h:259-342 */
 xtb_TResults xtb_newResults(void);

@@ -106,6 +106,18 @@ int main(void) {
   xtb_delete(res2);
   CHECK(res2 == NULL, "xtb_delete sets NULL");
 
+  /* numerical Hessian from a fresh results object (reference c_api_example.c:242-261) */
+  xtb_delete(res);
+  res = xtb_newResults();
+  double* hess = (double*)calloc(9 * 49, sizeof(double));
+  xtb_hessian(env, mol, calc, res, hess, NULL, NULL, NULL, NULL);
+  CHECK(!xtb_checkEnvironment(env), "hessian");
+  CHECK(fabs(hess[0] - 0.4790088649) < 1.0e-9, "Hessian[0,0] %.12f", hess[0]);
+  CHECK(fabs(hess[3] - -0.0528761190) < 1.0e-9, "Hessian[0,3] %.12f", hess[3]);
+  CHECK(fabs(hess[3] - hess[63]) < 1.0e-9, "Hessian symmetry");
+  CHECK(fabs(hess[9 * 49 - 1] - 0.3636571159) < 1.0e-9, "Hessian[21,21] %.12f", hess[9 * 49 - 1]);
+  free(hess);
+
   /* unsupported features leave an error */
   char solvent[] = "h2o", model[] = "gbsa";
   xtb_setSolvent(env, calc, solvent, NULL, NULL, NULL, model);

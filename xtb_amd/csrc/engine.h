@@ -237,8 +237,8 @@ class Batch {
   // refine W (init I) towards the eigenvectors of A; leaves A <- W^T A W
   void refine(double* Am, double* Wm);
   int oa_max_steps = 0;     // refinement steps before the Jacobi sweeps (0 = off)
-  int inner_sweeps = 1;
-  int jac_p = 32;          // block-Jacobi pivot size (64: 32-wide blocks, 32: 16-wide blocks)    // sweeps of each 64x64 pivot problem per block-Jacobi round
+  int inner_sweeps = 1;    // sweeps of each pivot problem per block-Jacobi round
+  int jac_p = 32;          // block-Jacobi pivot size (64: 32-wide blocks, 32: 16-wide blocks)
   bool use_mfma = true;   // FP64 matrix-core GEMM (else FP64 VALU FMA)
   double* Sm = nullptr;
   double* G = nullptr;

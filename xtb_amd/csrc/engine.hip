@@ -1540,7 +1540,11 @@ int xtbamd_batch_set_guess(xtbamd_batch b, int m, const double* qsh, const doubl
 }
 
 int xtbamd_batch_setup(xtbamd_batch b) {
-  try { b->batch->setup(); return 0; } catch (const std::exception& e) { return fail(e); }
+  try {
+    b->batch->select_all();
+    b->batch->setup();
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
 }
 
 int xtbamd_batch_iterate(xtbamd_batch b) {
@@ -1551,7 +1555,11 @@ int xtbamd_batch_iterate(xtbamd_batch b) {
 }
 
 int xtbamd_batch_gradient(xtbamd_batch b) {
-  try { b->batch->gradient(); return 0; } catch (const std::exception& e) { return fail(e); }
+  try {
+    b->batch->select_all();
+    b->batch->gradient();
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
 }
 
 int xtbamd_batch_run(xtbamd_batch b) {

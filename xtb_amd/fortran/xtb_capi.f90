@@ -51,6 +51,11 @@ module xtbamd_capi
          real(c_double), intent(in) :: qsh(*), dipm(*), qp(*)
          integer(c_int) :: rc
       end function engine_set_guess
+      function engine_setup(handle) result(rc) bind(C, name="xtbamd_batch_setup")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int) :: rc
+      end function engine_setup
       function engine_run(handle) result(rc) bind(C, name="xtbamd_batch_run")
          import :: c_int, c_ptr
          type(c_ptr), value :: handle
@@ -786,18 +791,192 @@ contains
       call env_error(env, "CPCM-X is not supported by this GFN2-xTB implementation", 'xtb_api_cpcmx_calc')
    end subroutine cpcmx_calc_api
 
-   subroutine hessian_api(venv, vmol, vcalc, vres, hess, list, step, dipgrad, polgrad) bind(C, name="xtb_hessian")
+   !> Numerical Hessian by central differences of analytic gradients
+   !> (hessian: src/type/calculator.f90:113-228, xtb_hessian:
+   !> src/api/interface.f90:310-454): all 6 x len(list) displaced single points
+   !> run as one batch on the GPU; each starts from the results object's cached
+   !> wavefunction, or from the EEQ guess at the undisplaced geometry.
+   !> Arguments follow the reference's binding: step (default 0.005 bohr), then
+   !> the 1-based atom list (NULL: all atoms; else read up to natoms entries or
+   !> the first entry <= 0).  The polarizability gradient is zero for GFN2-xTB.
+   subroutine hessian_api(venv, vmol, vcalc, vres, hess_out, c_step, c_list, c_dipgrad, c_polgrad) &
+         & bind(C, name="xtb_hessian")
       type(c_ptr), value :: venv, vmol, vcalc, vres
-      real(c_double), intent(inout) :: hess(*)
-      integer(c_int), intent(in), optional :: list(*)
-      integer(c_int), intent(in), optional :: step
-      real(c_double), intent(inout), optional :: dipgrad(*), polgrad(*)
+      real(c_double), intent(inout) :: hess_out(*)
+      real(c_double), intent(in), optional :: c_step
+      integer(c_int), intent(in), optional :: c_list(*)
+      real(c_double), intent(inout), optional :: c_dipgrad(*)
+      real(c_double), intent(inout), optional :: c_polgrad(*)
+      character(len=*), parameter :: source = 'xtb_api_hessian'
       type(environment_t), pointer :: env
+      type(molecule_t), pointer :: mol
+      type(calculator_t), pointer :: calc
+      type(results_t), pointer :: res
+      type(c_ptr) :: handle
+      type(engine_result), target :: er
+      real(wp) :: step, step2
+      real(wp), allocatable, target :: xyz(:, :, :), hess(:, :), dipgrad(:, :), grad(:, :, :), dip(:, :)
+      real(wp), allocatable, target :: qsh0(:), dipm0(:, :), qp0(:, :)
+      integer(c_int), allocatable :: natoms(:), numbers(:), uhf(:)
+      real(c_double), allocatable :: chrg(:)
+      integer, allocatable :: list(:)
+      integer :: n, nl, nmol, i, j, k, kat, ic, m, ii, nsh
+      integer(c_int) :: rc
+      logical :: conv
+
       if (.not. c_associated(venv)) return
       call c_f_pointer(venv, env)
-      call env_error(env, "Hessian calculations are not supported by this GFN2-xTB implementation", &
-         & 'xtb_api_hessian')
+      if (.not. c_associated(vmol)) then
+         call env_error(env, "Molecular structure data is not allocated", source)
+         return
+      end if
+      call c_f_pointer(vmol, mol)
+      if (.not. c_associated(vcalc)) then
+         call env_error(env, "Singlepoint calculator is not allocated", source)
+         return
+      end if
+      call c_f_pointer(vcalc, calc)
+      if (.not. calc%loaded) then
+         call env_error(env, "No calculator loaded for single point", source)
+         return
+      end if
+      if (.not. c_associated(vres)) then
+         call env_error(env, "Calculation results are not allocated", source)
+         return
+      end if
+      call c_f_pointer(vres, res)
+      n = mol%n
+      step = 0.005_wp
+      if (present(c_step)) step = c_step
+      step2 = 0.5_wp / step
+      if (present(c_list)) then
+         nl = 0
+         do i = 1, n
+            if (c_list(i) <= 0) exit
+            nl = nl + 1
+         end do
+         allocate(list(nl))
+         list = c_list(1:nl)
+         if (any(list > n)) then
+            call env_error(env, "Invalid atom index in the displacement list", source)
+            return
+         end if
+      else
+         nl = n
+         list = [(i, i = 1, n)]
+      end if
+
+      ! the starting wavefunction of every displaced point
+      if (res%has_wfn .and. res%n /= n) res%has_wfn = .false.
+      if (res%has_wfn) then
+         qsh0 = res%qsh
+         dipm0 = res%dipm
+         qp0 = res%qp
+         nsh = res%nsh
+      else
+         natoms = [int(n, c_int)]
+         uhf = [mol%uhf]
+         chrg = [mol%chrg]
+         handle = engine_new(1_c_int, natoms, mol%at, mol%xyz, chrg, uhf, calc%accuracy, calc%etemp, &
+            & int(calc%maxiter, c_int), 0_c_int)
+         if (.not. c_associated(handle)) then
+            call env_error(env, engine_message(), 'xtb_calculator_newWavefunction')
+            return
+         end if
+         rc = engine_setup(handle)
+         call clear_result(er)
+         rc = engine_get(handle, 0_c_int, er)
+         nsh = er%nshells
+         allocate(qsh0(nsh), dipm0(3, n), qp0(6, n))
+         er%qsh = c_loc(qsh0)
+         rc = engine_get(handle, 0_c_int, er)
+         call engine_free(handle)
+         if (rc /= 0) then
+            call env_error(env, engine_message(), 'xtb_calculator_newWavefunction')
+            return
+         end if
+         dipm0 = 0.0_wp
+         qp0 = 0.0_wp
+      end if
+
+      ! 6 x nl displaced geometries: (+step, -step) for each atom and direction
+      nmol = 6 * nl
+      if (allocated(natoms)) deallocate(natoms)
+      if (allocated(uhf)) deallocate(uhf)
+      if (allocated(chrg)) deallocate(chrg)
+      allocate(xyz(3, n, nmol), natoms(nmol), uhf(nmol), chrg(nmol), numbers(n * nmol))
+      do kat = 1, nl
+         do ic = 1, 3
+            do k = 1, 2
+               m = 6 * (kat - 1) + 2 * (ic - 1) + k
+               xyz(:, :, m) = mol%xyz
+               xyz(ic, list(kat), m) = mol%xyz(ic, list(kat)) + merge(step, -step, k == 1)
+               numbers((m - 1) * n + 1:m * n) = mol%at
+            end do
+         end do
+      end do
+      natoms = int(n, c_int)
+      uhf = mol%uhf
+      chrg = mol%chrg
+      handle = engine_new(int(nmol, c_int), natoms, numbers, xyz, chrg, uhf, calc%accuracy, calc%etemp, &
+         & int(calc%maxiter, c_int), 0_c_int)
+      if (.not. c_associated(handle)) then
+         call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
+         return
+      end if
+      do m = 1, nmol
+         rc = engine_set_guess(handle, int(m - 1, c_int), qsh0, dipm0, qp0)
+      end do
+      rc = engine_run(handle)
+      if (rc /= 0) then
+         call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
+         call engine_free(handle)
+         return
+      end if
+      allocate(grad(3, n, nmol), dip(3, nmol))
+      conv = .true.
+      do m = 1, nmol
+         call clear_result(er)
+         er%gradient = c_loc(grad(1, 1, m))
+         rc = engine_get(handle, int(m - 1, c_int), er)
+         dip(:, m) = er%dipole
+         conv = conv .and. er%converged /= 0
+      end do
+      call engine_free(handle)
+
+      allocate(hess(3 * n, 3 * n), dipgrad(3, 3 * n))
+      hess = 0.0_wp
+      dipgrad = 0.0_wp
+      do kat = 1, nl
+         do ic = 1, 3
+            ii = 3 * (list(kat) - 1) + ic
+            m = 6 * (kat - 1) + 2 * (ic - 1) + 1
+            dipgrad(:, ii) = (dip(:, m) - dip(:, m + 1)) * step2
+            do j = 1, n
+               do k = 1, 3
+                  hess(3 * (j - 1) + k, ii) = hess(3 * (j - 1) + k, ii) + (grad(k, j, m) - grad(k, j, m + 1)) * step2
+               end do
+            end do
+         end do
+      end do
+      do i = 1, 3 * n
+         do j = i + 1, 3 * n
+            hess(i, j) = 0.5_wp * (hess(i, j) + hess(j, i))
+            hess(j, i) = hess(i, j)
+         end do
+      end do
+      hess_out(1:9 * n * n) = reshape(hess, [9 * n * n])
+      if (present(c_dipgrad)) c_dipgrad(1:9 * n) = reshape(dipgrad, [9 * n])
+      if (present(c_polgrad)) c_polgrad(1:18 * n) = 0.0_wp
+      if (.not. conv) call env_error(env, "Self consistent charge iterator did not converge", 'scf')
    end subroutine hessian_api
+
+   subroutine clear_result(er)
+      type(engine_result), intent(inout) :: er
+      er%gradient = c_null_ptr; er%charges = c_null_ptr; er%qsh = c_null_ptr; er%dipm = c_null_ptr
+      er%qp = c_null_ptr; er%wbo = c_null_ptr; er%emo = c_null_ptr; er%focc = c_null_ptr
+      er%coeff = c_null_ptr; er%density = c_null_ptr
+   end subroutine clear_result
 
    ! --------------------------------------------------------------------------
    ! results (src/api/results.f90:53-501)
