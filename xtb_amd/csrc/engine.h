@@ -229,7 +229,9 @@ class Batch {
   const Element* elements_dev = nullptr;
   int ntiles = 0;
   int4* d_tiles = nullptr;
-  void jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w = false);
+  void jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w = false,
+              int banded = 0);
+  int banded_passes = 2;   // banded rounds pairs after a warm-start refinement
   // eigenvectors as rows of W: LDS Jacobi up to 64, tridiagonal QL above
   void eigh_rows(JacSet& js, double* Am, double* Wm, double* evout);
   bool use_ql = false;     // tridiagonal QL (rows of W) instead of block Jacobi (columns of W)

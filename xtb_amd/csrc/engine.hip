@@ -484,6 +484,7 @@ void Batch::init_common() {
   if (const char* r = std::getenv("XTBAMD_REFINE_STEPS")) oa_max_steps = std::atoi(r);
   if (const char* r = std::getenv("XTBAMD_INNER_SWEEPS")) inner_sweeps = std::max(1, std::atoi(r));
   if (const char* r = std::getenv("XTBAMD_PIVOT")) jac_p = std::atoi(r) == 64 ? 64 : 32;
+  if (const char* r = std::getenv("XTBAMD_BANDED")) banded_passes = std::max(0, std::atoi(r));
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
   if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
   intcut = std::max(20.0, 25.0 - 10.0 * std::log10(opt.accuracy));
@@ -849,7 +850,7 @@ Batch::~Batch() {
 }
 
 // ---------------------------------------------------------------------------
-void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w) {
+void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w, int banded) {
   const int nb = nmol;
   hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
                      check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0);
@@ -857,16 +858,21 @@ void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
                        (const double*)js.scale, js.state, keep_w ? 1 : 0, 0);
   if (js.npairs) {
+    // banded rounds first (neighbouring blocks only), then cyclic sweeps;
     // every round starts with the convergence test of the problems at a
     // sweep boundary; the host reads the number still running once per
     // max_rounds rounds
+    std::vector<int> sched;
+    for (int b = 0; b < 2 * banded; ++b) sched.push_back(kBanded + b);
     const int rmax = (kMaxSweeps + 1) * js.max_rounds;
-    for (int r = 0; r <= rmax; ++r) {
+    for (int r = 0; r <= rmax; ++r) sched.push_back(r);
+    for (size_t si = 0; si < sched.size(); ++si) {
+      const int r = sched[si];
       hipLaunchKernelGGL(k_jac_offmax, dim3(js.nto), dim3(256), 0, stream, js.d_probs, js.d_to, (const double*)Am,
                          (const int*)js.state, js.offmax, r);
       hipLaunchKernelGGL(k_jac_mark, dim3(1), dim3(256), 0, stream, js.d_probs, nb, (const double*)js.scale,
                          js.state, js.offmax, r, jac_nactive);
-      if (r % js.max_rounds == 0) {
+      if (r < kBanded && r % js.max_rounds == 0) {
         HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIPCHECK(hipStreamSynchronize(stream));
         if (h_pinned[0] <= 0) break;
@@ -1195,7 +1201,7 @@ bool Batch::scc_iteration() {
     eigh_rows(jac_n, A, W, emo);
     gemm(4, C, W, T, nullptr, perm, true);    // T = C W(perm, :)^T
   } else {
-    jacobi(jac_n, A, W, emo, true, warm);
+    jacobi(jac_n, A, W, emo, true, warm, warm ? banded_passes : 0);
     gemm(3, C, W, T, nullptr, perm, true);    // T = C W(:, perm)
   }
   hipLaunchKernelGGL(k_copy_active, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream,

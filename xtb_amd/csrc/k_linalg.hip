@@ -317,6 +317,34 @@ __device__ __forceinline__ void circle_pair(int nbp, int r, int k, int& bp, int&
   if (bp > bq) { const int t = bp; bp = bq; bq = t; }
 }
 
+// Round numbers from kBanded on select the banded schedule: even rounds pair
+// blocks (2k, 2k+1), odd rounds (2k+1, 2k+2) and the two end blocks; it
+// reaches every coupling between neighbouring blocks in two rounds.
+constexpr int kBanded = 1 << 28;
+
+__device__ __forceinline__ void jac_pair(int nbp, int rounds, int r, int k, int& bp, int& bq) {
+  if (r < kBanded) {
+    circle_pair(nbp, r % rounds, k, bp, bq);
+    return;
+  }
+  if (((r - kBanded) & 1) == 0) {
+    bp = 2 * k;
+    bq = 2 * k + 1;
+  } else if (2 * k + 2 < nbp) {
+    bp = 2 * k + 1;
+    bq = 2 * k + 2;
+  } else {
+    bp = 0;
+    bq = nbp - 1;
+  }
+}
+
+// a problem tests convergence when it starts a sweep (circle schedule) or a
+// pair of rounds (banded schedule)
+__device__ __forceinline__ bool jac_boundary(int rounds, int r) {
+  return r < kBanded ? (r % rounds == 0) : (((r - kBanded) & 1) == 0);
+}
+
 // state per problem: [0] done, [1] failed, [5] outer sweeps (block path) or
 // inner rounds with rotations (dim <= 64)
 constexpr int kJacState = 8;
@@ -393,9 +421,8 @@ __global__ void __launch_bounds__(256) k_jac_pairs(const JacobiProb* __restrict_
   const JacobiProb pr = probs[it.x];
   if (state[it.x * kJacState]) return;
   const int n = pr.dim;
-  const int lr = r % pr.rounds;
   int bp, bq;
-  circle_pair(pr.nbp, lr, it.y, bp, bq);
+  jac_pair(pr.nbp, pr.rounds, r, it.y, bp, bq);
   const double* Ap = A + pr.a_off;
   for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
     const int u = e >> 6, v = e & 63;
@@ -431,10 +458,9 @@ __global__ void __launch_bounds__(256) k_jac_apply_a(const JacobiProb* __restric
   const int rk = rotflag[qb + k], rl = rotflag[qb + l];
   if (!rk && !rl) return;
   const int n = pr.dim;
-  const int lr = r % pr.rounds;
   int kp, kq, lp, lq;
-  circle_pair(pr.nbp, lr, k, kp, kq);
-  circle_pair(pr.nbp, lr, l, lp, lq);
+  jac_pair(pr.nbp, pr.rounds, r, k, kp, kq);
+  jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
   double* Ap = A + pr.a_off;
   const double* Qk = Qbuf + pr.q_off + (long long)k * 4096;
   const double* Ql = Qbuf + pr.q_off + (long long)l * 4096;
@@ -519,9 +545,8 @@ __global__ void __launch_bounds__(256) k_jac_apply_w(const JacobiProb* __restric
   const int qb = (int)(pr.q_off / 4096);
   if (!rotflag[qb + l]) return;
   const int n = pr.dim;
-  const int lr = r % pr.rounds;
   int lp, lq;
-  circle_pair(pr.nbp, lr, l, lp, lq);
+  jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
   double* Wp = W + pr.w_off;
   const double* Ql = Qbuf + pr.q_off + (long long)l * 4096;
   for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
@@ -616,10 +641,9 @@ __global__ void __launch_bounds__(256) k_jac_apply_a_mfma(const JacobiProb* __re
   const int rk = rotflag[qb + k], rl = rotflag[qb + l];
   if (!rk && !rl) return;
   const int n = pr.dim;
-  const int lr = r % pr.rounds;
   int kp, kq, lp, lq;
-  circle_pair(pr.nbp, lr, k, kp, kq);
-  circle_pair(pr.nbp, lr, l, lp, lq);
+  jac_pair(pr.nbp, pr.rounds, r, k, kp, kq);
+  jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
   double* Ap = A + pr.a_off;
   const double* Qk = Qbuf + pr.q_off + (long long)k * 4096;
   const double* Ql = Qbuf + pr.q_off + (long long)l * 4096;
@@ -677,9 +701,8 @@ __global__ void __launch_bounds__(256) k_jac_apply_w_mfma(const JacobiProb* __re
   const int qb = (int)(pr.q_off / 4096);
   if (!rotflag[qb + l]) return;
   const int n = pr.dim;
-  const int lr = r % pr.rounds;
   int lp, lq;
-  circle_pair(pr.nbp, lr, l, lp, lq);
+  jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
   double* Wp = W + pr.w_off;
   const double* Ql = Qbuf + pr.q_off + (long long)l * 4096;
   for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
@@ -833,9 +856,8 @@ __global__ void __launch_bounds__(NT) k_jac_pairsP(const JacobiProb* __restrict_
   const JacobiProb pr = probs[it.x];
   if (state[it.x * kJacState]) return;
   const int n = pr.dim;
-  const int lr = r % pr.rounds;
   int bp, bq;
-  circle_pair(pr.nbp, lr, it.y, bp, bq);
+  jac_pair(pr.nbp, pr.rounds, r, it.y, bp, bq);
   const double* Ap = A + pr.a_off;
   for (int e = threadIdx.x; e < P * P; e += NT) {
     const int u = e / P, v = e % P;
@@ -912,10 +934,9 @@ __global__ void __launch_bounds__((P / 32) * (P / 32) * 64) k_jac_apply_aP(
   const int rk = rotflag[qb + k], rl = rotflag[qb + l];
   if (!rk && !rl) return;
   const int n = pr.dim;
-  const int lr = r % pr.rounds;
   int kp, kq, lp, lq;
-  circle_pair(pr.nbp, lr, k, kp, kq);
-  circle_pair(pr.nbp, lr, l, lp, lq);
+  jac_pair(pr.nbp, pr.rounds, r, k, kp, kq);
+  jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
   double* Ap = A + pr.a_off;
   const double* Qk = Qbuf + pr.q_off + (long long)k * P * P;
   const double* Ql = Qbuf + pr.q_off + (long long)l * P * P;
@@ -967,9 +988,8 @@ __global__ void __launch_bounds__((P / 32) * (P / 32) * 64) k_jac_apply_wP(
   const int qb = (int)(pr.q_off / (P * P));
   if (!rotflag[qb + l]) return;
   const int n = pr.dim;
-  const int lr = r % pr.rounds;
   int lp, lq;
-  circle_pair(pr.nbp, lr, l, lp, lq);
+  jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
   double* Wp = W + pr.w_off;
   const double* Ql = Qbuf + pr.q_off + (long long)l * P * P;
   for (int e = threadIdx.x; e < P * P; e += NT) {
@@ -1041,7 +1061,7 @@ __global__ void __launch_bounds__(256) k_jac_offmax(const JacobiProb* __restrict
   __shared__ double sh[4];
   const int4 it = list[blockIdx.x];
   const JacobiProb pr = probs[it.x];
-  if (state[it.x * kJacState] || r % pr.rounds != 0) return;
+  if (state[it.x * kJacState] || !jac_boundary(pr.rounds, r)) return;
   const int n = pr.dim;
   const double* Ap = A + pr.a_off;
   double m = 0.0;
@@ -1067,8 +1087,8 @@ __global__ void __launch_bounds__(256) k_jac_mark(const JacobiProb* __restrict__
     if (st[0]) continue;
     const JacobiProb pr = probs[p];
     if (pr.dim <= 64) continue;   // solved by k_jac_small
-    if (r % pr.rounds == 0) {
-      const int sweep = r / pr.rounds;
+    if (jac_boundary(pr.rounds, r)) {
+      const int sweep = r < kBanded ? r / pr.rounds : 0;
       const double m = __longlong_as_double((long long)offmax[p]);
       offmax[p] = 0ull;
       if (m <= 1e-12 * scale[p]) {
