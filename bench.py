@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-qps", type=float, default=40.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=16.0)
+    ap.add_argument("--copy-probe-mb", type=float, default=0.0,
+                    help="run one plain device copy of this size first (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -124,6 +126,8 @@ def main():
 
     import xtb_amd  # first GPU use
     batch_in = [(z, x, 0.0, 0) for z, x in mols]
+    if args.copy_probe_mb > 0:
+        xtb_amd.library().xtbamd_copy_probe(int(args.copy_probe_mb * 1048576))
 
     # plain path: one molecule per batch, synchronous
     lat = []

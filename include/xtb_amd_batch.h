@@ -90,6 +90,9 @@ int xtbamd_server_result(xtbamd_server s, long long id, xtbamd_result* out);
 int xtbamd_server_pending(xtbamd_server s, int* queued, int* running);
 void xtbamd_server_free(xtbamd_server s);
 
+/* copies `bytes` bytes device to device with one plain copy kernel (k_copy):
+ * the reference point for hardware byte counters */
+int xtbamd_copy_probe(long long bytes);
 /* device / runtime information: number of visible GPUs */
 int xtbamd_device_count(void);
 

@@ -1501,6 +1501,21 @@ extern "C" {
 
 const char* xtbamd_last_error(void) { return g_last_error.c_str(); }
 
+int xtbamd_copy_probe(long long bytes) {
+  try {
+    const long long n = std::max(1LL, bytes / (long long)sizeof(double));
+    double *a = nullptr, *b = nullptr;
+    HIPCHECK(hipMalloc(&a, n * sizeof(double)));
+    HIPCHECK(hipMalloc(&b, n * sizeof(double)));
+    hipLaunchKernelGGL(xtbamd::k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, a, n, 1.0);
+    hipLaunchKernelGGL(xtbamd::k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, (const double*)a, b, n);
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipFree(a));
+    HIPCHECK(hipFree(b));
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
 int xtbamd_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;

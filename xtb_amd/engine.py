@@ -47,6 +47,8 @@ def library():
     lib.xtbamd_batch_free.argtypes = [ctypes.c_void_p]
     lib.xtbamd_batch_free.restype = None
     lib.xtbamd_device_count.restype = ctypes.c_int
+    lib.xtbamd_copy_probe.argtypes = [ctypes.c_longlong]
+    lib.xtbamd_copy_probe.restype = ctypes.c_int
     lib.xtbamd_server_new.restype = ctypes.c_void_p
     lib.xtbamd_server_new.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int]
