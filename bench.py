@@ -20,6 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from xtb_amd import workloads  # noqa: E402
+from xtb_amd.profiling import traffic_table  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
@@ -187,17 +188,25 @@ def main():
         "roofline": None,
         "cpu_baseline": cpu,
     }
+    ql = os.environ.get("XTBAMD_EIG") == "ql"
     names = {"gemm": "k_gemm (batched FP64 GEMM on the matrix cores)",
-             "jacobi": os.environ.get("XTBAMD_EIG") == "ql" and
-             "k_tred_ql (tridiagonalisation + QL eigensolver, one workgroup per molecule)" or
+             "jacobi": ql and "k_tred_ql (tridiagonalisation + QL eigensolver, one workgroup per molecule)" or
              "k_jac_pairsP (32x32 pivot problems of the block-Jacobi eigensolver, in LDS)"}
+    short = {"gemm": "k_gemm_mfma", "jacobi": ql and "k_tred_ql" or "k_jac_pairsP"}
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs of this
+    # workload (copy-calibrated); only valid for the default workload and schedule
+    table = {}
+    if args.nmol == 512 and args.seed == 2024 and not any(
+            k.startswith("XTBAMD_") for k in os.environ):
+        table = traffic_table(os.path.join(ROOT, "profiles", "batched512_hbm_traffic.md"))
     lines = {}
     for k, (n, tms, work) in prof.items():
         if not n:
             continue
         ach = work / (tms * 1e-3) / 1e12
         lines[k] = {"kernel": names[k], "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None, "launches": n,
+                    "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": table.get(short[k]),
+                    "launches": n,
                     "avg_us": 1e3 * tms / n, "flops_per_launch": work / n,
                     "share_of_step": tms / (args.steps * ms)}
     if lines:

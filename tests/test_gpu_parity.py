@@ -94,6 +94,47 @@ def test_alkane_c64():
     _cmp(r, o, "alkane64")
 
 
+def test_water96_device_wide_eeq():
+    """96 H2O (288 atoms, 576 AOs): the 289-unknown EEQ system is factorised
+    column by column across the device rather than in one workgroup."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    z, x = w.water_cluster(96, 5)
+    r = xa.singlepoint(z, x)
+    o = po.singlepoint(z, x, lapack=True, threads=8)
+    _cmp(r, o, "water96")
+    assert r["iterations"] == o["iterations"]
+
+
+def test_water700_against_oracle_fixture():
+    """The 700-water cluster (2100 atoms, 4200 AOs) against the oracle's values
+    committed in tests/golden/water700_oracle.json."""
+    import json
+    import os
+    xa = _engine()
+    from xtb_amd import workloads as w
+    with open(os.path.join(os.path.dirname(__file__), "golden", "water700_oracle.json")) as f:
+        fx = json.load(f)
+    z, x = w.large_water_cluster(2024)
+    b = xa.Batch([(z, x, 0.0, 0)]).run()
+    r = b.result(0)
+    b.close()
+    g = np.asarray(r["gradient"])
+    q = np.asarray(r["charges"])
+    dE = abs(r["energy"] - fx["e_total"])
+    print(f"water700: dE={dE:.2e} iters={r['iterations']}/{fx['iterations']}", flush=True)
+    assert r["converged"] and r["nao"] == fx["nao"] == 4200
+    assert r["iterations"] == fx["iterations"]
+    assert dE <= TOL_E
+    assert abs(np.linalg.norm(g) - fx["gnorm"]) <= TOL_G
+    assert np.max(np.abs(q[:24] - fx["q_first"])) <= TOL_Q
+    assert np.max(np.abs(g[:8] - np.asarray(fx["gradient_first"]))) <= TOL_G
+    assert abs(np.abs(q).sum() - fx["q_abs_sum"]) <= TOL_Q * len(q)
+    assert abs(np.abs(g).sum() - fx["gradient_abs_sum"]) <= TOL_G * g.size
+    assert np.max(np.abs(np.asarray(r["dipole"]) - fx["dipole"])) <= 1e-4
+    assert abs(r["hl_gap_ev"] - fx["hl_gap_ev"]) <= 1e-4
+
+
 def test_mixed_batch_vs_oracle():
     """16 molecules of the batched workload's distribution, as one batch."""
     xa = _engine()

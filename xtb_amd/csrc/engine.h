@@ -220,6 +220,7 @@ class Batch {
   int* jac_nactive = nullptr;
   int* perm = nullptr;
   int* eeq_failed = nullptr;
+  int* eeq_big = nullptr;        // sub-list indices of EEQ systems factorised column by column
   double* ns_c = nullptr;
   unsigned long long* ns_res = nullptr;
   int *ns_stage = nullptr, *ns_act = nullptr, *ns_failed = nullptr;
@@ -229,7 +230,8 @@ class Batch {
   const Element* elements_dev = nullptr;
   int ntiles = 0;
   int4* d_tiles = nullptr;
-  void jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w = false,
+  // returns true when W comes out as rows (row j = eigenvector j)
+  bool jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w = false,
               int banded = 0);
   int banded_passes = 2;   // banded rounds pairs after a warm-start refinement
   // eigenvectors as rows of W: LDS Jacobi up to 64, tridiagonal QL above
@@ -240,6 +242,8 @@ class Batch {
   void refine(double* Am, double* Wm);
   int oa_max_steps = 0;     // refinement steps before the Jacobi sweeps (0 = off)
   int inner_sweeps = 1;    // sweeps of each pivot problem per block-Jacobi round
+  bool pivot_lds = false;
+  bool apply_lds = false;  // 32x32 pivots: LDS tile updates with W as columns instead of register ones with W as rows  // 32x32 pivots: LDS kernel instead of the register-resident one
   int jac_p = 32;          // block-Jacobi pivot size (64: 32-wide blocks, 32: 16-wide blocks)
   bool use_mfma = true;   // FP64 matrix-core GEMM (else FP64 VALU FMA)
   double* Sm = nullptr;

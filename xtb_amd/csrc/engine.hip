@@ -44,15 +44,22 @@ __global__ void k_copy(const double* a, double* b, long long n) {
   if (i < n) b[i] = a[i];
 }
 
+// EEQ systems with at least this many unknowns are factorised column by column
+// across the whole device instead of inside one workgroup
+constexpr int kEeqBigNe = 256;
+
 // EEQ charges: Gaussian elimination with partial pivoting of the constrained
 // (n+1) x (n+1) system E y = x in place (the reference solves it with dsysv,
 // src/eeq_model.f90:678-1015), then the shell charges (iniqshell:
 // src/scc_core.f90:1475-1506); uniform charges if the solve fails
 // (src/xtb/calculator.f90:479-484).  One workgroup per molecule.
+// Systems of at least big_ne unknowns arrive factorised by k_eeq_lu_pivot /
+// k_eeq_lu_update (the same elimination spread over the whole device); this
+// kernel then only substitutes.
 __global__ void __launch_bounds__(256) k_eeq_gesv(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
                                                   const int* __restrict__ at, const int* __restrict__ at_sh0, double* E,
                                                   double* xE, double* yE, double* q, double* qsh,
-                                                  const double* __restrict__ zsh, int* failed) {
+                                                  const double* __restrict__ zsh, int* failed, int big_ne) {
   __shared__ double sh[4];
   __shared__ double shv[4];
   __shared__ int shi[4];
@@ -63,8 +70,9 @@ __global__ void __launch_bounds__(256) k_eeq_gesv(const MolDesc* __restrict__ de
   double* b = xE + md.y0;
   double* y = yE + md.y0;
   const int tid = threadIdx.x;
-  bool ok = true;
-  for (int k = 0; k < ne; ++k) {
+  const bool factored = ne >= big_ne;
+  bool ok = !(factored && failed[md.mol]);
+  for (int k = 0; k < (factored ? 0 : ne); ++k) {
     // pivot: largest |M[i][k]|, i >= k, smallest index on ties
     double best = -1.0;
     int bi = k;
@@ -140,6 +148,81 @@ __global__ void __launch_bounds__(256) k_eeq_gesv(const MolDesc* __restrict__ de
       qsh[k] = (zsh[k] / e.zval) * q[md.a0 + i];
     }
   }
+}
+
+__global__ void k_eeq_reset(const MolDesc* __restrict__ desc, int ns, int* failed) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ns) failed[desc[k].mol] = 0;
+}
+
+// Column k of the LU factorisation with partial pivoting of large EEQ systems
+// (one workgroup per listed molecule): pivot search with the same tie rule as
+// k_eeq_gesv, row swap, multipliers and the right-hand-side update.
+__global__ void __launch_bounds__(256) k_eeq_lu_pivot(const MolDesc* __restrict__ desc, const int* __restrict__ list,
+                                                      int k, double* E, double* xE, int* failed) {
+  __shared__ double shv[4];
+  __shared__ int shi[4];
+  __shared__ int piv;
+  const MolDesc md = desc[list[blockIdx.x]];
+  const int ne = md.ne;
+  if (k >= ne || failed[md.mol]) return;
+  double* M = E + md.e0;
+  double* b = xE + md.y0;
+  const int tid = threadIdx.x;
+  double best = -1.0;
+  int bi = k;
+  for (int i = k + tid; i < ne; i += blockDim.x) {
+    const double v = fabs(M[(long long)i * ne + k]);
+    if (v > best) { best = v; bi = i; }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(best, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+  }
+  if ((tid & 63) == 0) { shv[tid >> 6] = best; shi[tid >> 6] = bi; }
+  __syncthreads();
+  if (tid == 0) {
+    double bv = shv[0];
+    int bx = shi[0];
+    for (int w = 1; w < 4; ++w)
+      if (shv[w] > bv || (shv[w] == bv && shi[w] < bx)) { bv = shv[w]; bx = shi[w]; }
+    piv = (bv > 0.0) ? bx : -1;
+    if (piv < 0) failed[md.mol] = 1;
+  }
+  __syncthreads();
+  const int p = piv;
+  if (p < 0) return;
+  if (p != k) {
+    for (int j = tid; j < ne; j += blockDim.x) {
+      const double t = M[(long long)k * ne + j];
+      M[(long long)k * ne + j] = M[(long long)p * ne + j];
+      M[(long long)p * ne + j] = t;
+    }
+    if (tid == 0) { const double t = b[k]; b[k] = b[p]; b[p] = t; }
+    __syncthreads();
+  }
+  const double dkk = M[(long long)k * ne + k], bk = b[k];
+  for (int i = k + 1 + tid; i < ne; i += blockDim.x) {
+    const double l = M[(long long)i * ne + k] / dkk;
+    M[(long long)i * ne + k] = l;
+    b[i] -= l * bk;
+  }
+}
+
+// rank-1 update of the trailing block after column k: one workgroup per row
+__global__ void __launch_bounds__(256) k_eeq_lu_update(const MolDesc* __restrict__ desc, const int* __restrict__ list,
+                                                       int k, double* E, const int* __restrict__ failed) {
+  const MolDesc md = desc[list[blockIdx.y]];
+  const int ne = md.ne;
+  const int i = k + 1 + blockIdx.x;
+  if (i >= ne || failed[md.mol]) return;
+  double* M = E + md.e0;
+  const double l = M[(long long)i * ne + k];
+  double* Mi = M + (long long)i * ne;
+  const double* Mk = M + (long long)k * ne;
+  for (int j = k + 1 + threadIdx.x; j < ne; j += blockDim.x) Mi[j] -= l * Mk[j];
 }
 
 // zero the atomic dipoles and quadrupoles (or the gradient) of the selected molecules
@@ -484,6 +567,8 @@ void Batch::init_common() {
   if (const char* r = std::getenv("XTBAMD_REFINE_STEPS")) oa_max_steps = std::atoi(r);
   if (const char* r = std::getenv("XTBAMD_INNER_SWEEPS")) inner_sweeps = std::max(1, std::atoi(r));
   if (const char* r = std::getenv("XTBAMD_PIVOT")) jac_p = std::atoi(r) == 64 ? 64 : 32;
+  if (const char* r = std::getenv("XTBAMD_PIVOT_KERNEL")) pivot_lds = std::strcmp(r, "lds") == 0;
+  if (const char* r = std::getenv("XTBAMD_APPLY_KERNEL")) apply_lds = std::strcmp(r, "lds") == 0;
   if (const char* r = std::getenv("XTBAMD_BANDED")) banded_passes = std::max(0, std::atoi(r));
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
   if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
@@ -557,6 +642,7 @@ void Batch::allocate(const Capacity& c) {
   G = alloc<double>(tmm);
   jac_nactive = alloc<int>(1);
   eeq_failed = alloc<int>(nmol);
+  eeq_big = alloc<int>(nmol);
   eig_failed = alloc<int>(nmol);
   HIPCHECK(hipMemset(eig_failed, 0, nmol * sizeof(int)));
   d_sub = alloc<MolDesc>(nmol);
@@ -850,13 +936,14 @@ Batch::~Batch() {
 }
 
 // ---------------------------------------------------------------------------
-void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w, int banded) {
+bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w, int banded) {
   const int nb = nmol;
+  const bool rows = jac_p == 32 && !apply_lds && !keep_w && banded == 0;
   hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
                      check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0);
   if (js.nsmall)
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
-                       (const double*)js.scale, js.state, keep_w ? 1 : 0, 0);
+                       (const double*)js.scale, js.state, keep_w ? 1 : 0, rows ? 1 : 0);
   if (js.npairs) {
     // banded rounds first (neighbouring blocks only), then cyclic sweeps;
     // every round starts with the convergence test of the problems at a
@@ -880,14 +967,26 @@ void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
       unsigned long long* jw = (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr;
       if (jac_p == 32) {
         prof_begin(2);
-        hipLaunchKernelGGL((k_jac_pairsP<32, 64>), dim3(js.npairs), dim3(64), 0, stream, js.d_probs, js.d_pairs, r,
-                           (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
-                           inner_sweeps, jw);
+        if (pivot_lds)
+          hipLaunchKernelGGL((k_jac_pairsP<32, 64>), dim3(js.npairs), dim3(64), 0, stream, js.d_probs, js.d_pairs, r,
+                             (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
+                             inner_sweeps, jw);
+        else
+          hipLaunchKernelGGL(k_jac_pairsR, dim3(js.npairs), dim3(64), 0, stream, js.d_probs, js.d_pairs, r,
+                             (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
+                             inner_sweeps, jw);
         prof_end(2, nullptr, 0);
-        hipLaunchKernelGGL((k_jac_apply_aP<32>), dim3(js.nta), dim3(64), 0, stream, js.d_probs, js.d_ta, r, Am,
-                           (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
-        hipLaunchKernelGGL((k_jac_apply_wP<32>), dim3(js.ntw), dim3(64), 0, stream, js.d_probs, js.d_tw, r, Wm,
-                           (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+        if (rows) {
+          hipLaunchKernelGGL(k_jac_apply_aR, dim3(js.nta), dim3(64), 0, stream, js.d_probs, js.d_ta, r, Am,
+                             (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+          hipLaunchKernelGGL(k_jac_apply_wR, dim3(js.ntw), dim3(64), 0, stream, js.d_probs, js.d_tw, r, Wm,
+                             (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+        } else {
+          hipLaunchKernelGGL((k_jac_apply_aP<32>), dim3(js.nta), dim3(64), 0, stream, js.d_probs, js.d_ta, r, Am,
+                             (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+          hipLaunchKernelGGL((k_jac_apply_wP<32>), dim3(js.ntw), dim3(64), 0, stream, js.d_probs, js.d_tw, r, Wm,
+                             (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+        }
         jacobi_rounds_ += 1;
         continue;
       }
@@ -913,6 +1012,7 @@ void Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   if (evout)
     hipLaunchKernelGGL(k_jac_sort, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm,
                        check_active ? (const MolState*)d_state : (const MolState*)nullptr, 1 << 30);
+  return rows;
 }
 
 void Batch::eigh_rows(JacSet& js, double* Am, double* Wm, double* evout) {
@@ -1144,8 +1244,28 @@ void Batch::setup() {
     const long long ne = sub_max_n + 1;
     hipLaunchKernelGGL(k_eeq_matrix, dim3(grid1(ne * ne, 256), ns), dim3(256), 0, stream, d_sub, el, at, xyz, cn_eeq,
                        E, xE);
+    hipLaunchKernelGGL(k_eeq_reset, dim3(grid1(ns, 64)), dim3(64), 0, stream, d_sub, ns, eeq_failed);
+    // large systems: one launch pair per column over the whole device
+    std::vector<int> big;
+    int big_max = 0;
+    for (int k = 0; k < ns; ++k)
+      if (desc[sel_[k]].ne >= kEeqBigNe) {
+        big.push_back(k);
+        big_max = std::max(big_max, desc[sel_[k]].ne);
+      }
+    if (!big.empty()) {
+      HIPCHECK(hipMemcpyAsync(eeq_big, big.data(), big.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+      const int nb = (int)big.size();
+      for (int k = 0; k < big_max; ++k) {
+        hipLaunchKernelGGL(k_eeq_lu_pivot, dim3(nb), dim3(256), 0, stream, d_sub, (const int*)eeq_big, k, E, xE,
+                           eeq_failed);
+        if (big_max - k - 1 > 0)
+          hipLaunchKernelGGL(k_eeq_lu_update, dim3(big_max - k - 1, nb), dim3(256), 0, stream, d_sub,
+                             (const int*)eeq_big, k, E, (const int*)eeq_failed);
+      }
+    }
     hipLaunchKernelGGL(k_eeq_gesv, dim3(ns), dim3(256), 0, stream, d_sub, el, at, at_sh0, E, xE, yE, q, qsh,
-                       (const double*)zsh, eeq_failed);
+                       (const double*)zsh, eeq_failed, kEeqBigNe);
   } else {
     hipLaunchKernelGGL(k_sad_guess, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, at_sh0, q,
                        qsh, (const double*)zsh);
@@ -1201,8 +1321,10 @@ bool Batch::scc_iteration() {
     eigh_rows(jac_n, A, W, emo);
     gemm(4, C, W, T, nullptr, perm, true);    // T = C W(perm, :)^T
   } else {
-    jacobi(jac_n, A, W, emo, true, warm, warm ? banded_passes : 0);
-    gemm(3, C, W, T, nullptr, perm, true);    // T = C W(:, perm)
+    if (jacobi(jac_n, A, W, emo, true, warm, warm ? banded_passes : 0))
+      gemm(4, C, W, T, nullptr, perm, true);  // T = C W(perm, :)^T
+    else
+      gemm(3, C, W, T, nullptr, perm, true);  // T = C W(:, perm)
   }
   hipLaunchKernelGGL(k_copy_active, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream,
                      d_desc, (const MolState*)d_state, (const double*)T, C);

@@ -1018,6 +1018,343 @@ template __global__ void k_jac_apply_aP<32>(const JacobiProb*, const int4*, int,
 template __global__ void k_jac_apply_wP<32>(const JacobiProb*, const int4*, int, double*, const double*, const int*,
                                             const int*);
 
+// ---------------------------------------------------------------------------
+// Register-resident Jacobi sweep of a 32 x 32 pivot problem, one wave.
+//
+// The 32 indices sit in 16 slot pairs k = 0..15 (a top and a bottom slot;
+// initially index k and index 16 + k).  Lane L holds column pair c = L & 15 of
+// the row pairs 4g..4g+3 (g = L >> 4) of A, 2 x 2 doubles per pair block, and
+// the column pair c of Q for rows g, g+4, ..., g+28.  An inner round rotates
+// every pair (top, bottom) at once, the lane owning the diagonal block of a
+// pair computing its rotation, then moves the slots by the circle method: the
+// top of pair 0 stays, the bottom of pair 0 becomes the top of pair 1, tops
+// move one pair up, the top of pair 15 becomes its bottom and bottoms move one
+// pair down.  31 rounds meet every pair of indices once and restore the slot
+// order, so Q comes out with its columns in index order.  Row moves stay in a
+// lane except across the 16-lane groups (one shuffle each way); column moves
+// are DPP shifts within a 16-lane row.  Only the rotation cosines and sines go
+// through LDS.  The rotation is the one of the LDS kernels (Rutishauser's
+// formulas, exact update of the pivot's diagonal pair).
+
+// src moved one lane within each row of 16: CTRL 0x111 reads lane - 1
+// (row_shr:1), 0x101 reads lane + 1 (row_shl:1); a lane without a source
+// (the first, resp. last, of its row) returns old
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_f64(double old, double src) {
+  const long long o = __double_as_longlong(old), b = __double_as_longlong(src);
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned int)(o & 0xffffffffll), (int)(unsigned int)(b & 0xffffffffll),
+                                             CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// circle-method move of a (top, bottom) column pair held by lane c of a row:
+// top(c) <- top(c-1) for c >= 2, bottom(0) for c = 1, itself for c = 0;
+// bottom(c) <- bottom(c+1) for c <= 14, top(15) for c = 15
+__device__ __forceinline__ void circle_cols(double& top, double& bot, int c) {
+  const double send = (c == 0) ? bot : top;
+  const double nt = dpp_row_f64<0x111>(top, send);
+  const double nb = dpp_row_f64<0x101>(top, bot);
+  top = nt;
+  bot = nb;
+}
+
+__global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
+                                                   int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
+                                                   const double* __restrict__ scale_in, const int* __restrict__ state,
+                                                   int inner_sweeps, unsigned long long* work) {
+  __shared__ double2 rcs[16];
+  __shared__ double red[2];
+  const int2 it = list[blockIdx.x];
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return;
+  const int n = pr.dim;
+  int bp, bq;
+  jac_pair(pr.nbp, pr.rounds, r, it.y, bp, bq);
+  const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+  const double* Ap = A + pr.a_off;
+  double a[4][4];   // [row pair 4g+i][2 * row half + column half]
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int hr = 0; hr < 2; ++hr)
+#pragma unroll
+      for (int hc = 0; hc < 2; ++hc) {
+        const int gu = (hr ? 16 * bq : 16 * bp) + 4 * g + i, gv = (hc ? 16 * bq : 16 * bp) + c;
+        a[i][2 * hr + hc] = (gu < n && gv < n) ? Ap[(long long)gu * n + gv] : 0.0;
+      }
+  double qv[8][2];  // Q[g + 4j][slot (c, half)]
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int u = g + 4 * j;
+    qv[j][0] = (u == c) ? 1.0 : 0.0;
+    qv[j][1] = (u == c + 16) ? 1.0 : 0.0;
+  }
+  const double tol = 1e-12 * scale_in[it.x];
+  const bool diag = (c >> 2) == g;   // this lane owns the diagonal block of pair c
+  const int id = c & 3;
+  int total = 0;
+  for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
+    double m = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const bool on_diag = diag && i == id && (e == 0 || e == 3);
+        if (!on_diag) m = fmax(m, fabs(a[i][e]));
+      }
+    m = block_max<64>(m, red);
+    if (m <= tol) break;
+    for (int rr = 0; rr < 31; ++rr) {
+      double app = a[0][0], apq = a[0][1], aqq = a[0][3];
+#pragma unroll
+      for (int i = 1; i < 4; ++i)
+        if (id == i) { app = a[i][0]; apq = a[i][1]; aqq = a[i][3]; }
+      double cr = 1.0, sr = 0.0, tr = 0.0;
+      bool rot = false;
+      if (diag && fabs(apq) > tol) {
+        const double theta = (aqq - app) / (2.0 * apq);
+        tr = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        cr = 1.0 / sqrt(tr * tr + 1.0);
+        sr = tr * cr;
+        rot = true;
+      }
+      const unsigned long long any = __ballot(rot);
+      if (any) {
+        total += 1;
+        if (diag) rcs[c] = make_double2(cr, sr);
+        __syncthreads();
+        const double2 col = rcs[c];
+        double2 row[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) row[i] = rcs[4 * g + i];
+        __syncthreads();
+        const double c2 = col.x, s2 = col.y;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const double c1 = row[i].x, s1 = row[i].y;
+          const double tt = a[i][0], tb = a[i][1], bt = a[i][2], bb = a[i][3];
+          const double btt = c2 * tt - s2 * tb, btb = s2 * tt + c2 * tb;
+          const double bbt = c2 * bt - s2 * bb, bbb = s2 * bt + c2 * bb;
+          a[i][0] = c1 * btt - s1 * bbt;
+          a[i][2] = s1 * btt + c1 * bbt;
+          a[i][1] = c1 * btb - s1 * bbb;
+          a[i][3] = s1 * btb + c1 * bbb;
+          if (rot && i == id) {
+            a[i][0] = app - tr * apq;
+            a[i][3] = aqq + tr * apq;
+            a[i][1] = 0.0;
+            a[i][2] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const double qt = qv[j][0], qb = qv[j][1];
+          qv[j][0] = c2 * qt - s2 * qb;
+          qv[j][1] = s2 * qt + c2 * qb;
+        }
+      }
+      // slot moves along the rows: in-lane, plus one pair across the groups
+      const double x0 = __shfl(a[3][0], (lane + 48) & 63, 64), x1 = __shfl(a[3][1], (lane + 48) & 63, 64);
+      const double y0 = __shfl(a[0][2], (lane + 16) & 63, 64), y1 = __shfl(a[0][3], (lane + 16) & 63, 64);
+      {
+        const double t0 = a[0][0], t1 = a[0][1], u0 = a[0][2], u1 = a[0][3];
+        const double v0 = a[3][0], v1 = a[3][1];
+        // tops: pair 4g+i takes the top of pair 4g+i-1 (pair 1 the bottom of 0, pair 0 stays)
+        a[3][0] = a[2][0]; a[3][1] = a[2][1];
+        a[2][0] = a[1][0]; a[2][1] = a[1][1];
+        a[1][0] = (g == 0) ? u0 : t0;
+        a[1][1] = (g == 0) ? u1 : t1;
+        a[0][0] = (g == 0) ? t0 : x0;
+        a[0][1] = (g == 0) ? t1 : x1;
+        // bottoms: pair 4g+i takes the bottom of pair 4g+i+1 (pair 15 its own top)
+        a[0][2] = a[1][2]; a[0][3] = a[1][3];
+        a[1][2] = a[2][2]; a[1][3] = a[2][3];
+        a[2][2] = a[3][2]; a[2][3] = a[3][3];
+        a[3][2] = (g == 3) ? v0 : y0;
+        a[3][3] = (g == 3) ? v1 : y1;
+      }
+      // slot moves along the columns
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        circle_cols(a[i][0], a[i][1], c);
+        circle_cols(a[i][2], a[i][3], c);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) circle_cols(qv[j][0], qv[j][1], c);
+    }
+  }
+  double* Qp = Qbuf + pr.q_off + (long long)it.y * 32 * 32;
+  if (total > 0)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int u = g + 4 * j;
+      Qp[u * 32 + c] = qv[j][0];
+      Qp[u * 32 + 16 + c] = qv[j][1];
+    }
+  if (threadIdx.x == 0) {
+    rotflag[(int)(pr.q_off / (32 * 32)) + it.y] = total > 0 ? 1 : 0;
+    if (work && total > 0) atomicAdd(work, (unsigned long long)(9 * 32 * 32) * (unsigned long long)total);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tile updates of the block-Jacobi rounds with 32 x 32 pivots, one wave per
+// tile, without LDS: operands go from global memory straight into the lane
+// layout of v_mfma_f64_16x16x4f64 (lane = 16 k4 + i16; A operand (row i16,
+// k k4), B operand (k k4, col i16), accumulator register i (row k4 + 4i,
+// col i16)) and results are stored from it.  A 16 x 16 accumulator block is
+// the B operand of a following product for its k-steps 4i..4i+3, so the
+// two-sided update chains two products in registers.  W is kept as rows
+// (row j = eigenvector j), which makes every load and store coalesced.
+
+// global index of position u (0..31) of pivot pair (bp, bq)
+__device__ __forceinline__ int pair_index(int bp, int bq, int u) { return u < 16 ? 16 * bp + u : 16 * bq + u - 16; }
+
+// A[I_k, I_l] <- Q_k^T A[I_k, I_l] Q_l for pair tiles k <= l (A symmetric)
+__global__ void __launch_bounds__(64) k_jac_apply_aR(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
+                                                     int r, double* A, const double* __restrict__ Qbuf,
+                                                     const int* __restrict__ rotflag, const int* __restrict__ state) {
+  const int4 it = list[blockIdx.x];
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return;
+  const int k = it.y, l = it.z;
+  const int qb = (int)(pr.q_off / (32 * 32));
+  const int rk = rotflag[qb + k], rl = rotflag[qb + l];
+  if (!rk && !rl) return;
+  const int n = pr.dim;
+  int kp, kq, lp, lq;
+  jac_pair(pr.nbp, pr.rounds, r, k, kp, kq);
+  jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
+  double* Ap = A + pr.a_off;
+  const double* Qk = Qbuf + pr.q_off + (long long)k * 32 * 32;
+  const double* Ql = Qbuf + pr.q_off + (long long)l * 32 * 32;
+  const int lane = threadIdx.x, i16 = lane & 15, k4 = lane >> 4;
+  v4d t[2][2];
+  if (rl) {
+    // T = A_kl Q_l; A_kl(u, w) is read as A(I_l[w], I_k[u]) (coalesced over u)
+    double av[8][2], bv[8][2];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int w = 4 * s + k4, gw = pair_index(lp, lq, w);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int gu = pair_index(kp, kq, 16 * h + i16);
+        av[s][h] = (gw < n && gu < n) ? Ap[(long long)gw * n + gu] : 0.0;
+        bv[s][h] = Ql[w * 32 + 16 * h + i16];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) t[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) t[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s][a], bv[s][b], t[a][b], 0, 0, 0);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int gu = pair_index(kp, kq, 16 * a + k4 + 4 * i), gv = pair_index(lp, lq, 16 * b + i16);
+          t[a][b][i] = (gu < n && gv < n) ? Ap[(long long)gu * n + gv] : 0.0;
+        }
+  }
+  v4d o[2][2];
+  if (rk) {
+    // out = Q_k^T T; the A operand Q_k^T(u, w) = Q_k(w, u), the B operand is T
+    double qv[8][2];
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) qv[s][h] = Qk[(4 * s + k4) * 32 + 16 * h + i16];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) o[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          o[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(qv[s][a], t[s >> 2][b][s & 3], o[a][b], 0, 0, 0);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) o[a][b] = t[a][b];
+  }
+  // store the tile and its mirror; a diagonal tile stores its upper triangle
+  // to both places, so A stays exactly symmetric
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int u = 16 * a + k4 + 4 * i, v = 16 * b + i16;
+        const int gu = pair_index(kp, kq, u), gv = pair_index(lp, lq, v);
+        if (gu >= n || gv >= n || (k == l && u > v)) continue;
+        Ap[(long long)gu * n + gv] = o[a][b][i];
+        Ap[(long long)gv * n + gu] = o[a][b][i];
+      }
+}
+
+// W rows I_l, columns of row block rb: Wr[I_l, ...] <- Q_l^T Wr[I_l, ...]
+__global__ void __launch_bounds__(64) k_jac_apply_wR(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
+                                                     int r, double* W, const double* __restrict__ Qbuf,
+                                                     const int* __restrict__ rotflag, const int* __restrict__ state) {
+  const int4 it = list[blockIdx.x];
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return;
+  const int l = it.z, rb = it.y;
+  const int qb = (int)(pr.q_off / (32 * 32));
+  if (!rotflag[qb + l]) return;
+  const int n = pr.dim;
+  int lp, lq;
+  jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
+  double* Wp = W + pr.w_off;
+  const double* Ql = Qbuf + pr.q_off + (long long)l * 32 * 32;
+  const int lane = threadIdx.x, i16 = lane & 15, k4 = lane >> 4;
+  double qv[8][2], wv[8][2];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int w = 4 * s + k4, gw = pair_index(lp, lq, w);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int x = 32 * rb + 16 * h + i16;
+      qv[s][h] = Ql[w * 32 + 16 * h + i16];
+      wv[s][h] = (gw < n && x < n) ? Wp[(long long)gw * n + x] : 0.0;
+    }
+  }
+  v4d o[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) o[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) o[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(qv[s][a], wv[s][b], o[a][b], 0, 0, 0);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int gv = pair_index(lp, lq, 16 * a + k4 + 4 * i), x = 32 * rb + 16 * b + i16;
+        if (gv < n && x < n) Wp[(long long)gv * n + x] = o[a][b][i];
+      }
+}
+
 // initialise W = I and the per-problem scale max|A_ii| and state; molecules
 // that finished their SCC are marked done and skipped
 __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__ probs, const double* __restrict__ A,

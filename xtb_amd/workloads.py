@@ -51,6 +51,11 @@ def water_cluster(nwater, seed, spacing=3.0, jitter=0.3, grid=None):
     return np.array(numbers, dtype=np.int32), np.array(pos) * AATOAU
 
 
+def large_water_cluster(seed=2024):
+    """700 waters on a 10 x 10 x 7 grid (2100 atoms, 4200 AOs)."""
+    return water_cluster(700, seed, grid=(10, 10, 7))
+
+
 def alkane(ncarbon, seed, jitter=0.02):
     """(numbers, positions[bohr]) of a seeded all-trans CnH(2n+2)."""
     rng = np.random.default_rng(seed)
