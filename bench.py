@@ -93,6 +93,26 @@ def serve(args):
             "workload": "Poisson arrivals, seeded water clusters / alkanes, 20-400 atoms log-uniform"}
 
 
+def single_instances(xtb_amd):
+    """One molecule per batch: the 32-water cluster (192 AOs), C64H130 (386
+    AOs) and the 700-water cluster (4200 AOs); the second of two runs is timed
+    (the first allocates and loads the kernels)."""
+    cases = [("water32", workloads.water_cluster(32, 0)), ("alkane_c64h130", workloads.alkane(64, 1)),
+             ("water700", workloads.large_water_cluster(2024))]
+    out = {}
+    for name, (z, x) in cases:
+        b = xtb_amd.Batch([(z, x, 0.0, 0)])
+        b.run()
+        t = time.perf_counter()
+        b.run()
+        dt = time.perf_counter() - t
+        r = b.result(0)
+        out[name] = {"atoms": int(len(z)), "aos": int(r["nao"]), "ms": 1e3 * dt, "single_points_per_s": 1.0 / dt,
+                     "iterations": int(r["iterations"]), "energy": r["energy"], "converged": bool(r["converged"])}
+        b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +127,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-qps", type=float, default=40.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=16.0)
+    ap.add_argument("--no-singles", action="store_true", help="skip the one-molecule timings (C0, C1, C4 sizes)")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="run one plain device copy of this size first (byte-counter calibration)")
     args = ap.parse_args()
@@ -189,10 +210,12 @@ def main():
         "cpu_baseline": cpu,
     }
     ql = os.environ.get("XTBAMD_EIG") == "ql"
+    lds = os.environ.get("XTBAMD_PIVOT_KERNEL") == "lds"
     names = {"gemm": "k_gemm (batched FP64 GEMM on the matrix cores)",
              "jacobi": ql and "k_tred_ql (tridiagonalisation + QL eigensolver, one workgroup per molecule)" or
-             "k_jac_pairsP (32x32 pivot problems of the block-Jacobi eigensolver, in LDS)"}
-    short = {"gemm": "k_gemm_mfma", "jacobi": ql and "k_tred_ql" or "k_jac_pairsP"}
+             lds and "k_jac_pairsP (32x32 pivot problems of the block-Jacobi eigensolver, in LDS)" or
+             "k_jac_pairsR (register-resident Jacobi sweep of each 32x32 pivot problem, one wave)"}
+    short = {"gemm": "k_gemm_mfma", "jacobi": ql and "k_tred_ql" or lds and "k_jac_pairsP" or "k_jac_pairsR"}
     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs of this
     # workload (copy-calibrated); only valid for the default workload and schedule
     table = {}
@@ -219,6 +242,8 @@ def main():
     b.close()
     if args.serve_requests > 0:
         out["serving"] = serve(args)
+    if not args.no_singles:
+        out["single_instances"] = single_instances(xtb_amd)
     print(json.dumps(out), flush=True)
 
 

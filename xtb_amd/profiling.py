@@ -10,6 +10,10 @@ MiB), so the table states bytes, not counter units.
     python -m xtb_amd.profiling FETCH_DIR WRITE_DIR COPY_MIB > table.md
 
 prints a markdown table; ``traffic_table`` reads such a table back.
+
+    python -m xtb_amd.profiling stats RUN_DIR
+
+prints the per-kernel table of a ``--kernel-trace --stats`` run.
 """
 import csv
 import glob
@@ -94,7 +98,28 @@ def traffic_table(path):
     return table
 
 
+def stats_table(run_dir, top=24):
+    """Markdown table of a ``rocprofv3 --kernel-trace --stats`` run (kernel_stats.csv)."""
+    paths = glob.glob(os.path.join(run_dir, "**", "*kernel_stats.csv"), recursive=True)
+    if not paths:
+        raise FileNotFoundError(f"no kernel_stats.csv under {run_dir}")
+    with open(paths[0]) as f:
+        rows = list(csv.DictReader(f))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["| kernel | calls | total ms | avg us | % of GPU time |", "|---|---:|---:|---:|---:|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:top]:
+        name = r["Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+        t = float(r["TotalDurationNs"])
+        out.append(f"| `{name}` | {r['Calls']} | {t / 1e6:.2f} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{100.0 * t / total:.1f} |")
+    out.append(f"| total | | {total / 1e6:.2f} | | 100.0 |")
+    return "\n".join(out)
+
+
 def main(argv):
+    if len(argv) == 2 and argv[0] == "stats":
+        print(stats_table(argv[1]))
+        return 0
     if len(argv) != 3:
         print(__doc__, file=sys.stderr)
         return 2
