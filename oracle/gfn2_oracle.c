@@ -25,6 +25,7 @@
 
 #include "gfn2_oracle.h"
 #include "../xtb_amd/csrc/gfn2_params.h"
+#include "../xtb_amd/csrc/solv_params.h"
 
 /* ---- constants: src/mctc/mctc_constants.f90:22-34, src/mctc/convert.f90:25-29 */
 static const double PI = 3.1415926535897932384626433832795029;
@@ -1747,9 +1748,409 @@ int om_integrals(int n, const int *at, const double *xyz, double accuracy, doubl
 /* ========================================================================
  * The single point: scf (src/scf_module.F90:74-920) + scc (scc_core.f90:255-640)
  * ====================================================================== */
+/* ---- implicit solvation: GBSA / ALPB -------------------------------------
+ * Solvent parameters: src/solv/model.f90 (normalizeSolventName,
+ * loadInternalParam, paramToModel); model set-up: src/solv/gbsa.f90:217-442;
+ * Born radii: src/solv/born.f90:35-378; surface: src/solv/sasa.f90:42-205;
+ * kernels: src/solv/kernel.f90; potential, energy and gradient:
+ * src/solv/gbsa.f90:445-777, 991-1079.  GFN2 couples through the Mulliken
+ * atomic charges (src/scf_module.F90:299-321). */
+static const double SOLV_W = 0.3 * AATOAU;
+#define SOLV_W3 (SOLV_W * (SOLV_W * SOLV_W))
+#define SOLV_AH0 0.5
+#define SOLV_AH1 (3.0 / (4.0 * SOLV_W))
+#define SOLV_AH3 (-1.0 / (4.0 * SOLV_W3))
+static const double KCALTOAU = 1.0 / 627.50947428;
+
+typedef struct {
+  int n, kernel, lhb;
+  double keps, alpbet, gshift, gsasa, adet;
+  double *vdwr, *rho, *svdw, *vdwsa, *wrp, *trj2, *gamsasa, *hbmag;
+  double *brad, *brdr, *sasa, *dsdrt, *dsdr, *hbw, *dhbdw, *bmat;
+} solv_t;
+
+static void solv_free(solv_t *s) {
+  free(s->vdwr); free(s->rho); free(s->svdw); free(s->vdwsa); free(s->wrp); free(s->trj2); free(s->gamsasa);
+  free(s->hbmag); free(s->brad); free(s->brdr); free(s->sasa); free(s->dsdrt); free(s->dsdr); free(s->hbw);
+  free(s->dhbdw); free(s->bmat);
+  memset(s, 0, sizeof(*s));
+}
+
+/* normalizeSolventName (src/solv/model.f90) */
+static void solv_normalize(const char *in, char *out, size_t cap) {
+  size_t k = 0;
+  for (; in[k] && k + 1 < cap; ++k) out[k] = (char)((in[k] >= 'A' && in[k] <= 'Z') ? in[k] - 'A' + 'a' : in[k]);
+  out[k] = 0;
+  static const char *alias[][2] = {
+      {"methylcyanide", "acetonitrile"}, {"methylenechloride", "ch2cl2"}, {"dichloromethane", "ch2cl2"},
+      {"dcm", "ch2cl2"}, {"chloroform", "chcl3"}, {"trichloromethane", "chcl3"}, {"tcm", "chcl3"},
+      {"carbondisulfide", "cs2"}, {"diethylether", "ether"}, {"dimethylformamide", "dmf"},
+      {"dimethylsulfoxide", "dmso"}, {"furan", "furane"}, {"nhexan", "hexane"}, {"n-hexan", "hexane"},
+      {"nhexane", "hexane"}, {"n-hexane", "hexane"}, {"tetrahydrofuran", "thf"}, {"h2o", "water"}};
+  for (size_t a = 0; a < sizeof(alias) / sizeof(alias[0]); ++a)
+    if (!strcmp(out, alias[a][0])) { snprintf(out, cap, "%s", alias[a][1]); break; }
+}
+
+/* loadInternalParam: Still kernel -> GFN2/GBSA sets, P16 kernel -> GFN2/ALPB sets */
+static const SolvParamSet *solv_lookup(const char *solvent, int kernel) {
+  char nm[64];
+  solv_normalize(solvent, nm, sizeof(nm));
+  if (!strcmp(nm, "dichlormethane")) snprintf(nm, sizeof(nm), "ch2cl2");
+  const SolvParamSet *tab = kernel == 1 ? kSolvGbsaGfn2 : kSolvAlpbGfn2;
+  const int cnt = kernel == 1 ? kSolvGbsaGfn2Count : kSolvAlpbGfn2Count;
+  if (kernel == 1) {
+    if (!strcmp(nm, "water")) snprintf(nm, sizeof(nm), "h2o");
+    if (!strcmp(nm, "hexane")) snprintf(nm, sizeof(nm), "nhexan");
+  }
+  for (int i = 0; i < cnt; ++i)
+    if (!strcmp(tab[i].name, nm)) return &tab[i];
+  return NULL;
+}
+
+static double det3(const double m[3][3]) {
+  return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+         m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+}
+
+/* shape descriptor: getADet / addADetDeriv (src/solv/gbsa.f90:661-777) */
+static double solv_adet(int n, const double *xyz, const double *rad, double inertia[3][3], double center[3],
+                        double *totrad3) {
+  const double tof = 2.0 / 5.0;
+  double tot = 0.0;
+  center[0] = center[1] = center[2] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double r2 = rad[i] * rad[i], r3 = r2 * rad[i];
+    tot += r3;
+    for (int k = 0; k < 3; ++k) center[k] += xyz[3 * i + k] * r3;
+  }
+  for (int k = 0; k < 3; ++k) center[k] /= tot;
+  memset(inertia, 0, sizeof(double) * 9);
+  for (int i = 0; i < n; ++i) {
+    double r2 = rad[i] * rad[i], r3 = r2 * rad[i], v[3], d2 = 0.0;
+    for (int k = 0; k < 3; ++k) { v[k] = xyz[3 * i + k] - center[k]; d2 += v[k] * v[k]; }
+    for (int k = 0; k < 3; ++k) inertia[k][k] += r3 * (d2 + tof * r2);
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < 3; ++k) inertia[k][c] -= r3 * v[c] * v[k];
+  }
+  *totrad3 = tot;
+  return sqrt(pow(det3((const double(*)[3])inertia), 1.0 / 3.0) / (tof * tot));
+}
+
+/* set-up and geometry update: paramToModel, initBorn, update (Born radii,
+ * surface, H-bond term, shape descriptor, interaction matrix) */
+static int solv_setup(solv_t *s, const char *solvent, int kernel, int alpb, int n, const int *at, const double *xyz) {
+  const SolvParamSet *p = solv_lookup(solvent, kernel);
+  if (!p) return 1;
+  memset(s, 0, sizeof(*s));
+  s->n = n;
+  s->kernel = kernel;
+  const double eps = p->epsv;
+  s->gshift = p->gshift * KCALTOAU;   /* + state shift: 0 for the default gsolv state */
+  const double c1 = p->c1, boff = p->soset * 0.1 * AATOAU, probe = p->rprobe * AATOAU;
+  int anyhb = 0;
+  for (int k = 0; k < 94; ++k) if (fabs(p->hbond[k]) > 1.e-3) anyhb = 1;
+  s->alpbet = alpb ? 0.571412 / eps : 0.0;
+  s->keps = (1.0 / eps - 1.0) / (1.0 + s->alpbet);
+  s->vdwr = NEW(double, n); s->rho = NEW(double, n); s->svdw = NEW(double, n); s->vdwsa = NEW(double, n);
+  s->wrp = NEW(double, n); s->trj2 = NEW(double, 2 * n); s->gamsasa = NEW(double, n); s->hbmag = NEW(double, n);
+  double maxsa = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const int z = at[i];
+    if (z < 1 || z > 94) return 2;
+    s->vdwr[i] = AATOAU * kVdwRadD3Aa[z - 1];
+    s->rho[i] = s->vdwr[i] * p->sx[z - 1];
+    s->svdw[i] = s->vdwr[i] - boff;
+    const double sa = s->vdwr[i] + probe;
+    s->vdwsa[i] = sa;
+    s->trj2[2 * i] = (sa - SOLV_W) * (sa - SOLV_W);
+    s->trj2[2 * i + 1] = (sa + SOLV_W) * (sa + SOLV_W);
+    double r = sa + SOLV_W;
+    double w = (0.25 / SOLV_W + 3.0 * SOLV_AH3 * (0.2 * r * r - 0.5 * r * sa + sa * sa / 3.0)) * r * r * r;
+    r = sa - SOLV_W;
+    w = w - (0.25 / SOLV_W + 3.0 * SOLV_AH3 * (0.2 * r * r - 0.5 * r * sa + sa * sa / 3.0)) * r * r * r;
+    s->wrp[i] = w;
+    s->gamsasa[i] = p->gamscale[z - 1] * (4.0 * PI) * 1.0e-5;
+    s->hbmag[i] = anyhb ? -(p->hbond[z - 1] * p->hbond[z - 1]) * KCALTOAU : 0.0;
+    if (s->hbmag[i] < 0.0) s->lhb = 1;
+    if (sa > maxsa) maxsa = sa;
+  }
+  const double lrcut = 35.0 * AATOAU, srcut = 2.0 * (SOLV_W + maxsa) + 2.0 * AATOAU;
+  const double lr2 = lrcut * lrcut, sr2 = srcut * srcut;
+  const size_t nn3 = (size_t)3 * n * n;
+
+  /* Born radii: compute_psi + compute_bornr (OBC alpha 1, beta 0.8, gamma 4.85);
+   * psi derivatives dpsi[(i*n + j)*3] = d psi_i / d R_j */
+  double *psi = NEW(double, n), *dpsi = NEW(double, nn3), *dtr = NEW(double, 3 * n);
+  for (int ii = 0; ii < n; ++ii)
+    for (int jj = 0; jj < ii; ++jj) {
+      double dr[3], d2 = 0.0;
+      for (int k = 0; k < 3; ++k) { dr[k] = xyz[3 * ii + k] - xyz[3 * jj + k]; d2 += dr[k] * dr[k]; }
+      if (!(d2 < lr2)) continue;
+      const double r = sqrt(d2), r1 = 1.0 / r;
+      const double rhoi = s->rho[ii], rhoj = s->rho[jj], rvdwi = s->vdwr[ii], rvdwj = s->vdwr[jj];
+      const int ovij = r >= (rvdwi + rhoj) ? 0 : 1, ovji = r >= (rhoi + rvdwj) ? 0 : 1;
+      /* contribution of sphere (radius rho_o) around the other atom to the atom with radius rvdw */
+      double gi = 0.0, dgi = 0.0, gj = 0.0, dgj = 0.0;
+      int has_i = 0, has_j = 0;
+      /* non-overlapping form */
+#define PSI_FAR(rh, g, dg)                                   \
+      do {                                                   \
+        const double ap = r + (rh), am = r - (rh), ab = ap * am; \
+        const double rhab = (rh) / ab, lnab = 0.5 * log(am / ap) * r1; \
+        g = rhab + lnab;                                     \
+        dg = -2.0 * rhab / ab + (rhab - lnab) * r1 * r1;     \
+      } while (0)
+      /* overlapping form */
+#define PSI_NEAR(rh, rv, g, dg)                                                                      \
+      do {                                                                                           \
+        const double r12 = 0.5 * r1, r24 = r12 * r12, ap = r + (rh), am = r - (rh);                  \
+        const double rh1 = 1.0 / (rv), rhr1 = 1.0 / ap, aprh1 = ap * rh1, lnab = log(aprh1);         \
+        g = rh1 - rhr1 + r12 * (0.5 * am * (rhr1 - rh1 * aprh1) - lnab);                             \
+        dg = rhr1 * rhr1 * (1.0 - 0.25 * am * r1 * (1.0 + aprh1 * aprh1)) + (rh) * r24 * (rhr1 - rh1 * aprh1) + \
+             r12 * (r1 * lnab - rhr1);                                                               \
+        dg = dg * r1;                                                                                \
+      } while (0)
+      if (!ovij) { PSI_FAR(rhoj, gi, dgi); has_i = 1; }
+      else if ((r + rhoj) > rvdwi) { PSI_NEAR(rhoj, rvdwi, gi, dgi); has_i = 1; }
+      if (!ovij && !ovji && fabs(rhoi - rhoj) < 1.e-8) { gj = gi; dgj = dgi; has_j = 1; }
+      else if (!ovji) { PSI_FAR(rhoi, gj, dgj); has_j = 1; }
+      else if ((r + rhoi) > rvdwj) { PSI_NEAR(rhoi, rvdwj, gj, dgj); has_j = 1; }
+#undef PSI_FAR
+#undef PSI_NEAR
+      if (has_i) {
+        psi[ii] += gi;
+        for (int k = 0; k < 3; ++k) {
+          const double d = dgi * dr[k];
+          dtr[3 * ii + k] += d;
+          dpsi[((size_t)ii * n + jj) * 3 + k] -= d;
+        }
+      }
+      if (has_j) {
+        psi[jj] += gj;
+        for (int k = 0; k < 3; ++k) {
+          const double d = dgj * dr[k];
+          dtr[3 * jj + k] -= d;
+          dpsi[((size_t)jj * n + ii) * 3 + k] += d;
+        }
+      }
+    }
+  for (int i = 0; i < n; ++i) for (int k = 0; k < 3; ++k) dpsi[((size_t)i * n + i) * 3 + k] = dtr[3 * i + k];
+  s->brad = NEW(double, n);
+  s->brdr = dpsi;
+  for (int i = 0; i < n; ++i) {
+    const double alp = 1.0, bet = 0.8, gam = 4.85;
+    double br = psi[i];
+    const double svdwi = s->svdw[i], vdwri = s->vdwr[i], s1 = 1.0 / svdwi, v1 = 1.0 / vdwri, s2 = 0.5 * svdwi;
+    br = br * s2;
+    double arg2 = br * (gam * br - bet);
+    const double arg = br * (alp + arg2);
+    arg2 = 2.0 * arg2 + alp + gam * br * br;
+    const double th = tanh(arg), ch = cosh(arg);
+    br = 1.0 / (s1 - v1 * th);
+    br = c1 * br;
+    double dp = ch * (s1 - v1 * th);
+    dp = s2 * v1 * arg2 / (dp * dp);
+    dp = c1 * dp;
+    s->brad[i] = br;
+    for (int j = 0; j < n; ++j) for (int k = 0; k < 3; ++k) s->brdr[((size_t)i * n + j) * 3 + k] *= dp;
+  }
+  free(psi); free(dtr);
+
+  /* solvent accessible surface: compute_numsa / compute_w_sp; neighbours in
+   * ascending index order within srcut; dsdrt[(i*n + j)*3] = d sasa_i / d R_j */
+  s->sasa = NEW(double, n);
+  s->dsdrt = NEW(double, nn3);
+  {
+    int *nb = NEW(int, n), *gi = NEW(int, n);
+    double *gd = NEW(double, 3 * n);
+    for (int i = 0; i < n; ++i) {
+      int nno = 0;
+      for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        double d2 = 0.0;
+        for (int k = 0; k < 3; ++k) { double d = xyz[3 * i + k] - xyz[3 * j + k]; d2 += d * d; }
+        if (d2 < lr2 && d2 < sr2) nb[nno++] = j;
+      }
+      const double rsas = s->vdwsa[i], wr = s->wrp[i];
+      double sasai = 0.0;
+      double *grads = s->dsdrt + (size_t)i * n * 3;
+      for (int ip = 0; ip < 230; ++ip) {
+        double xp[3];
+        for (int k = 0; k < 3; ++k) xp[k] = xyz[3 * i + k] + rsas * kLebedev230[ip][k];
+        double sp = 1.0;
+        int nni = 0;
+        for (int m = 0; m < nno; ++m) {
+          const int a = nb[m];
+          double tj[3], tj2;
+          for (int k = 0; k < 3; ++k) tj[k] = xp[k] - xyz[3 * a + k];
+          tj2 = tj[0] * tj[0] + tj[1] * tj[1] + tj[2] * tj[2];
+          if (tj2 < s->trj2[2 * a + 1]) {
+            if (tj2 <= s->trj2[2 * a]) { sp = 0.0; break; }
+            const double sqtj = sqrt(tj2), uj = sqtj - s->vdwsa[a], ah3uj2 = SOLV_AH3 * uj * uj;
+            double ds = SOLV_AH1 + 3.0 * ah3uj2;
+            double sij = SOLV_AH0 + (SOLV_AH1 + ah3uj2) * uj;
+            sij = fmin(sij, 1.0);
+            sij = fmax(0.0, sij);
+            if (sij < 2.2250738585072014e-308) { sp = 0.0; break; }
+            sp = sp * sij;
+            ds = ds / (sij * sqtj);
+            gi[nni] = a;
+            for (int k = 0; k < 3; ++k) gd[3 * nni + k] = ds * tj[k];
+            ++nni;
+          }
+        }
+        if (sp > 1.e-6) {
+          const double wsa = kLebedev230[ip][3] * wr * sp;
+          sasai += wsa;
+          for (int m = 0; m < nni; ++m)
+            for (int k = 0; k < 3; ++k) {
+              const double d = wsa * gd[3 * m + k];
+              grads[3 * i + k] += d;
+              grads[3 * gi[m] + k] -= d;
+            }
+        }
+      }
+      s->sasa[i] = sasai;
+    }
+    free(nb); free(gi); free(gd);
+  }
+  s->dsdr = NEW(double, 3 * n);
+  s->gsasa = 0.0;
+  for (int i = 0; i < n; ++i) {
+    for (size_t x = 0; x < (size_t)3 * n; ++x) s->dsdr[x] += s->dsdrt[(size_t)i * n * 3 + x] * s->gamsasa[i];
+    s->gsasa += s->sasa[i] * s->gamsasa[i];
+  }
+  /* hydrogen bond term: compute_fhb */
+  s->hbw = NEW(double, n);
+  s->dhbdw = NEW(double, n);
+  if (s->lhb)
+    for (int i = 0; i < n; ++i) {
+      const double smaxd = 1.0 / (s->vdwsa[i] * s->vdwsa[i]);
+      s->hbw[i] = s->hbmag[i] * (s->sasa[i] * smaxd);
+      s->dhbdw[i] = s->hbmag[i] * smaxd;
+    }
+  if (s->alpbet > 0.0) {
+    double inertia[3][3], center[3], tot;
+    s->adet = solv_adet(n, xyz, s->vdwr, inertia, center, &tot);
+  }
+  /* interaction matrix: addBornMatStill / addBornMatP16 + H-bond + ALPB */
+  s->bmat = NEW(double, (size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) {
+      double d2 = 0.0;
+      for (int k = 0; k < 3; ++k) { double d = xyz[3 * i + k] - xyz[3 * j + k]; d2 += d * d; }
+      const double r1 = sqrt(d2);
+      double f;
+      if (kernel == 1) {
+        const double r2 = r1 * r1, aa = s->brad[i] * s->brad[j], dd = 0.25 * r2 / aa, expd = exp(-dd);
+        f = 1.0 / sqrt(r2 + aa * expd);
+      } else {
+        const double ab = sqrt(s->brad[i] * s->brad[j]);
+        double arg = ab / (ab + (1.028 / 16.0) * r1);
+        arg = arg * arg; arg = arg * arg; arg = arg * arg; arg = arg * arg;
+        f = 1.0 / (r1 + ab * arg);
+      }
+      s->bmat[(size_t)i * n + j] = s->keps * f + s->bmat[(size_t)i * n + j];
+      s->bmat[(size_t)j * n + i] = s->keps * f + s->bmat[(size_t)j * n + i];
+    }
+  for (int i = 0; i < n; ++i) s->bmat[(size_t)i * n + i] += s->keps * (1.0 / s->brad[i]);
+  if (s->lhb) for (int i = 0; i < n; ++i) s->bmat[(size_t)i * n + i] += 2 * s->hbw[i];
+  if (s->alpbet > 0.0) {
+    const double c = s->keps * s->alpbet / s->adet;
+    for (size_t x = 0; x < (size_t)n * n; ++x) s->bmat[x] += c;
+  }
+  return 0;
+}
+
+/* solvation energy at charges q: getEnergy */
+static double solv_energy(const solv_t *s, const double *q, double *work) {
+  symv(s->n, s->bmat, q, work, 0.5);
+  double e = 0.0;
+  for (int i = 0; i < s->n; ++i) e += q[i] * work[i];
+  return e + s->gsasa + s->gshift;
+}
+
+/* gradient: addGradientStill / addGradientP16, surface, H-bond, ALPB shape */
+static void solv_gradient(const solv_t *s, const double *xyz, const double *q, double *grad) {
+  const int n = s->n;
+  const double keps = s->keps;
+  double *grddb = NEW(double, n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) {
+      double v[3], d2 = 0.0;
+      for (int k = 0; k < 3; ++k) { v[k] = xyz[3 * i + k] - xyz[3 * j + k]; d2 += v[k] * v[k]; }
+      const double r = sqrt(d2), qq = q[i] * q[j];
+      if (s->kernel == 1) {
+        const double r2 = r * r, aa = s->brad[i] * s->brad[j], dd = 0.25 * r2 / aa, expd = exp(-dd);
+        const double fgb2 = r2 + aa * expd, dfgb2 = 1.0 / fgb2, dfgb = sqrt(dfgb2), dfgb3 = dfgb2 * dfgb * keps;
+        const double ap = (1.0 - 0.25 * expd) * dfgb3;
+        for (int k = 0; k < 3; ++k) {
+          const double dr = ap * v[k];
+          grad[3 * i + k] = grad[3 * i + k] - dr * qq;
+          grad[3 * j + k] = grad[3 * j + k] + dr * qq;
+        }
+        const double bp = -0.5 * expd * (1.0 + dd) * dfgb3;
+        grddb[i] = grddb[i] + (s->brad[j] * bp) * qq;
+        grddb[j] = grddb[j] + (s->brad[i] * bp) * qq;
+      } else {
+        const double z = 1.028, ab = sqrt(s->brad[i] * s->brad[j]);
+        const double arg1 = ab / (ab + (z / 16.0) * r);
+        double a16 = arg1 * arg1;
+        a16 = a16 * a16; a16 = a16 * a16; a16 = a16 * a16;
+        const double fgb = r + ab * a16, dfgb = 1.0 / fgb, dfgb2 = dfgb * dfgb;
+        const double ap = (1.0 - z * arg1 * a16) * dfgb2;
+        for (int k = 0; k < 3; ++k) {
+          const double dG = ap * v[k] * keps / r * qq;
+          grad[3 * i + k] = grad[3 * i + k] - dG;
+          grad[3 * j + k] = grad[3 * j + k] + dG;
+        }
+        const double bp = -0.5 * (r * z / ab * arg1 + 1.0) / ab * a16 * dfgb2;
+        grddb[i] = grddb[i] + s->brad[j] * bp * keps * qq;
+        grddb[j] = grddb[j] + s->brad[i] * bp * keps * qq;
+      }
+    }
+  for (int i = 0; i < n; ++i) {
+    const double bp = 1.0 / s->brad[i], qq = q[i] * bp;
+    grddb[i] = grddb[i] + (-0.5 * keps * qq * bp) * q[i];
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < 3; ++k) grad[3 * j + k] += s->brdr[((size_t)i * n + j) * 3 + k] * grddb[i];
+  free(grddb);
+  for (int x = 0; x < 3 * n; ++x) grad[x] += s->dsdr[x];
+  if (s->lhb)
+    for (int i = 0; i < n; ++i) {
+      double dh = s->dhbdw[i];
+      if (fabs(dh) <= 0.0) continue;
+      dh = dh * q[i] * q[i];
+      for (int j = 0; j < n; ++j)
+        for (int k = 0; k < 3; ++k) grad[3 * j + k] += s->dsdrt[((size_t)i * n + j) * 3 + k] * dh;
+    }
+  if (s->alpbet > 0.0) {
+    double I[3][3], center[3], tot, qt = 0.0;
+    for (int i = 0; i < n; ++i) qt += q[i];
+    const double adet = solv_adet(n, xyz, s->vdwr, I, center, &tot);
+    const double ke = s->keps * s->alpbet;
+    const double f = (250.0 / (48.0 * tot * tot * tot * pow(adet, 5))) * (-0.5 * ke * qt * qt / (adet * adet));
+    double A[3][3];
+    A[0][0] = I[0][0] * (I[1][1] + I[2][2]) - I[0][1] * I[0][1] - I[0][2] * I[0][2];
+    A[1][0] = A[0][1] = I[0][1] * I[2][2] - I[0][2] * I[1][2];
+    A[2][0] = A[0][2] = I[0][2] * I[1][1] - I[0][1] * I[2][1];
+    A[1][1] = I[1][1] * (I[0][0] + I[2][2]) - I[0][1] * I[0][1] - I[1][2] * I[1][2];
+    A[2][1] = A[1][2] = I[0][0] * I[1][2] - I[0][1] * I[0][2];
+    A[2][2] = I[2][2] * (I[0][0] + I[1][1]) - I[0][2] * I[0][2] - I[1][2] * I[1][2];
+    for (int i = 0; i < n; ++i) {
+      const double r3 = s->vdwr[i] * s->vdwr[i] * s->vdwr[i];
+      double v[3];
+      for (int k = 0; k < 3; ++k) v[k] = xyz[3 * i + k] - center[k];
+      for (int k = 0; k < 3; ++k) grad[3 * i + k] += r3 * ((A[k][0] * f) * v[0] + (A[k][1] * f) * v[1] + (A[k][2] * f) * v[2]);
+    }
+  }
+}
+
 int om_singlepoint(const om_options *opt_in, int n, const int *at, const double *xyz, double charge, int uhf, om_result *res) {
   init_elements();
-  om_options opt = {1.0, 300.0, 250, 0, 0};
+  om_options opt = {1.0, 300.0, 250, 0, 0, NULL, 1, 0, NULL, NULL, NULL};
   if (opt_in) opt = *opt_in;
   if (opt.lapack) lapack_load();
   const int use_lib = opt.lapack && p_dsygvd && p_dgemm;
@@ -1811,6 +2212,13 @@ int om_singlepoint(const om_options *opt_in, int n, const int *at, const double 
     }
   }
   double *dipm = NEW(double, 3 * n), *qp = NEW(double, 6 * n);
+  if (opt.init_qsh) { /* restart from a previous wavefunction (src/api/interface.f90:83-101) */
+    memcpy(qsh, opt.init_qsh, sizeof(double) * NS);
+    for (int i = 0; i < n; ++i) q[i] = 0.0;
+    for (int s = 0; s < NS; ++s) q[b.sh_at[s]] += qsh[s];
+    if (opt.init_dipm) memcpy(dipm, opt.init_dipm, sizeof(double) * 3 * n);
+    if (opt.init_qp) memcpy(qp, opt.init_qp, sizeof(double) * 6 * n);
+  }
 
   /* classical terms */
   double *grad = NEW(double, 3 * n);
@@ -1824,6 +2232,20 @@ int om_singlepoint(const om_options *opt_in, int n, const int *at, const double 
   coulomb_matrix(n, at, xyz, &b, jmat);
   double *shgam = NEW(double, NS);
   for (int i = 0; i < n; ++i) for (int m = 0; m < EL[at[i]].nsh; ++m) shgam[b.at_sh0[i] + m] = EL[at[i]].third[m];
+  /* implicit solvation (src/scf_module.F90:299-321) */
+  solv_t sv;
+  memset(&sv, 0, sizeof(sv));
+  double *svwork = NEW(double, n);
+  if (opt.solvent && opt.solvent[0]) {
+    if (solv_setup(&sv, opt.solvent, opt.solv_kernel == 2 ? 2 : 1, opt.solv_alpb, n, at, xyz)) {
+      solv_free(&sv);
+      free(svwork);
+      free(shgam); free(jmat); d4_free(&d4); free(cn); free(dcndr); free(grad);
+      free(dipm); free(qp); free(q); free(qsh); free(focca); free(foccb); free(focc); free(zsh);
+      basis_free(&b);
+      return 6;
+    }
+  }
 
   /* integrals */
   double *selfe = NEW(double, NS), *dsedcn = NEW(double, NS);
@@ -1878,6 +2300,10 @@ int om_singlepoint(const om_options *opt_in, int n, const int *at, const double 
       symv(NS, jmat, qsh, jq, 1.0);
       for (int s = 0; s < NS; ++s) shift[s] += jq[s];
       aes_potential(n, at, xyz, q, dipm, qp, gab3, gab5, 1, vs, vd, vq);
+      if (sv.n) { /* solvation%addShift (src/solv/gbsa.f90:495-521) */
+        symv(n, sv.bmat, q, svwork, 1.0);
+        for (int i = 0; i < n; ++i) ashift[i] = ashift[i] + svwork[i];
+      }
       d4_potential(&d4, n, at, q, ashift);
       for (int s = 0; s < NS; ++s) shift[s] += ashift[b.sh_at[s]];
       /* Fock matrix: buildIsoAnisotropicH1 (src/scc_core.f90:154-251), eV */
@@ -1987,6 +2413,7 @@ int om_singlepoint(const om_options *opt_in, int n, const int *at, const double 
       eel += eaes + epol;
       ed = d4_energy(&d4, n, at, q);
       eel += ed;
+      if (sv.n) eel = eel + solv_energy(&sv, q, svwork);   /* src/scc_core.f90:544-548 */
       eel += ga + gb;
       int econv = fabs(eel - eold) < scfconv;
       for (int s = 0; s < NS; ++s) dq[s] = qsh[s] - q_in[s];
@@ -2131,6 +2558,8 @@ int om_singlepoint(const om_options *opt_in, int n, const int *at, const double 
     /* D4 gradient + ATM energy */
     double embd = 0.0;
     d4_gradient(&d4, n, at, xyz, q, grad, &embd);
+    /* solvation gradient (src/scf_module.F90:761-777) */
+    if (sv.n) solv_gradient(&sv, xyz, q, grad);
     /* Coulomb derivatives: getCoulombDerivsCluster + gemv(djdr, qsh) */
     for (int i = 0; i < n; ++i)
       for (int j = 0; j < i; ++j) {
@@ -2157,6 +2586,7 @@ int om_singlepoint(const om_options *opt_in, int n, const int *at, const double 
       res->iterations = jter; res->converged = converged; res->ihomo = ihomo;
       res->e_total = energy; res->e_elec = eel + embd; res->e_es = ees; res->e_aes = eaes; res->e_axc = epol;
       res->e_disp = ed + embd; res->e_rep = ep; res->e_atm = embd; res->hl_gap_ev = egap;
+      res->e_solv = sv.n ? solv_energy(&sv, q, svwork) : 0.0;
       double gn = 0.0;
       for (int k = 0; k < 3 * n; ++k) gn += grad[k] * grad[k];
       res->gnorm = sqrt(gn);
@@ -2204,6 +2634,7 @@ cleanup_early:
   free(U); free(radcn); free(gab3); free(gab5); free(dmask); free(qmask);
   free(S); free(D); free(Q); free(H0); free(H0n); free(selfe); free(dsedcn);
   free(jmat); free(shgam); d4_free(&d4); free(cn); free(dcndr); free(grad);
+  solv_free(&sv); free(svwork);
   free(dipm); free(qp); free(q); free(qsh); free(focca); free(foccb); free(focc); free(zsh);
   basis_free(&b);
   return ret;

@@ -23,6 +23,12 @@ typedef struct {
   int    maxiter;    /* max SCC iterations, default 250 */
   int    guess;      /* 0 = EEQ charges (default API path), 1 = SAD (q = charge/n) */
   int    lapack;     /* 1: use host LAPACK (dlopen) for the eigensolve if present */
+  const char *solvent; /* implicit solvent name, NULL: gas phase */
+  int    solv_kernel;  /* Born kernel: 1 Still (GFN2/GBSA parameters), 2 P16 (GFN2/ALPB parameters) */
+  int    solv_alpb;    /* 1: ALPB shape correction (alpha = 0.571412) */
+  /* restart: shell charges [nsh], atomic dipoles [n][3] and quadrupoles [n][6]
+   * of a previous result seed the SCC instead of the guess (NULL: guess) */
+  const double *init_qsh, *init_dipm, *init_qp;
 } om_options;
 
 typedef struct {
@@ -31,6 +37,7 @@ typedef struct {
   /* energies (Eh) */
   double e_total, e_elec, e_es, e_aes, e_axc, e_disp, e_rep, e_atm, hl_gap_ev, gnorm;
   double dipole[3];
+  double e_solv;     /* solvation free energy (Eh), 0 in the gas phase */
   /* caller-allocated outputs, may be NULL */
   double *gradient;  /* [n][3] */
   double *q;         /* [n] */

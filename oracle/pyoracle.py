@@ -15,7 +15,10 @@ _LIB = None
 
 class OmOptions(ctypes.Structure):
     _fields_ = [("accuracy", ctypes.c_double), ("etemp", ctypes.c_double),
-                ("maxiter", ctypes.c_int), ("guess", ctypes.c_int), ("lapack", ctypes.c_int)]
+                ("maxiter", ctypes.c_int), ("guess", ctypes.c_int), ("lapack", ctypes.c_int),
+                ("solvent", ctypes.c_char_p), ("solv_kernel", ctypes.c_int), ("solv_alpb", ctypes.c_int),
+                ("init_qsh", ctypes.POINTER(ctypes.c_double)), ("init_dipm", ctypes.POINTER(ctypes.c_double)),
+                ("init_qp", ctypes.POINTER(ctypes.c_double))]
 
 
 _D = ctypes.POINTER(ctypes.c_double)
@@ -30,7 +33,7 @@ class OmResult(ctypes.Structure):
                 ("e_axc", ctypes.c_double), ("e_disp", ctypes.c_double),
                 ("e_rep", ctypes.c_double), ("e_atm", ctypes.c_double),
                 ("hl_gap_ev", ctypes.c_double), ("gnorm", ctypes.c_double),
-                ("dipole", ctypes.c_double * 3),
+                ("dipole", ctypes.c_double * 3), ("e_solv", ctypes.c_double),
                 ("gradient", _D), ("q", _D), ("qsh", _D), ("dipm", _D), ("qp", _D),
                 ("wbo", _D), ("emo", _D), ("focc", _D), ("density", _D), ("coeff", _D)]
 
@@ -64,8 +67,14 @@ def dims(numbers):
 
 
 def singlepoint(numbers, positions, charge=0.0, uhf=0, accuracy=1.0, etemp=300.0,
-                maxiter=250, guess="eeq", lapack=False, want_matrices=False, threads=1):
-    """Run the oracle; returns a dict of results (Hartree, bohr, eV for emo)."""
+                maxiter=250, guess="eeq", lapack=False, want_matrices=False, threads=1,
+                solvent=None, solv_kernel="still", alpb=False, restart=None):
+    """Run the oracle; returns a dict of results (Hartree, bohr, eV for emo).
+
+    solvent: implicit solvent name (None: gas phase); solv_kernel "still"
+    (GFN2/GBSA parameter sets) or "p16" (GFN2/ALPB parameter sets); alpb adds
+    the ALPB shape correction.  restart: a previous result dict whose qsh,
+    dipm and qp seed the SCC (the reference's reused results object)."""
     at = np.ascontiguousarray(numbers, dtype=np.int32)
     xyz = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
     n = len(at)
@@ -82,7 +91,13 @@ def singlepoint(numbers, positions, charge=0.0, uhf=0, accuracy=1.0, etemp=300.0
     res = OmResult()
     for k, v in out.items():
         setattr(res, k, _p(v))
-    opt = OmOptions(accuracy, etemp, maxiter, 0 if guess == "eeq" else 1, 1 if lapack else 0)
+    opt = OmOptions(accuracy, etemp, maxiter, 0 if guess == "eeq" else 1, 1 if lapack else 0,
+                    solvent.encode() if solvent else None, 2 if solv_kernel == "p16" else 1, 1 if alpb else 0)
+    if restart is not None:
+        keep = [np.ascontiguousarray(restart[k], dtype=np.float64).ravel() for k in ("qsh", "dipm", "qp")]
+        if keep[0].size != nsh or keep[1].size != 3 * n or keep[2].size != 6 * n:
+            raise ValueError("restart data of another molecule")
+        opt.init_qsh, opt.init_dipm, opt.init_qp = (_p(a) for a in keep)
     rc = lib().om_singlepoint(ctypes.byref(opt), n, at.ctypes.data_as(ctypes.c_void_p),
                               _p(xyz), ctypes.c_double(charge), ctypes.c_int(uhf),
                               ctypes.byref(res))
@@ -90,7 +105,7 @@ def singlepoint(numbers, positions, charge=0.0, uhf=0, accuracy=1.0, etemp=300.0
         raise RuntimeError(f"oracle single point failed with code {rc}")
     for k in ("n", "nsh", "nao", "ndim_d4", "iterations", "converged", "ihomo", "e_total",
               "e_elec", "e_es", "e_aes", "e_axc", "e_disp", "e_rep", "e_atm", "hl_gap_ev",
-              "gnorm"):
+              "gnorm", "e_solv"):
         out[k] = getattr(res, k)
     out["dipole"] = np.array(res.dipole[:])
     return out

@@ -28,7 +28,7 @@ SYMBOLS = ("H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe 
            "Zn Ga Ge As Se Br Kr Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba "
            "La Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb Lu Hf Ta W Re Os Ir Pt Au Hg Tl Pb Bi "
            "Po At Rn").split()
-FLT = r"[-+]?\d+\.\d*(?:[eEdD][-+]?\d+)?"
+FLT = r"[-+]?(?:\d+\.\d*|\.\d+)(?:[eEdD][-+]?\d+)?"
 
 
 def read(path):
@@ -140,6 +140,38 @@ def main():
             "numbers": numbers_from(mb), "positions": xyz_from(mb), "charge": 0.0,
             "uhf": int(mu.group(1)) if mu else 0, "guess": "eeq", "maxiter": 250,
             "accuracy": 1.0, "etemp": 300.0,
+            "expect": {"e_total": [e_ref[k], 1e-8], "gnorm": [g_ref[k], 1e-8], "hl_gap_ev": [h_ref[k], 1e-8]},
+        }
+
+    # --- GBSA (Still kernel, water) through the calculator (test_gfn2.f90:237-308)
+    b = block(gfn2, "subroutine test_gfn2gbsa_api", "end subroutine test_gfn2gbsa_api")
+    c = dict(checks(b))
+    out["gbsa_water_api"] = {
+        "source": "test/unit/test_gfn2.f90:237-308",
+        "numbers": numbers_from(b), "positions": xyz_from(b), "charge": 0.0, "uhf": 0,
+        "guess": "eeq", "maxiter": 30, "accuracy": 1.0, "etemp": 300.0,
+        "solvent": "h2o", "solv_kernel": "still", "alpb": False,
+        "expect": {
+            "hl_gap_ev": [c["hl_gap"], 1e-5], "e_total": [c["energy"], 5e-7],
+            "gnorm": [c["norm2(gradient)"], 5e-7],
+            "gradient[0][0]": [c["gradient(1,1)"], 5e-7], "gradient[1][2]": [c["gradient(3,2)"], 5e-7],
+            "gradient[7][0]": [c["gradient(1,8)"], 5e-7],
+        },
+    }
+
+    # --- mindless01-10 in ten solvents, Still kernel, ALPB shape term on the
+    # even ones (test_gfn2.f90:745-828)
+    b = block(gfn2, "subroutine test_gfn2_mindless_solvation", "end subroutine test_gfn2_mindless_solvation")
+    e_ref, g_ref, h_ref = arr("ref_energies"), arr("ref_gnorms"), arr("ref_hlgaps")
+    solvents = re.findall(r'"(\w+)"', re.search(r"solvents\(10\)\s*=\s*\[(.*?)\]", b, re.S).group(1))
+    for k in range(10):
+        name = f"mindless{k + 1:02d}"
+        base = out[name]
+        out[name + "_solv"] = {
+            "source": f"test/unit/test_gfn2.f90:745-828; test/unit/molstock.f90 ({name})",
+            "numbers": base["numbers"], "positions": base["positions"], "charge": 0.0, "uhf": base["uhf"],
+            "guess": "eeq", "maxiter": 250, "accuracy": 1.0, "etemp": 300.0,
+            "solvent": solvents[k], "solv_kernel": "still", "alpb": (k + 1) % 2 == 0,
             "expect": {"e_total": [e_ref[k], 1e-8], "gnorm": [g_ref[k], 1e-8], "hl_gap_ev": [h_ref[k], 1e-8]},
         }
 

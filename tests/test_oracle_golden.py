@@ -108,3 +108,76 @@ def test_overlap_closed_form():
             s += c[i] * c[j] * (np.pi / g) ** 1.5 * np.exp(-a[i] * a[j] / g * r2)
     assert abs(S[0, 1] - s) < 1e-13
     assert abs(S[0, 0] - 1.0) < 1e-15
+
+
+SOLV_CASES = ["gbsa_water_api"] + [f"mindless{k:02d}_solv" for k in range(1, 11)]
+
+
+@pytest.mark.parametrize("case", SOLV_CASES)
+def test_solvation_singlepoint_golden(golden, case):
+    """GBSA and GBSA + ALPB shape term (Still kernel) in eleven solvents against
+    the reference's unit-test values (test/unit/test_gfn2.f90:237-308, 745-828)."""
+    g = golden[case]
+    r = po.singlepoint(g["numbers"], g["positions"], charge=g["charge"], uhf=g["uhf"],
+                       guess=g["guess"], maxiter=g["maxiter"], accuracy=g["accuracy"],
+                       etemp=g["etemp"], solvent=g["solvent"], solv_kernel=g["solv_kernel"],
+                       alpb=g["alpb"], threads=4)
+    assert r["converged"]
+    assert r["e_solv"] < 0.0
+    for key, (ref, tol) in g["expect"].items():
+        val = _lookup(r, key)
+        assert abs(val - ref) <= tol, f"{case} {key}: {val} vs {ref} (tol {tol})"
+
+
+def test_solvation_c_api_restart_chain(golden):
+    """The C API sequence of the reference's example program: propyne in the gas
+    phase, then GBSA (Still kernel, water) restarted from it, then ALPB (P16
+    kernel, ALPB water parameters) restarted from that
+    (test/api/c_api_example.c:114-158)."""
+    g = golden["propyne"]
+    z, x = g["numbers"], g["positions"]
+    r0 = po.singlepoint(z, x, maxiter=30)
+    r1 = po.singlepoint(z, x, maxiter=30, solvent="h2o", solv_kernel="still", restart=r0)
+    r2 = po.singlepoint(z, x, maxiter=30, solvent="h2o", solv_kernel="p16", alpb=True, restart=r1)
+    for r, e, q5, d2, w9 in ((r1, -8.38393864716134, 0.06090868805034, -0.35455233974705, 2.89453979224265),
+                             (r2, -8.384076843892, 0.0644849340, -0.3641866008, 2.8932146955)):
+        assert abs(r["e_total"] - e) <= 1e-9
+        assert abs(r["q"][5] - q5) <= 1e-8
+        assert abs(r["dipole"][2] - d2) <= 1e-6
+        assert abs(r["wbo"].ravel()[9] - w9) <= 1e-8
+
+
+@pytest.mark.parametrize("kernel,alpb", [("still", False), ("p16", True), ("still", True)])
+def test_solvation_gradient_finite_difference(kernel, alpb):
+    """Analytic gradient with implicit solvent (Born radii, surface and H-bond
+    derivatives, ALPB shape term) against central differences, all components.
+    The surface quadrature drops grid points whose accessibility is below 1e-6
+    (src/solv/sasa.f90:125), so the energy has tiny steps; the geometry is one
+    where no grid point crosses that threshold within the difference step."""
+    at = [6, 7, 8, 1, 1, 1]
+    xyz = np.array([[0.0, 0.0, 0.0], [2.5, 0.3, -0.2], [-1.2, 2.11, 0.4],
+                    [-1.1, -1.6, 0.9], [3.5, -1.2, 0.3], [-2.9, 2.0, -0.5]])
+    kw = dict(accuracy=1e-4, maxiter=200, solvent="water", solv_kernel=kernel, alpb=alpb)
+    r = po.singlepoint(at, xyz, **kw)
+    h = 1e-4
+    for a, k in [(a, k) for a in range(6) for k in range(3)]:
+        xp = xyz.copy(); xp[a, k] += h
+        xm = xyz.copy(); xm[a, k] -= h
+        ep = po.singlepoint(at, xp, **kw)["e_total"]
+        em = po.singlepoint(at, xm, **kw)["e_total"]
+        assert abs((ep - em) / (2 * h) - r["gradient"][a, k]) < 2e-8, (kernel, alpb, a, k)
+
+
+def test_solvent_names_and_unknown_solvent():
+    """Solvent aliases resolve like the reference's normalizeSolventName; an
+    unknown solvent is an error, not a silent gas-phase run."""
+    at = [8, 1, 1]
+    xyz = np.array([[0.0, 0.0, -0.73], [1.43, 0.0, 0.37], [-1.43, 0.0, 0.37]])
+    e1 = po.singlepoint(at, xyz, solvent="water")["e_total"]
+    e2 = po.singlepoint(at, xyz, solvent="H2O")["e_total"]
+    assert e1 == e2
+    e3 = po.singlepoint(at, xyz, solvent="dichloromethane")["e_total"]
+    e4 = po.singlepoint(at, xyz, solvent="ch2cl2")["e_total"]
+    assert e3 == e4 and e3 != e1
+    with pytest.raises(RuntimeError):
+        po.singlepoint(at, xyz, solvent="no-such-solvent")
