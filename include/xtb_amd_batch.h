@@ -23,6 +23,7 @@ typedef struct {
   int natoms, nshells, nao, iterations, converged, failed, ihomo;
   double energy, gnorm, hl_gap_ev;
   double e_elec, e_es, e_aes, e_axc, e_disp, e_rep, e_atm;
+  double e_solv;      /* solvation free energy (0 in the gas phase) */
   double dipole[3];
   /* caller-owned output arrays, may be NULL */
   double* gradient;   /* [natoms][3] */
@@ -48,6 +49,17 @@ const char* xtbamd_last_error(void);
  * atomic dipoles [natoms][3], quadrupoles [natoms][6] of a previous result;
  * mirrors the wavefunction reuse of xtb_singlepoint (src/api/interface.f90:83-101) */
 int xtbamd_batch_set_guess(xtbamd_batch b, int m, const double* qsh, const double* dipm, const double* qp);
+/* implicit solvent for every molecule of the batch (call before setup/run;
+ * NULL or "" returns to the gas phase).  kernel 1: Still kernel with the
+ * GFN2/GBSA parameter sets; 2: P16 kernel with the GFN2/ALPB sets; alpb adds
+ * the ALPB shape correction; state 1 gsolv, 2 reference, 3 1 bar / 1 M;
+ * temperature in K.  xtb_setSolvent's "gbsa" is (1, 0), "alpb" is (2, 1)
+ * (src/api/calculator.f90:341-440, src/solv/model.f90).  Returns -1 for a
+ * solvent without parameters. */
+int xtbamd_batch_set_solvent(xtbamd_batch b, const char* solvent, int kernel, int alpb, int state,
+                             double temperature);
+/* 1 when the solvent has a parameter set for the kernel, else 0 */
+int xtbamd_solvent_known(const char* solvent, int kernel);
 int xtbamd_batch_setup(xtbamd_batch b);
 /* one SCC iteration of every active molecule; returns the number still active */
 int xtbamd_batch_iterate(xtbamd_batch b);

@@ -118,10 +118,51 @@ int main(void) {
   CHECK(fabs(hess[9 * 49 - 1] - 0.3636571159) < 1.0e-9, "Hessian[21,21] %.12f", hess[9 * 49 - 1]);
   free(hess);
 
-  /* unsupported features leave an error */
-  char solvent[] = "h2o", model[] = "gbsa";
-  xtb_setSolvent(env, calc, solvent, NULL, NULL, NULL, model);
-  CHECK(xtb_checkEnvironment(env) == 1, "setSolvent must fail");
+  /* implicit solvent, the reference example's sequence on one results object:
+   * gas phase, then GBSA, then ALPB, each restarted from the previous
+   * wavefunction (reference test/api/c_api_example.c:114-158) */
+  char solvent[] = "h2o", gbsa[] = "gbsa", alpb[] = "alpb", cosmo[] = "cosmo", nosuch[] = "nosuchsolvent";
+  xtb_TResults rs = xtb_newResults();
+  xtb_singlepoint(env, mol, calc, rs);
+  xtb_setSolvent(env, calc, solvent, NULL, NULL, NULL, gbsa);
+  CHECK(!xtb_checkEnvironment(env), "setSolvent gbsa");
+  xtb_singlepoint(env, mol, calc, rs);
+  xtb_getEnergy(env, rs, &energy);
+  xtb_getCharges(env, rs, q);
+  xtb_getDipole(env, rs, dipole);
+  xtb_getBondOrders(env, rs, wbo);
+  CHECK(!xtb_checkEnvironment(env), "GBSA singlepoint");
+  CHECK(fabs(energy - -8.38393864716134) < 1.0e-9, "GBSA energy %.12f", energy);
+  CHECK(fabs(q[5] - 0.06090868805034) < 1.0e-8, "GBSA charge %.12f", q[5]);
+  CHECK(fabs(dipole[2] - -0.35455233974705) < 1.0e-6, "GBSA dipole %.12f", dipole[2]);
+  CHECK(fabs(wbo[9] - 2.89453979224265) < 1.0e-8, "GBSA bond order %.12f", wbo[9]);
+  xtb_releaseSolvent(env, calc);
+  xtb_setSolvent(env, calc, solvent, NULL, NULL, NULL, alpb);
+  CHECK(!xtb_checkEnvironment(env), "setSolvent alpb");
+  xtb_singlepoint(env, mol, calc, rs);
+  xtb_getEnergy(env, rs, &energy);
+  xtb_getCharges(env, rs, q);
+  xtb_getDipole(env, rs, dipole);
+  xtb_getBondOrders(env, rs, wbo);
+  CHECK(!xtb_checkEnvironment(env), "ALPB singlepoint");
+  CHECK(fabs(energy - -8.384076843892) < 1.0e-9, "ALPB energy %.12f", energy);
+  CHECK(fabs(q[5] - 0.0644849340) < 1.0e-8, "ALPB charge %.12f", q[5]);
+  CHECK(fabs(dipole[2] - -0.3641866008) < 1.0e-6, "ALPB dipole %.12f", dipole[2]);
+  CHECK(fabs(wbo[9] - 2.8932146955) < 1.0e-8, "ALPB bond order %.12f", wbo[9]);
+  /* as in the reference, only CPCM-X results carry a separate solvation energy */
+  xtb_getSolvationEnergy(env, rs, &e3);
+  CHECK(xtb_checkEnvironment(env) == 1, "getSolvationEnergy outside CPCM-X must fail");
+  xtb_getError(env, buffer, &buffersize);
+  xtb_releaseSolvent(env, calc);
+  xtb_delete(rs);
+
+  /* unknown solvents and unsupported models leave an error */
+  xtb_setSolvent(env, calc, nosuch, NULL, NULL, NULL, gbsa);
+  CHECK(xtb_checkEnvironment(env) == 1, "unknown solvent must fail");
+  xtb_getError(env, buffer, &buffersize);
+  CHECK(strstr(buffer, "Could not add solvation model") != NULL, "unknown solvent text: %s", buffer);
+  xtb_setSolvent(env, calc, solvent, NULL, NULL, NULL, cosmo);
+  CHECK(xtb_checkEnvironment(env) == 1, "setSolvent cosmo must fail");
   xtb_getError(env, buffer, &buffersize);
   CHECK(strstr(buffer, "not supported") != NULL, "unsupported text: %s", buffer);
 

@@ -58,6 +58,48 @@ def test_golden_single(golden, case):
         assert abs(val - ref) <= max(tol, 1e-7), f"{case} {key}: {val} vs {ref}"
 
 
+SOLV_CASES = ["gbsa_water_api"] + [f"mindless{k:02d}_solv" for k in range(1, 11)]
+
+
+@pytest.mark.parametrize("case", SOLV_CASES)
+def test_solvation_golden_single(golden, case):
+    """GBSA and GBSA + ALPB shape term in eleven solvents (Born radii, surface,
+    H-bond terms on the device) against the oracle and the reference's values."""
+    xa = _engine()
+    g = golden[case]
+    r = xa.singlepoint(g["numbers"], g["positions"], g["charge"], g["uhf"], g["accuracy"], g["etemp"],
+                       g["maxiter"], g["guess"], solvent=g["solvent"], solv_kernel=g["solv_kernel"], alpb=g["alpb"])
+    o = po.singlepoint(g["numbers"], g["positions"], charge=g["charge"], uhf=g["uhf"], guess=g["guess"],
+                       maxiter=g["maxiter"], accuracy=g["accuracy"], etemp=g["etemp"], solvent=g["solvent"],
+                       solv_kernel=g["solv_kernel"], alpb=g["alpb"], threads=4)
+    _cmp(r, o, case)
+    assert abs(r["e_solv"] - o["e_solv"]) <= TOL_E, f"{case} e_solv {r['e_solv']} vs {o['e_solv']}"
+    assert r["iterations"] == o["iterations"]
+    res = dict(r)
+    res["e_total"] = r["energy"]
+    for key, (ref, tol) in g["expect"].items():
+        val = _lookup(res, key)
+        assert abs(val - ref) <= max(tol, 1e-7), f"{case} {key}: {val} vs {ref}"
+
+
+@pytest.mark.parametrize("kernel,alpb", [("p16", True), ("still", False)])
+def test_solvation_clusters_vs_oracle(kernel, alpb):
+    """Water clusters and an alkane in water (ALPB / P16 kernel and GBSA / Still
+    kernel), one batch: against the oracle, and bitwise the alone results."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    mols = [w.water_cluster(12, 3), w.alkane(10, 4), w.water_cluster(32, 0)]
+    b = xa.Batch([(z, x, 0.0, 0) for z, x in mols]).set_solvent("water", kernel, alpb).run()
+    mix = b.results()
+    b.close()
+    for k, ((z, x), rm) in enumerate(zip(mols, mix)):
+        o = po.singlepoint(z, x, solvent="water", solv_kernel=kernel, alpb=alpb, threads=8, lapack=True)
+        _cmp(rm, o, f"{kernel} mol{k} n={len(z)}")
+        assert abs(rm["e_solv"] - o["e_solv"]) <= TOL_E
+        ra = xa.singlepoint(z, x, solvent="water", solv_kernel=kernel, alpb=alpb)
+        _same(ra, rm, f"{kernel} alone mol{k}")
+
+
 def test_golden_batch_same_alone_and_in_mix(golden):
     """All golden molecules in one batch: bitwise the alone results."""
     xa = _engine()

@@ -90,7 +90,7 @@ struct Capacity {
 struct MolResult {
   int n = 0, nsh = 0, nao = 0, iterations = 0, converged = 0, failed = 0, ihomo = 0;
   double energy = 0, gnorm = 0, hl_gap_ev = 0, e_elec = 0, e_es = 0, e_aes = 0, e_axc = 0;
-  double e_disp = 0, e_rep = 0, e_atm = 0;
+  double e_disp = 0, e_rep = 0, e_atm = 0, e_solv = 0;
   double dipole[3] = {0, 0, 0};
   std::vector<double> gradient, q, qsh, dipm, qp, wbo, emo, focc, coeff, density;
 };
@@ -124,6 +124,8 @@ struct JacSet {
   long long qsize = 0;
 };
 
+struct SolvDev;   // implicit-solvent model on the device (k_solv.hip)
+
 class Batch {
  public:
   // a batch of fixed molecules (packed layout)
@@ -145,6 +147,12 @@ class Batch {
   // restart guess for molecule m: shell charges, atomic dipoles [n][3] and
   // quadrupoles [n][6] of a previous wavefunction (used by the next setup)
   void set_guess(int m, const double* qsh_in, const double* dipm_in, const double* qp_in);
+  // implicit solvent for every molecule of the batch (empty name: gas phase);
+  // kernel 1 Still (GBSA parameter sets) or 2 P16 (ALPB parameter sets), alpb
+  // adds the ALPB shape term, state 1 gsolv / 2 reference / 3 1 bar.  Throws
+  // for a solvent without parameters.  Takes effect at the next setup().
+  void set_solvent(const std::string& name, int kernel, bool alpb, int state = 1, double temperature = 298.15);
+  bool solvated() const { return solv_on_; }
   MolResult result(int m, bool matrices = false) const;
   int size() const { return nmol; }
   // sizes and index maps of one molecule; throws for invalid input
@@ -221,6 +229,16 @@ class Batch {
   int* perm = nullptr;
   int* eeq_failed = nullptr;
   int* eeq_big = nullptr;        // sub-list indices of EEQ systems factorised column by column
+  // implicit solvation: device model, Born radii and chain factors, surface
+  // (per atom and per grid point), H-bond weights, potential, dE/dbrad, the
+  // interaction matrix (n x n at p0) and per-molecule surface energy, shape
+  // descriptor and q^T B q / 2
+  bool solv_on_ = false;
+  SolvDev* d_solv = nullptr;
+  double *sv_brad = nullptr, *sv_dpsi = nullptr, *sv_sasa = nullptr, *sv_hbw = nullptr, *sv_pot = nullptr;
+  double *sv_grddb = nullptr, *sv_w = nullptr, *sv_B = nullptr, *sv_gsasa = nullptr, *sv_adet = nullptr;
+  double *sv_qbq = nullptr, sv_gshift = 0.0;
+  int* sv_failed = nullptr;
   double* ns_c = nullptr;
   unsigned long long* ns_res = nullptr;
   int *ns_stage = nullptr, *ns_act = nullptr, *ns_failed = nullptr;

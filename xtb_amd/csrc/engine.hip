@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cctype>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -19,6 +20,7 @@
 #include "k_linalg.hip"
 #include "k_scc.hip"
 #include "k_setup.hip"
+#include "k_solv.hip"
 
 #define HIPCHECK(x)                                                                              \
   do {                                                                                           \
@@ -928,6 +930,37 @@ Batch::Batch(const Capacity& cap, const Options& o, hipStream_t s) : opt(o), str
   rebuild();
 }
 
+void Batch::set_solvent(const std::string& name, int kernel, bool alpb, int state, double temperature) {
+  if (name.empty()) {
+    solv_on_ = false;
+    setup_done_ = false;
+    return;
+  }
+  if (kernel != 1 && kernel != 2) throw std::runtime_error("unknown Born kernel");
+  const SolvParamSet* ps = solv_param_set(name, kernel);
+  if (!ps) throw std::runtime_error("solvent: '" + name + "' is not parametrized");
+  for (int z = 1; z <= XTB_MAX_Z; ++z)
+    if (elem_present_[z] && kVdwRadD3Aa[z - 1] <= 0.0) throw std::runtime_error("element without solvation radius");
+  const SolvDev model = solv_model(*ps, kernel, alpb, state, temperature);
+  if (!d_solv) {
+    const long long tn = std::max(1LL, cap_.n), tpp = std::max(1LL, cap_.pp);
+    d_solv = alloc<SolvDev>(1);
+    sv_brad = alloc<double>(tn); sv_dpsi = alloc<double>(tn); sv_sasa = alloc<double>(tn);
+    sv_hbw = alloc<double>(tn); sv_pot = alloc<double>(tn); sv_grddb = alloc<double>(tn);
+    sv_w = alloc<double>(tn * kSolvGrid);
+    sv_B = alloc<double>(tpp);
+    sv_gsasa = alloc<double>(nmol); sv_adet = alloc<double>(nmol); sv_qbq = alloc<double>(nmol);
+    sv_failed = alloc<int>(nmol);
+    HIPCHECK(hipMemset(sv_failed, 0, nmol * sizeof(int)));
+    HIPCHECK(hipMemset(sv_qbq, 0, nmol * sizeof(double)));
+    HIPCHECK(hipMemset(sv_gsasa, 0, nmol * sizeof(double)));
+  }
+  HIPCHECK(hipMemcpy(d_solv, &model, sizeof(SolvDev), hipMemcpyHostToDevice));
+  sv_gshift = model.gshift;
+  solv_on_ = true;
+  setup_done_ = false;
+}
+
 Batch::~Batch() {
   for (auto& v : prof_ev_)
     for (hipEvent_t e : v) (void)hipEventDestroy(e);
@@ -1239,6 +1272,19 @@ void Batch::setup() {
                      d_sub, el, at, xyz, d4_at, d4_ref, gw, c6ref, dispmat);
   hipLaunchKernelGGL(k_aes_setup, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub,
                      el, at, xyz, cn, radcn, gab3, gab5);
+  // implicit solvent: Born radii, surface, H-bond weights, shape, matrix B
+  if (solv_on_ && sub_max_n > 0) {
+    hipLaunchKernelGGL(k_eeq_reset, dim3(grid1(ns, 64)), dim3(64), 0, stream, d_sub, ns, sv_failed);
+    hipLaunchKernelGGL(k_solv_born, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub,
+                       (const SolvDev*)d_solv, at, xyz, sv_brad, sv_dpsi);
+    hipLaunchKernelGGL(k_solv_sasa, dim3(sub_max_n, ns), dim3(64), 0, stream, d_sub, (const SolvDev*)d_solv, at, xyz,
+                       sv_w, sv_sasa, sv_failed);
+    hipLaunchKernelGGL(k_solv_mol, dim3(ns), dim3(256), 0, stream, d_sub, (const SolvDev*)d_solv, at, xyz,
+                       (const double*)sv_sasa, sv_hbw, sv_gsasa, sv_adet);
+    hipLaunchKernelGGL(k_solv_bmat, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream,
+                       d_sub, (const SolvDev*)d_solv, xyz, (const double*)sv_brad, (const double*)sv_hbw,
+                       (const double*)sv_adet, sv_B);
+  }
   // initial charges: EEQ (default) or SAD
   if (opt.guess == 0) {
     const long long ne = sub_max_n + 1;
@@ -1302,8 +1348,12 @@ bool Batch::scc_iteration() {
                      gw, q, zv, dzv, (const MolState*)d_state);
   hipLaunchKernelGGL(k_d4_symv, dim3(grid1(max_nd4, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)dispmat,
                      (const double*)zv, d4dum, (const MolState*)d_state);
+  if (solv_on_)
+    hipLaunchKernelGGL(k_solv_pot, dim3(max_n, nmol), dim3(64), 0, stream, d_desc, (const double*)sv_B,
+                       (const double*)q, sv_pot, (const MolState*)d_state);
   hipLaunchKernelGGL(k_shift_final, dim3(grid1(max_nsh, 64), nmol), dim3(64), 0, stream, d_desc, el, at, sh_at,
-                     (const double*)d4dum, (const double*)dzv, shift, (const MolState*)d_state);
+                     (const double*)d4dum, (const double*)dzv, solv_on_ ? (const double*)sv_pot : nullptr, shift,
+                     (const MolState*)d_state);
   hipLaunchKernelGGL(k_fock, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream, d_desc, S, H0,
                      D, Q, shift, vs, vd, vq, ao_sh, ao_at, thr2, H, (const MolState*)d_state);
   // generalized eigenproblem in the current orbital basis C (S-orthonormal)
@@ -1345,10 +1395,13 @@ bool Batch::scc_iteration() {
                      gw, q, zv, (double*)nullptr, (const MolState*)d_state);
   hipLaunchKernelGGL(k_d4_symv, dim3(grid1(max_nd4, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)dispmat,
                      (const double*)zv, d4dum, (const MolState*)d_state);
+  if (solv_on_)
+    hipLaunchKernelGGL(k_solv_energy, dim3(nmol), dim3(256), 0, stream, d_desc, (const double*)sv_B, (const double*)q,
+                       sv_qbq, (const MolState*)d_state);
   hipLaunchKernelGGL(k_scc_control, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, (const double*)jq,
                      (const double*)shgam, (const double*)row_e, (const double*)ae01, (const double*)zv,
-                     (const double*)d4dum, sh_at, qsh, q, dipm, qp, q_in, dq, qlast, dqlast, bdf, bu, ba, bomega,
-                     scfconv, qconv);
+                     (const double*)d4dum, solv_on_ ? (const double*)sv_qbq : nullptr, (const double*)sv_gsasa,
+                     sv_gshift, sh_at, qsh, q, dipm, qp, q_in, dq, qlast, dqlast, bdf, bu, ba, bomega, scfconv, qconv);
   HIPCHECK(hipGetLastError());
   iterations_ += 1;
   // any molecule still active?
@@ -1388,6 +1441,19 @@ void Batch::gradient() {
                      erep);
   hipLaunchKernelGGL(k_cn_chain, gat, dim3(64), 0, stream, d_sub, el, at, xyz, (const double*)cn,
                      (const double*)dhdcn, (const double*)tmp2, (const double*)dedcn, grad);
+  // implicit solvent: kernel, Born radius, surface, H-bond and shape terms, and
+  // the solvation free energy at the final charges
+  if (solv_on_) {
+    hipLaunchKernelGGL(k_solv_grddb, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub,
+                       (const SolvDev*)d_solv, xyz, (const double*)q, (const double*)sv_brad, sv_grddb);
+    hipLaunchKernelGGL(k_solv_grad, gat, dim3(64), 0, stream, d_sub, (const SolvDev*)d_solv, at, xyz, (const double*)q,
+                       (const double*)sv_brad, (const double*)sv_dpsi, (const double*)sv_grddb, (const double*)sv_w,
+                       grad, (const int*)nullptr);
+    hipLaunchKernelGGL(k_solv_alpb_grad, dim3(ns), dim3(64), 0, stream, d_sub, (const SolvDev*)d_solv, at, xyz,
+                       (const double*)q, grad, (const int*)nullptr);
+    hipLaunchKernelGGL(k_solv_energy, dim3(ns), dim3(256), 0, stream, d_sub, (const double*)sv_B, (const double*)q,
+                       sv_qbq, (const MolState*)nullptr);
+  }
   // bond orders from P S
   gemm(0, P, S, A, nullptr, nullptr, false, 1.0, phase_act);
   hipLaunchKernelGGL(k_wbo, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub, xyz,
@@ -1430,6 +1496,15 @@ MolResult Batch::result(int m, bool matrices) const {
   r.e_disp = st.ed + st.embd;
   r.e_rep = st.ep;
   r.e_atm = st.embd;
+  if (solv_on_) {
+    double qbq = 0.0, gs = 0.0;
+    int bad = 0;
+    HIPCHECK(hipMemcpy(&qbq, sv_qbq + m, sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(&gs, sv_gsasa + m, sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(&bad, sv_failed + m, sizeof(int), hipMemcpyDeviceToHost));
+    r.e_solv = (qbq + gs) + sv_gshift;
+    if (bad && !r.failed) r.failed = 7;
+  }
   for (int k = 0; k < 3; ++k) r.dipole[k] = st.dipole[k];
   auto get = [&](std::vector<double>& v, const double* src, long long off, long long cnt) {
     v.resize(cnt);
@@ -1602,6 +1677,7 @@ void copy_result(const xtbamd::MolResult& r, xtbamd_result* out) {
   out->e_disp = r.e_disp;
   out->e_rep = r.e_rep;
   out->e_atm = r.e_atm;
+  out->e_solv = r.e_solv;
   for (int k = 0; k < 3; ++k) out->dipole[k] = r.dipole[k];
   auto put = [](double* dst, const std::vector<double>& v) {
     if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(double));
@@ -1680,6 +1756,21 @@ int xtbamd_batch_set_guess(xtbamd_batch b, int m, const double* qsh, const doubl
     b->batch->set_guess(m, qsh, dipm, qp);
     return 0;
   } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_set_solvent(xtbamd_batch b, const char* solvent, int kernel, int alpb, int state,
+                             double temperature) {
+  try {
+    if (!b) throw std::runtime_error("invalid batch");
+    if (state < 1 || state > 3) throw std::runtime_error("unknown solution state");
+    b->batch->set_solvent(solvent ? std::string(solvent) : std::string(), kernel, alpb != 0, state, temperature);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_solvent_known(const char* solvent, int kernel) {
+  if (!solvent || (kernel != 1 && kernel != 2)) return 0;
+  return xtbamd::solv_param_set(solvent, kernel) ? 1 : 0;
 }
 
 int xtbamd_batch_setup(xtbamd_batch b) {

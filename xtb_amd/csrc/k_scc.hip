@@ -175,7 +175,8 @@ __global__ void __launch_bounds__(256) k_d4_symv(const MolDesc* __restrict__ des
 // atomic D4 potential and the final shell shift (addToShellShift: src/scc_core.f90:779-796)
 __global__ void k_shift_final(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
                               const int* __restrict__ at, const int* __restrict__ sh_at, const double* __restrict__ d4dum,
-                              const double* __restrict__ dzv, double* shift, const MolState* __restrict__ mst) {
+                              const double* __restrict__ dzv, const double* __restrict__ solv_pot, double* shift,
+                              const MolState* __restrict__ mst) {
   const MolDesc md = desc[blockIdx.y];
   if (mst && !mst[blockIdx.y].active) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -186,6 +187,8 @@ __global__ void k_shift_final(const MolDesc* __restrict__ desc, const Element* _
   const int nr = el[at[md.a0 + ia]].nref;
   double v = 0.0;
   for (int r = 0; r < nr; ++r) v += d4dum[md.d0 + k0 + r] * dzv[md.d0 + k0 + r];
+  // atomic shift: solvation first, then dispersion (src/scc_core.f90:440-452)
+  if (solv_pot) v = solv_pot[md.a0 + ia] + v;
   shift[md.s0 + s] += v;
 }
 
@@ -496,7 +499,8 @@ __global__ void k_aes_energy(const MolDesc* __restrict__ desc, const Element* __
 __global__ void __launch_bounds__(256) k_scc_control(
     const MolDesc* __restrict__ desc, MolState* state, const double* __restrict__ jq, const double* __restrict__ gam,
     const double* __restrict__ row_e, const double* __restrict__ ae, const double* __restrict__ zv,
-    const double* __restrict__ d4dum, const int* __restrict__ sh_at, double* qsh, double* q, double* dipm, double* qp,
+    const double* __restrict__ d4dum, const double* __restrict__ solv_qbq, const double* __restrict__ solv_gsasa,
+    double solv_gshift, const int* __restrict__ sh_at, double* qsh, double* q, double* dipm, double* qp,
     double* q_in, double* dq, double* qlast, double* dqlast, double* bdf, double* bu, double* ba, double* bomega,
     double scfconv, double qconv) {
   __shared__ double sh[4];
@@ -536,6 +540,8 @@ __global__ void __launch_bounds__(256) k_scc_control(
   const double eaes = a01 + a02 + a11;
   eel = eel + eaes + apol;
   eel = eel + edisp;
+  // solvation free energy (src/scc_core.f90:544-548, src/solv/gbsa.f90:525-548)
+  if (solv_qbq) eel = eel + ((solv_qbq[blockIdx.x] + solv_gsasa[blockIdx.x]) + solv_gshift);
   eel = eel + st.ga + st.gb;
   const double eold = st.eel;
   const bool econv = fabs(eel - eold) < scfconv;

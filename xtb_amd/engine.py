@@ -18,7 +18,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 class Result(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int) for k in ("natoms", "nshells", "nao", "iterations", "converged", "failed", "ihomo")] + \
         [(k, ctypes.c_double) for k in ("energy", "gnorm", "hl_gap_ev", "e_elec", "e_es", "e_aes", "e_axc",
-                                        "e_disp", "e_rep", "e_atm")] + \
+                                        "e_disp", "e_rep", "e_atm", "e_solv")] + \
         [("dipole", ctypes.c_double * 3)] + \
         [(k, _dp) for k in ("gradient", "charges", "qsh", "dipm", "qp", "wbo", "emo", "focc", "coeff", "density")]
 
@@ -41,6 +41,9 @@ def library():
         getattr(lib, f).argtypes = [ctypes.c_void_p]
         getattr(lib, f).restype = ctypes.c_int
     lib.xtbamd_batch_set_guess.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
+    lib.xtbamd_batch_set_solvent.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_double]
+    lib.xtbamd_solvent_known.argtypes = [ctypes.c_char_p, ctypes.c_int]
     lib.xtbamd_batch_set_guess.restype = ctypes.c_int
     lib.xtbamd_batch_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Result)]
     lib.xtbamd_batch_get.restype = ctypes.c_int
@@ -124,6 +127,15 @@ class Batch:
         qp = np.ascontiguousarray(qp, dtype=np.float64)
         self._check(self._lib.xtbamd_batch_set_guess(self._h, m, _ptr(qsh), _ptr(dipm), _ptr(qp)))
 
+    def set_solvent(self, solvent, kernel="still", alpb=False, state=1, temperature=298.15):
+        """Implicit solvent for every molecule (None: gas phase).  kernel "still"
+        uses the GFN2/GBSA parameter sets, "p16" the GFN2/ALPB sets; alpb adds
+        the ALPB shape term (the C API's "gbsa" is still/False, "alpb" p16/True)."""
+        k = {"still": 1, "p16": 2}[kernel]
+        self._check(self._lib.xtbamd_batch_set_solvent(self._h, solvent.encode() if solvent else None, k,
+                                                      1 if alpb else 0, int(state), float(temperature)))
+        return self
+
     def setup(self):
         self._check(self._lib.xtbamd_batch_setup(self._h))
 
@@ -177,7 +189,7 @@ class Batch:
             setattr(r, k, _ptr(v))
         self._check(lib.xtbamd_batch_get(self._h, m, ctypes.byref(r)))
         for k in ("natoms", "nshells", "nao", "iterations", "converged", "failed", "ihomo", "energy", "gnorm",
-                  "hl_gap_ev", "e_elec", "e_es", "e_aes", "e_axc", "e_disp", "e_rep", "e_atm"):
+                  "hl_gap_ev", "e_elec", "e_es", "e_aes", "e_axc", "e_disp", "e_rep", "e_atm", "e_solv"):
             out[k] = getattr(r, k)
         out["dipole"] = np.array(r.dipole[:])
         return out
@@ -209,7 +221,7 @@ def _fetch(lib, get, check):
         setattr(r, k, _ptr(v))
     check(get(ctypes.byref(r)))
     for k in ("natoms", "nshells", "nao", "iterations", "converged", "failed", "ihomo", "energy", "gnorm",
-              "hl_gap_ev", "e_elec", "e_es", "e_aes", "e_axc", "e_disp", "e_rep", "e_atm"):
+              "hl_gap_ev", "e_elec", "e_es", "e_aes", "e_axc", "e_disp", "e_rep", "e_atm", "e_solv"):
         out[k] = getattr(r, k)
     out["dipole"] = np.array(r.dipole[:])
     return out
@@ -270,9 +282,11 @@ class Server:
 
 
 def singlepoint(numbers, positions, charge=0.0, uhf=0, accuracy=1.0, etemp=300.0, maxiter=250, guess="eeq",
-                matrices=False):
+                matrices=False, solvent=None, solv_kernel="still", alpb=False):
     b = Batch([(numbers, positions, charge, uhf)], accuracy, etemp, maxiter, guess)
     try:
+        if solvent:
+            b.set_solvent(solvent, solv_kernel, alpb)
         b.run()
         return b.result(0, matrices)
     finally:

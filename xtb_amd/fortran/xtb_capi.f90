@@ -29,6 +29,7 @@ module xtbamd_capi
    type, bind(C) :: engine_result
       integer(c_int) :: natoms, nshells, nao, iterations, converged, failed, ihomo
       real(c_double) :: energy, gnorm, hl_gap_ev, e_elec, e_es, e_aes, e_axc, e_disp, e_rep, e_atm
+      real(c_double) :: e_solv
       real(c_double) :: dipole(3)
       type(c_ptr) :: gradient, charges, qsh, dipm, qp, wbo, emo, focc, coeff, density
    end type engine_result
@@ -68,6 +69,21 @@ module xtbamd_capi
          type(engine_result), intent(inout) :: res
          integer(c_int) :: rc
       end function engine_get
+      function engine_set_solvent(handle, solvent, kernel, alpb, state, temperature) result(rc) &
+            & bind(C, name="xtbamd_batch_set_solvent")
+         import :: c_int, c_double, c_ptr, c_char
+         type(c_ptr), value :: handle
+         character(kind=c_char), intent(in) :: solvent(*)
+         integer(c_int), value :: kernel, alpb, state
+         real(c_double), value :: temperature
+         integer(c_int) :: rc
+      end function engine_set_solvent
+      function engine_solvent_known(solvent, kernel) result(ok) bind(C, name="xtbamd_solvent_known")
+         import :: c_int, c_char
+         character(kind=c_char), intent(in) :: solvent(*)
+         integer(c_int), value :: kernel
+         integer(c_int) :: ok
+      end function engine_solvent_known
       subroutine engine_free(handle) bind(C, name="xtbamd_batch_free")
          import :: c_ptr
          type(c_ptr), value :: handle
@@ -108,6 +124,12 @@ module xtbamd_capi
       real(wp) :: accuracy = 1.0_wp
       real(wp) :: etemp = 300.0_wp
       integer :: maxiter = 250
+      !> implicit solvent (src/api/calculator.f90:341-440): name, Born kernel
+      !> (1 Still / GBSA parameters, 2 P16 / ALPB parameters), ALPB shape term,
+      !> solution state and temperature; unallocated name: gas phase
+      character(len=:), allocatable :: solvent
+      integer(c_int) :: solv_kernel = 1, solv_alpb = 0, solv_state = 1
+      real(wp) :: solv_temp = 298.15_wp
    end type calculator_t
 
    type :: results_t
@@ -556,12 +578,56 @@ contains
       character(len=*), parameter :: source = 'xtb_api_setSolvent'
       type(environment_t), pointer :: env
       type(calculator_t), pointer :: calc
+      character(len=:), allocatable :: solvent, model
+      integer(c_int) :: gstate, kernel, alpb, nang
+      real(wp) :: temp
       if (.not. calc_checks(venv, vcalc, source, env, calc)) return
       if (.not. calc%loaded) then
          call env_error(env, "No calculator loaded to add solvation model", source)
          return
       end if
-      call env_error(env, "Solvation models are not supported by this GFN2-xTB implementation", source)
+      gstate = 1
+      if (present(state)) gstate = state
+      temp = 298.15_wp
+      if (present(temperature)) temp = temperature
+      nang = 230
+      if (present(grid)) nang = grid
+      call c_to_f_string(charptr, solvent)
+      model = 'gbsa'
+      if (present(charptr2)) call c_to_f_string(charptr2, model)
+      select case(model)
+      case('gbsa')
+         kernel = 1
+         alpb = 0
+      case('alpb')
+         kernel = 2
+         alpb = 1
+      case('cosmo', 'tmcosmo', 'cpcmx')
+         call env_error(env, "Solvation model '" // model // "' is not supported by this GFN2-xTB implementation", &
+            & source)
+         return
+      case default
+         call env_error(env, "Unknown solvation model", source)
+         return
+      end select
+      if (nang /= 230) then
+         call env_error(env, "Only the 230-point angular grid is supported by this GFN2-xTB implementation", source)
+         return
+      end if
+      if (gstate < 1 .or. gstate > 3) then
+         call env_error(env, "Unknown solution state", source)
+         return
+      end if
+      if (engine_solvent_known(solvent // c_null_char, kernel) == 0) then
+         call env_error(env, "solvent: '" // solvent // "' is not parametrized", 'solv_model_loadInternalParam')
+         call env_error(env, "Could not add solvation model for '" // solvent // "'", source)
+         return
+      end if
+      calc%solvent = solvent
+      calc%solv_kernel = kernel
+      calc%solv_alpb = alpb
+      calc%solv_state = gstate
+      calc%solv_temp = temp
    end subroutine setSolvent_api
 
    subroutine releaseSolvent_api(venv, vcalc) bind(C, name="xtb_releaseSolvent")
@@ -569,6 +635,7 @@ contains
       type(environment_t), pointer :: env
       type(calculator_t), pointer :: calc
       if (.not. calc_checks(venv, vcalc, 'xtb_api_releaseSolvent', env, calc)) return
+      if (allocated(calc%solvent)) deallocate(calc%solvent)
    end subroutine releaseSolvent_api
 
    subroutine setExternalCharges_api(venv, vcalc, n, numbers, charges, positions) &
@@ -704,6 +771,7 @@ contains
          call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
          return
       end if
+      if (.not. apply_solvent(env, calc, handle)) return
       ! sizes
       er%gradient = c_null_ptr; er%charges = c_null_ptr; er%qsh = c_null_ptr; er%dipm = c_null_ptr
       er%qp = c_null_ptr; er%wbo = c_null_ptr; er%emo = c_null_ptr; er%focc = c_null_ptr
@@ -768,6 +836,26 @@ contains
          call env_error(env, "Self consistent charge iterator did not converge", 'scf')
       end if
    end subroutine singlepoint_api
+
+   !> pass the calculator's implicit solvent to an engine batch; frees the
+   !> batch and logs the error on failure
+   function apply_solvent(env, calc, handle) result(ok)
+      type(environment_t), intent(inout) :: env
+      type(calculator_t), intent(in) :: calc
+      type(c_ptr), intent(inout) :: handle
+      logical :: ok
+      integer(c_int) :: rc
+      ok = .true.
+      if (.not. allocated(calc%solvent)) return
+      rc = engine_set_solvent(handle, calc%solvent // c_null_char, calc%solv_kernel, calc%solv_alpb, &
+         & calc%solv_state, calc%solv_temp)
+      if (rc /= 0) then
+         call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
+         call engine_free(handle)
+         handle = c_null_ptr
+         ok = .false.
+      end if
+   end function apply_solvent
 
    function engine_message() result(text)
       character(len=:), allocatable :: text
@@ -924,6 +1012,7 @@ contains
          call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
          return
       end if
+      if (.not. apply_solvent(env, calc, handle)) return
       do m = 1, nmol
          rc = engine_set_guess(handle, int(m - 1, c_int), qsh0, dipm0, qp0)
       end do
