@@ -7,10 +7,11 @@
  * every call a no-op).  The implementation is the Fortran layer in
  * xtb_amd/fortran/xtb_capi.f90 on top of the MI355X engine.
  *
- * Supported: GFN2-xTB single points (energy + gradient) of molecules.
- * GFN0/GFN1/GFN-FF, solvation, external charges and CPCM-X set an
- * environment error "... is not supported by this GFN2-xTB implementation";
+ * Supported: GFN2-xTB single points (energy + gradient) of molecules, in the
+ * gas phase or in implicit solvent (xtb_setSolvent models "gbsa" and "alpb");
  * xtb_hessian runs its 6n displaced single points as one GPU batch.
+ * GFN0/GFN1/GFN-FF, COSMO/CPCM-X and external charges set an environment
+ * error "... is not supported by this GFN2-xTB implementation".
  * Units: bohr, Hartree, Hartree/bohr, e; orbital energies in Hartree.
  */
 #pragma once
