@@ -128,6 +128,7 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=40.0)
     ap.add_argument("--serve-hbm-gb", type=float, default=16.0)
     ap.add_argument("--no-singles", action="store_true", help="skip the one-molecule timings (C0, C1, C4 sizes)")
+    ap.add_argument("--no-solvent", action="store_true", help="skip the batched step in implicit water")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="run one plain device copy of this size first (byte-counter calibration)")
     args = ap.parse_args()
@@ -239,6 +240,18 @@ def main():
     out["counters"] = {"scc_steps": steps_run, "jacobi_rounds": jac_rounds}
     if out["plain"]["value"]:
         out["speedup_batched_vs_plain"] = out["value"] / out["plain"]["value"]
+    if not args.no_solvent:
+        # the same step in implicit water (ALPB: P16 kernel + shape term)
+        b.set_solvent("water", "p16", alpb=True)
+        b.run()
+        t = time.perf_counter()
+        b.run()
+        dt = time.perf_counter() - t
+        rs = b.results()
+        out["solvated"] = {"solvent": "water (ALPB)", "ms_per_step": 1e3 * dt, "value": len(mols) / dt,
+                           "unit": "single-points/s", "converged": all(r["converged"] for r in rs),
+                           "scc_iterations_mean": float(np.mean([r["iterations"] for r in rs])),
+                           "e_solv_mean": float(np.mean([r["e_solv"] for r in rs]))}
     b.close()
     if args.serve_requests > 0:
         out["serving"] = serve(args)

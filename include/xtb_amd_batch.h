@@ -100,6 +100,10 @@ int xtbamd_server_finished(xtbamd_server s, long long* ids, int max_ids);
  * are filled and the result is kept, otherwise it is released */
 int xtbamd_server_result(xtbamd_server s, long long id, xtbamd_result* out);
 int xtbamd_server_pending(xtbamd_server s, int* queued, int* running);
+/* implicit solvent of every request admitted from now on (arguments as
+ * xtbamd_batch_set_solvent); only while no request is running */
+int xtbamd_server_set_solvent(xtbamd_server s, const char* solvent, int kernel, int alpb, int state,
+                              double temperature);
 void xtbamd_server_free(xtbamd_server s);
 
 /* copies `bytes` bytes device to device with one plain copy kernel (k_copy):

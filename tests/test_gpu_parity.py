@@ -292,6 +292,28 @@ def test_server_results_same_alone_and_in_mix():
     srv.close()
 
 
+def test_server_in_solvent_same_alone_and_in_mix():
+    """Continuous batching in implicit solvent (ALPB water): staggered requests
+    give the bitwise results of the same molecules alone in a batch."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    mols = w.mixed_batch(8, seed=33, lo=10, hi=90)
+    srv = xa.Server(hbm_gb=1.0, max_atoms=200, max_slots=32).set_solvent("water", "p16", alpb=True)
+    ids = [srv.submit(z, x) for z, x in mols[:4]]
+    for _ in range(2):
+        srv.step()
+    ids += [srv.submit(z, x) for z, x in mols[4:]]
+    steps = 0
+    while srv.step() > 0:
+        steps += 1
+        assert steps < 2000
+    for rid, (z, x) in zip(ids, mols):
+        rs = srv.result(rid)
+        assert rs["converged"] and rs["e_solv"] < 0.0
+        _same(xa.singlepoint(z, x, solvent="water", solv_kernel="p16", alpb=True), rs, f"request {rid}")
+    srv.close()
+
+
 def test_server_admits_by_budget():
     """A budget that holds only a few molecules at a time still serves a
     long queue, and the results agree with the oracle."""

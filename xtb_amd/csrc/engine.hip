@@ -1899,6 +1899,18 @@ int xtbamd_server_pending(xtbamd_server s, int* queued, int* running) {
   return 0;
 }
 
+int xtbamd_server_set_solvent(xtbamd_server s, const char* solvent, int kernel, int alpb, int state,
+                              double temperature) {
+  try {
+    if (!s) throw std::runtime_error("invalid server");
+    if (s->server->running() > 0) throw std::runtime_error("the solvent can only change while no request is running");
+    if (state < 1 || state > 3) throw std::runtime_error("unknown solution state");
+    s->server->batch.set_solvent(solvent ? std::string(solvent) : std::string(), kernel, alpb != 0, state,
+                                 temperature);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
 void xtbamd_server_free(xtbamd_server s) {
   if (!s) return;
   delete s->server;

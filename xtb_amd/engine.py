@@ -67,6 +67,8 @@ def library():
     lib.xtbamd_server_pending.argtypes = [ctypes.c_void_p, ip, ip]
     lib.xtbamd_server_free.restype = None
     lib.xtbamd_server_free.argtypes = [ctypes.c_void_p]
+    lib.xtbamd_server_set_solvent.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_double]
     lib.xtbamd_batch_counters.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_longlong)]
     lib.xtbamd_batch_counters.restype = ctypes.c_int
@@ -251,6 +253,13 @@ class Server:
         if rid < 0:
             raise RuntimeError(_err(self._lib))
         return rid
+
+    def set_solvent(self, solvent, kernel="still", alpb=False, state=1, temperature=298.15):
+        """Implicit solvent of the requests admitted from now on (see Batch.set_solvent)."""
+        k = {"still": 1, "p16": 2}[kernel]
+        self._check(self._lib.xtbamd_server_set_solvent(self._h, solvent.encode() if solvent else None, k,
+                                                       1 if alpb else 0, int(state), float(temperature)))
+        return self
 
     def step(self):
         """Admit, one SCC iteration of all resident molecules, finish; returns requests not finished."""
