@@ -101,7 +101,7 @@ struct JacobiProb {
   int dim, nb, nbp, rounds;   // 32-blocks, padded even count, rounds per sweep
   long long a_off, w_off, q_off;
   int ev_off;                 // eigenvalue offset
-  int pad;
+  int group;                  // lane (stream) of the two-lane round schedule
 };
 
 // A set of Jacobi problems (one per molecule) with its device work lists.
@@ -113,6 +113,9 @@ struct JacSet {
   int2* d_pairs = nullptr;
   int4 *d_ta = nullptr, *d_tw = nullptr, *d_to = nullptr;
   int nto = 0;
+  // the work lists hold lane 0's items first: lane g owns pairs[gpairs[g],
+  // gpairs[g+1]) and likewise for the tile lists
+  int gpairs[3] = {0, 0, 0}, gta[3] = {0, 0, 0}, gtw[3] = {0, 0, 0}, gto[3] = {0, 0, 0};
   int* d_big = nullptr;   // problems above 64 (tridiagonal QL path)
   int nbig = 0, big_max_dim = 0;
   unsigned long long* offmax = nullptr;
@@ -170,6 +173,11 @@ class Batch {
   // --- layout (public for the kernel launchers) ---
   Options opt;
   hipStream_t stream;
+  // second lane of the block-Jacobi rounds: half of the problems run their
+  // rounds on stream2, beside the other half on stream
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool two_lanes = true;
   int nmol = 0, max_n = 0, max_nsh = 0, max_nao = 0, max_nd4 = 0, max_nbr = 0;
   double intcut = 25.0, neglect = 1e-8, scfconv = 1e-6, qconv = 1e-4;
   std::vector<MolDesc> desc;
@@ -180,6 +188,7 @@ class Batch {
   int* phase_act = nullptr;     // 1 for selected molecules (per slot)
   std::vector<int> sel_;
   MolState* d_state = nullptr;
+  MolState* d_state_stage = nullptr;   // staging for setup()'s one upload of initial states
   Capacity cap_;
   Arena ar_[13];
   std::vector<char> occupied_;
@@ -249,6 +258,7 @@ class Batch {
   int ntiles = 0;
   int4* d_tiles = nullptr;
   // returns true when W comes out as rows (row j = eigenvector j)
+  void jacobi_lanes(JacSet& js, double* Am, double* Wm);
   bool jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w = false,
               int banded = 0);
   int banded_passes = 2;   // banded rounds pairs after a warm-start refinement
@@ -260,8 +270,10 @@ class Batch {
   void refine(double* Am, double* Wm);
   int oa_max_steps = 0;     // refinement steps before the Jacobi sweeps (0 = off)
   int inner_sweeps = 1;    // sweeps of each pivot problem per block-Jacobi round
-  bool pivot_lds = false;
-  bool apply_lds = false;  // 32x32 pivots: LDS tile updates with W as columns instead of register ones with W as rows  // 32x32 pivots: LDS kernel instead of the register-resident one
+  double jac_tol = 1e-12;   // Jacobi convergence: max |A_ij| <= jac_tol * max(|A_ii|, 1)
+  bool trace_rounds = false;   // print the Jacobi rounds of each SCC iteration to stderr
+  bool pivot_lds = false;  // 32x32 pivots: LDS kernel instead of the register-resident one
+  bool apply_lds = false;  // 32x32 pivots: LDS tile updates with W as columns instead of register ones with W as rows
   int jac_p = 32;          // block-Jacobi pivot size (64: 32-wide blocks, 32: 16-wide blocks)
   bool use_mfma = true;   // FP64 matrix-core GEMM (else FP64 VALU FMA)
   double* Sm = nullptr;
@@ -283,12 +295,13 @@ class Batch {
   std::vector<hipEvent_t> prof_ev_[3];
   int prof_used_[3] = {0, 0, 0};
   unsigned long long* prof_work_ = nullptr;   // [3]
-  void prof_begin(int kind);
-  void prof_end(int kind, const int* mask, int mstride);
+  void prof_begin(int kind, hipStream_t s = nullptr);
+  void prof_end(int kind, const int* mask, int mstride, hipStream_t s = nullptr);
   int prof_result(int kind, int* launches, double* total_ms, double* work);
   void prof_enable(int mask);
   std::vector<void*> allocs_;
   std::vector<MolState> h_state0_;   // initial SCC state, restored by every setup()
+  std::vector<MolState> state_stage_;
   std::vector<std::vector<double>> guess_qsh_, guess_dipm_, guess_qp_;
   bool setup_done_ = false;
   int iterations_ = 0;

@@ -572,6 +572,13 @@ void Batch::init_common() {
   if (const char* r = std::getenv("XTBAMD_PIVOT_KERNEL")) pivot_lds = std::strcmp(r, "lds") == 0;
   if (const char* r = std::getenv("XTBAMD_APPLY_KERNEL")) apply_lds = std::strcmp(r, "lds") == 0;
   if (const char* r = std::getenv("XTBAMD_BANDED")) banded_passes = std::max(0, std::atoi(r));
+  if (const char* r = std::getenv("XTBAMD_JAC_TOL")) jac_tol = std::atof(r);
+  trace_rounds = std::getenv("XTBAMD_TRACE_ROUNDS") != nullptr;
+  if (const char* r = std::getenv("XTBAMD_JAC_LANES")) two_lanes = std::atoi(r) != 1;
+  // default priority (as every stream here), no implicit ordering with the null stream
+  HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
   if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
   intcut = std::max(20.0, 25.0 - 10.0 * std::log10(opt.accuracy));
@@ -642,7 +649,8 @@ void Batch::allocate(const Capacity& c) {
   perm = alloc<int>(tao);
   Sm = alloc<double>(tmm);
   G = alloc<double>(tmm);
-  jac_nactive = alloc<int>(1);
+  jac_nactive = alloc<int>(2);
+  d_state_stage = alloc<MolState>(nmol);
   eeq_failed = alloc<int>(nmol);
   eeq_big = alloc<int>(nmol);
   eig_failed = alloc<int>(nmol);
@@ -851,17 +859,44 @@ void Batch::rebuild() {
     js.big_max_dim = std::max(js.big_max_dim, p.dim);
     qoff += (long long)(p.nbp / 2) * jac_p * jac_p;
     js.max_rounds = std::max(js.max_rounds, p.rounds);
-    const int np = p.nbp / 2;
-    for (int k = 0; k < np; ++k) pairs.push_back(make_int2(m, k));
-    for (int k = 0; k < np; ++k)
-      for (int l = k; l < np; ++l) ta.push_back(make_int4(m, k, l, 0));
-    const int nwb = (p.dim + jac_p - 1) / jac_p;
-    for (int rb = 0; rb < nwb; ++rb)
-      for (int l = 0; l < np; ++l) tw.push_back(make_int4(m, rb, l, 0));
-    const int nrb = (p.dim + 63) / 64;
-    for (int a = 0; a < nrb; ++a)
-      for (int b = a; b < nrb; ++b) to.push_back(make_int4(m, a, b, 0));
   }
+  // two lanes of equal tile-update bytes per round (dim^2): largest problems
+  // first, each to the lane with less so far
+  {
+    std::vector<int> order(big);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return js.probs[x].dim > js.probs[y].dim; });
+    double load[2] = {0.0, 0.0};
+    for (int m : order) {
+      const int g = (two_lanes && load[1] < load[0]) ? 1 : 0;
+      js.probs[m].group = g;
+      load[g] += (double)js.probs[m].dim * js.probs[m].dim;
+    }
+  }
+  for (int g = 0; g < 2; ++g) {
+    js.gpairs[g] = (int)pairs.size();
+    js.gta[g] = (int)ta.size();
+    js.gtw[g] = (int)tw.size();
+    js.gto[g] = (int)to.size();
+    for (int m : big) {
+      const JacobiProb& p = js.probs[m];
+      if (p.group != g) continue;
+      const int np = p.nbp / 2;
+      for (int k = 0; k < np; ++k) pairs.push_back(make_int2(m, k));
+      for (int k = 0; k < np; ++k)
+        for (int l = k; l < np; ++l) ta.push_back(make_int4(m, k, l, 0));
+      const int nwb = (p.dim + jac_p - 1) / jac_p;
+      for (int rb = 0; rb < nwb; ++rb)
+        for (int l = 0; l < np; ++l) tw.push_back(make_int4(m, rb, l, 0));
+      const int nrb = (p.dim + 63) / 64;
+      for (int a = 0; a < nrb; ++a)
+        for (int b = a; b < nrb; ++b) to.push_back(make_int4(m, a, b, 0));
+    }
+  }
+  js.gpairs[2] = (int)pairs.size();
+  js.gta[2] = (int)ta.size();
+  js.gtw[2] = (int)tw.size();
+  js.gto[2] = (int)to.size();
   if (qoff > js.qsize) throw std::runtime_error("Jacobi pivot buffer too small");
   HIPCHECK(hipMemcpyAsync(js.d_probs, js.probs.data(), nmol * sizeof(JacobiProb), hipMemcpyHostToDevice, stream));
   js.nsmall = (int)small.size();
@@ -966,6 +1001,9 @@ Batch::~Batch() {
     for (hipEvent_t e : v) (void)hipEventDestroy(e);
   for (void* p : allocs_) (void)hipFree(p);
   if (h_pinned) (void)hipHostFree(h_pinned);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (stream2) (void)hipStreamDestroy(stream2);
 }
 
 // ---------------------------------------------------------------------------
@@ -974,10 +1012,22 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   const bool rows = jac_p == 32 && !apply_lds && !keep_w && banded == 0;
   hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
                      check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0);
+  // two lanes: the problems of lane 1 run their rounds on stream2 beside
+  // those of lane 0; the problems of a lane never touch the other lane's
+  // data, so the results do not depend on the lanes
+  const bool lanes = rows && !pivot_lds && js.gpairs[1] < js.gpairs[2];
+  if (lanes) {
+    HIPCHECK(hipEventRecord(ev_fork, stream));
+    HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
+  }
   if (js.nsmall)
-    hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
-                       (const double*)js.scale, js.state, keep_w ? 1 : 0, rows ? 1 : 0);
-  if (js.npairs) {
+    hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, lanes ? stream2 : stream, js.d_probs, js.d_small, Am,
+                       Wm, (const double*)js.scale, js.state, keep_w ? 1 : 0, rows ? 1 : 0);
+  if (lanes) {
+    jacobi_lanes(js, Am, Wm);
+    HIPCHECK(hipEventRecord(ev_join, stream2));
+    HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
+  } else if (js.npairs) {
     // banded rounds first (neighbouring blocks only), then cyclic sweeps;
     // every round starts with the convergence test of the problems at a
     // sweep boundary; the host reads the number still running once per
@@ -991,7 +1041,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
       hipLaunchKernelGGL(k_jac_offmax, dim3(js.nto), dim3(256), 0, stream, js.d_probs, js.d_to, (const double*)Am,
                          (const int*)js.state, js.offmax, r);
       hipLaunchKernelGGL(k_jac_mark, dim3(1), dim3(256), 0, stream, js.d_probs, nb, (const double*)js.scale,
-                         js.state, js.offmax, r, jac_nactive);
+                         js.state, js.offmax, r, jac_nactive, jac_tol, -1);
       if (r < kBanded && r % js.max_rounds == 0) {
         HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIPCHECK(hipStreamSynchronize(stream));
@@ -1007,7 +1057,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
         else
           hipLaunchKernelGGL(k_jac_pairsR, dim3(js.npairs), dim3(64), 0, stream, js.d_probs, js.d_pairs, r,
                              (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
-                             inner_sweeps, jw);
+                             inner_sweeps, jw, jac_tol);
         prof_end(2, nullptr, 0);
         if (rows) {
           hipLaunchKernelGGL(k_jac_apply_aR, dim3(js.nta), dim3(64), 0, stream, js.d_probs, js.d_ta, r, Am,
@@ -1046,6 +1096,59 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
     hipLaunchKernelGGL(k_jac_sort, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm,
                        check_active ? (const MolState*)d_state : (const MolState*)nullptr, 1 << 30);
   return rows;
+}
+
+// block-Jacobi rounds of the two lanes (cyclic sweeps, register pivots and
+// tile updates).  Lane g's kernels go to its stream with its part of the work
+// lists, so the small kernels and launch gaps of one lane overlap the other
+// lane's work.  Each lane tests its problems at their sweep boundaries; once
+// per max_rounds rounds the host reads both lanes' counts of problems still
+// running and stops a lane that has none left.
+void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
+  const hipStream_t ls[2] = {stream, stream2};
+  bool live[2] = {js.gpairs[0] < js.gpairs[1], js.gpairs[1] < js.gpairs[2]};
+  unsigned long long* jw = (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr;
+  auto tests = [&](int g, int r) {
+    const int nto = js.gto[g + 1] - js.gto[g];
+    if (nto > 0)
+      hipLaunchKernelGGL(k_jac_offmax, dim3(nto), dim3(256), 0, ls[g], js.d_probs, js.d_to + js.gto[g],
+                         (const double*)Am, (const int*)js.state, js.offmax, r);
+    hipLaunchKernelGGL(k_jac_mark, dim3(1), dim3(256), 0, ls[g], js.d_probs, nmol, (const double*)js.scale, js.state,
+                       js.offmax, r, jac_nactive + g, jac_tol, g);
+  };
+  auto pivots = [&](int g, int r) {
+    prof_begin(2, ls[g]);
+    hipLaunchKernelGGL(k_jac_pairsR, dim3(js.gpairs[g + 1] - js.gpairs[g]), dim3(64), 0, ls[g], js.d_probs,
+                       js.d_pairs + js.gpairs[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
+                       (const int*)js.state, inner_sweeps, jw, jac_tol);
+    prof_end(2, nullptr, 0, ls[g]);
+  };
+  auto updates = [&](int g, int r) {
+    hipLaunchKernelGGL(k_jac_apply_aR, dim3(js.gta[g + 1] - js.gta[g]), dim3(64), 0, ls[g], js.d_probs,
+                       js.d_ta + js.gta[g], r, Am, (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+    hipLaunchKernelGGL(k_jac_apply_wR, dim3(js.gtw[g + 1] - js.gtw[g]), dim3(64), 0, ls[g], js.d_probs,
+                       js.d_tw + js.gtw[g], r, Wm, (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+  };
+  for (int g = 0; g < 2; ++g)
+    if (live[g]) tests(g, 0);
+  const int rmax = (kMaxSweeps + 1) * js.max_rounds;
+  for (int r = 0; r <= rmax; ++r) {
+    if (r % js.max_rounds == 0) {
+      HIPCHECK(hipEventRecord(ev_join, stream2));
+      HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
+      HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipStreamSynchronize(stream));
+      for (int g = 0; g < 2; ++g) live[g] = live[g] && h_pinned[g] > 0;
+      if (!live[0] && !live[1]) break;
+    }
+    for (int g = 0; g < 2; ++g) {
+      if (!live[g]) continue;
+      pivots(g, r);
+      updates(g, r);
+      tests(g, r + 1);
+    }
+    jacobi_rounds_ += 1;
+  }
 }
 
 void Batch::eigh_rows(JacSet& js, double* Am, double* Wm, double* evout) {
@@ -1154,7 +1257,7 @@ void Batch::prof_enable(int mask) {
   HIPCHECK(hipStreamSynchronize(stream));
 }
 
-void Batch::prof_begin(int kind) {
+void Batch::prof_begin(int kind, hipStream_t s) {
   if (!(prof_mask & (1 << kind))) return;
   std::vector<hipEvent_t>& ev = prof_ev_[kind];
   if (prof_used_[kind] + 2 > (int)ev.size()) {
@@ -1162,12 +1265,12 @@ void Batch::prof_begin(int kind) {
     ev.resize(old + 1024);
     for (size_t i = old; i < ev.size(); ++i) HIPCHECK(hipEventCreate(&ev[i]));
   }
-  HIPCHECK(hipEventRecord(ev[prof_used_[kind]], stream));
+  HIPCHECK(hipEventRecord(ev[prof_used_[kind]], s ? s : stream));
 }
 
-void Batch::prof_end(int kind, const int* mask, int mstride) {
+void Batch::prof_end(int kind, const int* mask, int mstride, hipStream_t s) {
   if (!(prof_mask & (1 << kind))) return;
-  HIPCHECK(hipEventRecord(prof_ev_[kind][prof_used_[kind] + 1], stream));
+  HIPCHECK(hipEventRecord(prof_ev_[kind][prof_used_[kind] + 1], s ? s : stream));
   prof_used_[kind] += 2;
   if (kind == 1)
     hipLaunchKernelGGL(k_count_gemm, dim3(grid1(nmol, 256)), dim3(256), 0, stream, d_desc, nmol, mask, mstride,
@@ -1248,9 +1351,14 @@ void Batch::setup() {
   if (nsub == 0) return;
   const Element* el = elements_dev;
   const int ns = nsub;
-  // every single point starts from the initial SCC state (iteration 0, active)
-  for (int m : sel_)
-    HIPCHECK(hipMemcpyAsync(d_state + m, &h_state0_[m], sizeof(MolState), hipMemcpyHostToDevice, stream));
+  // every single point starts from the initial SCC state (iteration 0,
+  // active): one upload of the selected molecules' states, scattered to
+  // their slots on the device
+  state_stage_.resize(ns);
+  for (int k = 0; k < ns; ++k) state_stage_[k] = h_state0_[sel_[k]];
+  HIPCHECK(hipMemcpyAsync(d_state_stage, state_stage_.data(), ns * sizeof(MolState), hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_state_scatter, dim3(grid1(ns, 64)), dim3(64), 0, stream, (const MolDesc*)d_sub,
+                     (const MolState*)d_state_stage, d_state, ns);
   HIPCHECK(hipStreamSynchronize(stream));
   iterations_ = 0;
   jacobi_rounds_ = 0;
@@ -1338,6 +1446,7 @@ void Batch::setup() {
 }
 
 bool Batch::scc_iteration() {
+  const long long rounds_before = jacobi_rounds_;
   const Element* el = elements_dev;
   const double thr2 = neglect * 1.0e-2;
   hipLaunchKernelGGL(k_shell_pot, dim3(grid1(max_nsh, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)J,
@@ -1404,6 +1513,8 @@ bool Batch::scc_iteration() {
                      sv_gshift, sh_at, qsh, q, dipm, qp, q_in, dq, qlast, dqlast, bdf, bu, ba, bomega, scfconv, qconv);
   HIPCHECK(hipGetLastError());
   iterations_ += 1;
+  if (trace_rounds)
+    std::fprintf(stderr, "scc iteration %d: %lld Jacobi rounds\n", iterations_, jacobi_rounds_ - rounds_before);
   // any molecule still active?
   hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, stream, (const MolState*)d_state, nmol, jac_nactive);
   HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));

@@ -1024,17 +1024,22 @@ template __global__ void k_jac_apply_wP<32>(const JacobiProb*, const int4*, int,
 // The 32 indices sit in 16 slot pairs k = 0..15 (a top and a bottom slot;
 // initially index k and index 16 + k).  Lane L holds column pair c = L & 15 of
 // the row pairs 4g..4g+3 (g = L >> 4) of A, 2 x 2 doubles per pair block, and
-// the column pair c of Q for rows g, g+4, ..., g+28.  An inner round rotates
-// every pair (top, bottom) at once, the lane owning the diagonal block of a
-// pair computing its rotation, then moves the slots by the circle method: the
-// top of pair 0 stays, the bottom of pair 0 becomes the top of pair 1, tops
-// move one pair up, the top of pair 15 becomes its bottom and bottoms move one
-// pair down.  31 rounds meet every pair of indices once and restore the slot
-// order, so Q comes out with its columns in index order.  Row moves stay in a
-// lane except across the 16-lane groups (one shuffle each way); column moves
-// are DPP shifts within a 16-lane row.  Only the rotation cosines and sines go
-// through LDS.  The rotation is the one of the LDS kernels (Rutishauser's
-// formulas, exact update of the pivot's diagonal pair).
+// the same row pairs of Q^T (rows are slots, columns the indices c and
+// c + 16).  An inner round rotates every pair (top, bottom) at once, the lane
+// owning the diagonal block of a pair computing its rotation, then moves the
+// slots by the circle method: the top of pair 0 stays, the bottom of pair 0
+// becomes the top of pair 1, tops move one pair up, the top of pair 15
+// becomes its bottom and bottoms move one pair down.  31 rounds meet every
+// pair of indices once and restore the slot order, so Q comes out with its
+// columns in index order.  Column moves of A are DPP shifts within a 16-lane
+// row.  Row moves of A and Q^T stay in a lane except across the 16-lane
+// groups (one shuffle each way); the in-lane ones are register renamings:
+// at phase ph = round mod 4 the top row of pair 4g + i sits in T[(i - ph) & 3]
+// and its bottom row in B[(i + ph) & 3], so rounds are unrolled by four and
+// only the moves across groups and the fixed pair 0 / pair 15 rows cost
+// instructions.  Only the rotation cosines and sines go through LDS.  The
+// rotation is the one of the LDS kernels (Rutishauser's formulas, exact
+// update of the pivot's diagonal pair).
 
 // src moved one lane within each row of 16: CTRL 0x111 reads lane - 1
 // (row_shr:1), 0x101 reads lane + 1 (row_shl:1); a lane without a source
@@ -1059,10 +1064,97 @@ __device__ __forceinline__ void circle_cols(double& top, double& bot, int c) {
   bot = nb;
 }
 
+// circle-method row move of one lane's rows at phase PH (top row of pair
+// 4g + i in T[(i - PH) & 3], bottom row in B[(i + PH) & 3]), leaving phase
+// PH + 1: the new top of pair 4g comes from the previous group (pair 0: its
+// own top), the new top of pair 4g + 1 of group 0 is the old bottom of pair 0,
+// the new bottom of pair 4g + 3 comes from the next group (pair 15: its own
+// top); every other row keeps its register
+template <int PH>
+__device__ __forceinline__ void circle_rows(double (&T)[4][2], double (&B)[4][2], int lane, int g) {
+  const int up = (lane + 48) & 63, dn = (lane + 16) & 63;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const double t0 = T[(0 - PH) & 3][h], t3 = T[(3 - PH) & 3][h], b0 = B[PH & 3][h];
+    const double x = __shfl(t3, up, 64), y = __shfl(b0, dn, 64);
+    T[(3 - PH) & 3][h] = (g == 0) ? t0 : x;
+    T[(0 - PH) & 3][h] = (g == 0) ? b0 : t0;
+    B[PH & 3][h] = (g == 3) ? t3 : y;
+  }
+}
+
+template <int PH>
+__device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2], double (&QT)[4][2],
+                                           double (&QB)[4][2], double2* rcs, int lane, int g, int c, int id, bool diag,
+                                           double tol, int& total) {
+  double app = 0.0, apq = 0.0, aqq = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (id == i) {
+      app = T[(i - PH) & 3][0];
+      apq = T[(i - PH) & 3][1];
+      aqq = B[(i + PH) & 3][1];
+    }
+  double cr = 1.0, sr = 0.0, tr = 0.0;
+  bool rot = false;
+  if (diag && fabs(apq) > tol) {
+    const double theta = (aqq - app) / (2.0 * apq);
+    tr = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+    cr = 1.0 / sqrt(tr * tr + 1.0);
+    sr = tr * cr;
+    rot = true;
+  }
+  if (__ballot(rot)) {
+    total += 1;
+    if (diag) rcs[c] = make_double2(cr, sr);
+    __syncthreads();
+    const double2 col = rcs[c];
+    double2 row[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) row[i] = rcs[4 * g + i];
+    __syncthreads();
+    const double c2 = col.x, s2 = col.y;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double* t = T[(i - PH) & 3];
+      double* b = B[(i + PH) & 3];
+      const double c1 = row[i].x, s1 = row[i].y;
+      const double tt = t[0], tb = t[1], bt = b[0], bb = b[1];
+      const double btt = c2 * tt - s2 * tb, btb = s2 * tt + c2 * tb;
+      const double bbt = c2 * bt - s2 * bb, bbb = s2 * bt + c2 * bb;
+      t[0] = c1 * btt - s1 * bbt;
+      b[0] = s1 * btt + c1 * bbt;
+      t[1] = c1 * btb - s1 * bbb;
+      b[1] = s1 * btb + c1 * bbb;
+      if (rot && i == id) {
+        t[0] = app - tr * apq;
+        b[1] = aqq + tr * apq;
+        t[1] = 0.0;
+        b[0] = 0.0;
+      }
+      double* qt = QT[(i - PH) & 3];
+      double* qb = QB[(i + PH) & 3];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double x = qt[h], y = qb[h];
+        qt[h] = c1 * x - s1 * y;
+        qb[h] = s1 * x + c1 * y;
+      }
+    }
+  }
+  circle_rows<PH>(T, B, lane, g);
+  circle_rows<PH>(QT, QB, lane, g);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    circle_cols(T[i][0], T[i][1], c);
+    circle_cols(B[i][0], B[i][1], c);
+  }
+}
+
 __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
                                                    int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
                                                    const double* __restrict__ scale_in, const int* __restrict__ state,
-                                                   int inner_sweeps, unsigned long long* work) {
+                                                   int inner_sweeps, unsigned long long* work, double tolrel) {
   __shared__ double2 rcs[16];
   __shared__ double red[2];
   const int2 it = list[blockIdx.x];
@@ -1073,124 +1165,79 @@ __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict_
   jac_pair(pr.nbp, pr.rounds, r, it.y, bp, bq);
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
   const double* Ap = A + pr.a_off;
-  double a[4][4];   // [row pair 4g+i][2 * row half + column half]
+  double T[4][2], B[4][2];   // rows (top, bottom) of pair 4g + i, columns (top, bottom) of pair c
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int hr = 0; hr < 2; ++hr)
+    for (int hc = 0; hc < 2; ++hc) {
+      const int gt = 16 * bp + 4 * g + i, gb = 16 * bq + 4 * g + i, gv = (hc ? 16 * bq : 16 * bp) + c;
+      T[i][hc] = (gt < n && gv < n) ? Ap[(long long)gt * n + gv] : 0.0;
+      B[i][hc] = (gb < n && gv < n) ? Ap[(long long)gb * n + gv] : 0.0;
+    }
+  double QT[4][2], QB[4][2];   // Q^T rows (top, bottom slot of pair 4g + i), columns (c, c + 16)
 #pragma unroll
-      for (int hc = 0; hc < 2; ++hc) {
-        const int gu = (hr ? 16 * bq : 16 * bp) + 4 * g + i, gv = (hc ? 16 * bq : 16 * bp) + c;
-        a[i][2 * hr + hc] = (gu < n && gv < n) ? Ap[(long long)gu * n + gv] : 0.0;
-      }
-  double qv[8][2];  // Q[g + 4j][slot (c, half)]
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int u = g + 4 * j;
-    qv[j][0] = (u == c) ? 1.0 : 0.0;
-    qv[j][1] = (u == c + 16) ? 1.0 : 0.0;
+  for (int i = 0; i < 4; ++i) {
+    const double d = (4 * g + i == c) ? 1.0 : 0.0;
+    QT[i][0] = d;
+    QT[i][1] = 0.0;
+    QB[i][0] = 0.0;
+    QB[i][1] = d;
   }
-  const double tol = 1e-12 * scale_in[it.x];
+  const double tol = tolrel * scale_in[it.x];
   const bool diag = (c >> 2) == g;   // this lane owns the diagonal block of pair c
   const int id = c & 3;
   int total = 0;
   for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
     double m = 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const bool on_diag = diag && i == id && (e == 0 || e == 3);
-        if (!on_diag) m = fmax(m, fabs(a[i][e]));
-      }
+    for (int i = 0; i < 4; ++i) {
+      const bool on = diag && i == id;
+      if (!on) m = fmax(m, fabs(T[i][0]));
+      m = fmax(m, fabs(T[i][1]));
+      m = fmax(m, fabs(B[i][0]));
+      if (!on) m = fmax(m, fabs(B[i][1]));
+    }
     m = block_max<64>(m, red);
     if (m <= tol) break;
-    for (int rr = 0; rr < 31; ++rr) {
-      double app = a[0][0], apq = a[0][1], aqq = a[0][3];
-#pragma unroll
-      for (int i = 1; i < 4; ++i)
-        if (id == i) { app = a[i][0]; apq = a[i][1]; aqq = a[i][3]; }
-      double cr = 1.0, sr = 0.0, tr = 0.0;
-      bool rot = false;
-      if (diag && fabs(apq) > tol) {
-        const double theta = (aqq - app) / (2.0 * apq);
-        tr = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        cr = 1.0 / sqrt(tr * tr + 1.0);
-        sr = tr * cr;
-        rot = true;
-      }
-      const unsigned long long any = __ballot(rot);
-      if (any) {
-        total += 1;
-        if (diag) rcs[c] = make_double2(cr, sr);
-        __syncthreads();
-        const double2 col = rcs[c];
-        double2 row[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) row[i] = rcs[4 * g + i];
-        __syncthreads();
-        const double c2 = col.x, s2 = col.y;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const double c1 = row[i].x, s1 = row[i].y;
-          const double tt = a[i][0], tb = a[i][1], bt = a[i][2], bb = a[i][3];
-          const double btt = c2 * tt - s2 * tb, btb = s2 * tt + c2 * tb;
-          const double bbt = c2 * bt - s2 * bb, bbb = s2 * bt + c2 * bb;
-          a[i][0] = c1 * btt - s1 * bbt;
-          a[i][2] = s1 * btt + c1 * bbt;
-          a[i][1] = c1 * btb - s1 * bbb;
-          a[i][3] = s1 * btb + c1 * bbb;
-          if (rot && i == id) {
-            a[i][0] = app - tr * apq;
-            a[i][3] = aqq + tr * apq;
-            a[i][1] = 0.0;
-            a[i][2] = 0.0;
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const double qt = qv[j][0], qb = qv[j][1];
-          qv[j][0] = c2 * qt - s2 * qb;
-          qv[j][1] = s2 * qt + c2 * qb;
-        }
-      }
-      // slot moves along the rows: in-lane, plus one pair across the groups
-      const double x0 = __shfl(a[3][0], (lane + 48) & 63, 64), x1 = __shfl(a[3][1], (lane + 48) & 63, 64);
-      const double y0 = __shfl(a[0][2], (lane + 16) & 63, 64), y1 = __shfl(a[0][3], (lane + 16) & 63, 64);
-      {
-        const double t0 = a[0][0], t1 = a[0][1], u0 = a[0][2], u1 = a[0][3];
-        const double v0 = a[3][0], v1 = a[3][1];
-        // tops: pair 4g+i takes the top of pair 4g+i-1 (pair 1 the bottom of 0, pair 0 stays)
-        a[3][0] = a[2][0]; a[3][1] = a[2][1];
-        a[2][0] = a[1][0]; a[2][1] = a[1][1];
-        a[1][0] = (g == 0) ? u0 : t0;
-        a[1][1] = (g == 0) ? u1 : t1;
-        a[0][0] = (g == 0) ? t0 : x0;
-        a[0][1] = (g == 0) ? t1 : x1;
-        // bottoms: pair 4g+i takes the bottom of pair 4g+i+1 (pair 15 its own top)
-        a[0][2] = a[1][2]; a[0][3] = a[1][3];
-        a[1][2] = a[2][2]; a[1][3] = a[2][3];
-        a[2][2] = a[3][2]; a[2][3] = a[3][3];
-        a[3][2] = (g == 3) ? v0 : y0;
-        a[3][3] = (g == 3) ? v1 : y1;
-      }
-      // slot moves along the columns
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        circle_cols(a[i][0], a[i][1], c);
-        circle_cols(a[i][2], a[i][3], c);
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) circle_cols(qv[j][0], qv[j][1], c);
+    for (int rr = 0; rr < 28; rr += 4) {
+      jac_roundR<0>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+      jac_roundR<1>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+      jac_roundR<2>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+      jac_roundR<3>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
     }
+    jac_roundR<0>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+    jac_roundR<1>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+    jac_roundR<2>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+    // phase 3 -> 0: top row i from T[(i + 1) & 3], bottom row i from B[(i + 3) & 3]
+    double t0[4][2], b0[4][2], qt0[4][2], qb0[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        t0[i][h] = T[(i + 1) & 3][h];
+        b0[i][h] = B[(i + 3) & 3][h];
+        qt0[i][h] = QT[(i + 1) & 3][h];
+        qb0[i][h] = QB[(i + 3) & 3][h];
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        T[i][h] = t0[i][h];
+        B[i][h] = b0[i][h];
+        QT[i][h] = qt0[i][h];
+        QB[i][h] = qb0[i][h];
+      }
   }
   double* Qp = Qbuf + pr.q_off + (long long)it.y * 32 * 32;
   if (total > 0)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int u = g + 4 * j;
-      Qp[u * 32 + c] = qv[j][0];
-      Qp[u * 32 + 16 + c] = qv[j][1];
+    for (int i = 0; i < 4; ++i) {
+      const int k = 4 * g + i;
+      Qp[c * 32 + k] = QT[i][0];
+      Qp[(c + 16) * 32 + k] = QT[i][1];
+      Qp[c * 32 + 16 + k] = QB[i][0];
+      Qp[(c + 16) * 32 + 16 + k] = QB[i][1];
     }
   if (threadIdx.x == 0) {
     rotflag[(int)(pr.q_off / (32 * 32)) + it.y] = total > 0 ? 1 : 0;
@@ -1391,7 +1438,7 @@ __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__
 // start a sweep at global round r (r a multiple of the problem's rounds per
 // sweep), one workgroup per upper-triangular 64 x 64 tile; max over
 // non-negative doubles is taken on their bit patterns, which order the same
-// way (order independent)
+// way (order independent).
 __global__ void __launch_bounds__(256) k_jac_offmax(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
                                                     const double* __restrict__ A, const int* __restrict__ state,
                                                     unsigned long long* offmax, int r) {
@@ -1413,22 +1460,25 @@ __global__ void __launch_bounds__(256) k_jac_offmax(const JacobiProb* __restrict
 // convergence test of the problems at a sweep boundary (per problem, so a
 // problem's rounds never depend on the other problems of the batch): done
 // when every off-diagonal element is <= tol, failed after kMaxSweeps sweeps;
-// counts the problems still running into nactive
+// counts the problems still running into nactive.  group >= 0 restricts the
+// test to the problems of that round lane.
 __global__ void __launch_bounds__(256) k_jac_mark(const JacobiProb* __restrict__ probs, int nprob,
                                                   const double* __restrict__ scale, int* state,
-                                                  unsigned long long* offmax, int r, int* nactive) {
+                                                  unsigned long long* offmax, int r, int* nactive, double tolrel,
+                                                  int group) {
   __shared__ double sh[4];
   double c = 0.0;
   for (int p = threadIdx.x; p < nprob; p += blockDim.x) {
-    int* st = state + p * kJacState;
-    if (st[0]) continue;
     const JacobiProb pr = probs[p];
     if (pr.dim <= 64) continue;   // solved by k_jac_small
+    if (group >= 0 && pr.group != group) continue;
+    int* st = state + p * kJacState;
+    if (st[0]) continue;
     if (jac_boundary(pr.rounds, r)) {
       const int sweep = r < kBanded ? r / pr.rounds : 0;
       const double m = __longlong_as_double((long long)offmax[p]);
       offmax[p] = 0ull;
-      if (m <= 1e-12 * scale[p]) {
+      if (m <= tolrel * scale[p]) {
         st[0] = 1;
         st[5] = sweep;
         continue;
@@ -1781,6 +1831,13 @@ __global__ void k_jac_sort(const JacobiProb* __restrict__ probs, const double* _
 }  // namespace xtbamd
 
 namespace xtbamd {
+// st[desc[k].mol] <- stage[k]: initial SCC states of a sub-batch, uploaded in one copy
+__global__ void k_state_scatter(const MolDesc* __restrict__ desc, const MolState* __restrict__ stage, MolState* st,
+                                int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) st[desc[k].mol] = stage[k];
+}
+
 __global__ void k_count_active(const MolState* __restrict__ st, int nmol, int* out) {
   __shared__ double sh[4];
   double c = 0.0;
