@@ -1038,8 +1038,11 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
     for (int r = 0; r <= rmax; ++r) sched.push_back(r);
     for (size_t si = 0; si < sched.size(); ++si) {
       const int r = sched[si];
-      hipLaunchKernelGGL(k_jac_offmax, dim3(js.nto), dim3(256), 0, stream, js.d_probs, js.d_to, (const double*)Am,
-                         (const int*)js.state, js.offmax, r);
+      // the register tile updates measure the off-diagonal maximum themselves
+      // in the last round of each sweep
+      if (!rows || r == 0)
+        hipLaunchKernelGGL(k_jac_offmax, dim3(js.nto), dim3(256), 0, stream, js.d_probs, js.d_to, (const double*)Am,
+                           (const int*)js.state, js.offmax, r);
       hipLaunchKernelGGL(k_jac_mark, dim3(1), dim3(256), 0, stream, js.d_probs, nb, (const double*)js.scale,
                          js.state, js.offmax, r, jac_nactive, jac_tol, -1);
       if (r < kBanded && r % js.max_rounds == 0) {
@@ -1061,7 +1064,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
         prof_end(2, nullptr, 0);
         if (rows) {
           hipLaunchKernelGGL(k_jac_apply_aR, dim3(js.nta), dim3(64), 0, stream, js.d_probs, js.d_ta, r, Am,
-                             (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+                             (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state, js.offmax);
           hipLaunchKernelGGL(k_jac_apply_wR, dim3(js.ntw), dim3(64), 0, stream, js.d_probs, js.d_tw, r, Wm,
                              (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
         } else {
@@ -1108,9 +1111,11 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   const hipStream_t ls[2] = {stream, stream2};
   bool live[2] = {js.gpairs[0] < js.gpairs[1], js.gpairs[1] < js.gpairs[2]};
   unsigned long long* jw = (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr;
+  // the sweep boundary tests: the off-diagonal maxima come from the tile
+  // updates of the round before, except at round 0
   auto tests = [&](int g, int r) {
     const int nto = js.gto[g + 1] - js.gto[g];
-    if (nto > 0)
+    if (r == 0 && nto > 0)
       hipLaunchKernelGGL(k_jac_offmax, dim3(nto), dim3(256), 0, ls[g], js.d_probs, js.d_to + js.gto[g],
                          (const double*)Am, (const int*)js.state, js.offmax, r);
     hipLaunchKernelGGL(k_jac_mark, dim3(1), dim3(256), 0, ls[g], js.d_probs, nmol, (const double*)js.scale, js.state,
@@ -1125,7 +1130,8 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   };
   auto updates = [&](int g, int r) {
     hipLaunchKernelGGL(k_jac_apply_aR, dim3(js.gta[g + 1] - js.gta[g]), dim3(64), 0, ls[g], js.d_probs,
-                       js.d_ta + js.gta[g], r, Am, (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
+                       js.d_ta + js.gta[g], r, Am, (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state,
+                       js.offmax);
     hipLaunchKernelGGL(k_jac_apply_wR, dim3(js.gtw[g + 1] - js.gtw[g]), dim3(64), 0, ls[g], js.d_probs,
                        js.d_tw + js.gtw[g], r, Wm, (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
   };

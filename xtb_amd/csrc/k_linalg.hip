@@ -1258,17 +1258,23 @@ __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict_
 // global index of position u (0..31) of pivot pair (bp, bq)
 __device__ __forceinline__ int pair_index(int bp, int bq, int u) { return u < 16 ? 16 * bp + u : 16 * bq + u - 16; }
 
-// A[I_k, I_l] <- Q_k^T A[I_k, I_l] Q_l for pair tiles k <= l (A symmetric)
+// A[I_k, I_l] <- Q_k^T A[I_k, I_l] Q_l for pair tiles k <= l (A symmetric).
+// In the last round of a problem's sweep every tile also folds the largest
+// off-diagonal magnitude it leaves (strict upper triangle; the tiles k <= l
+// hold every index pair once) into offmax, the convergence test of the next
+// sweep boundary (order independent: max on the bit patterns).
 __global__ void __launch_bounds__(64) k_jac_apply_aR(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
                                                      int r, double* A, const double* __restrict__ Qbuf,
-                                                     const int* __restrict__ rotflag, const int* __restrict__ state) {
+                                                     const int* __restrict__ rotflag, const int* __restrict__ state,
+                                                     unsigned long long* offmax) {
   const int4 it = list[blockIdx.x];
   const JacobiProb pr = probs[it.x];
   if (state[it.x * kJacState]) return;
   const int k = it.y, l = it.z;
   const int qb = (int)(pr.q_off / (32 * 32));
   const int rk = rotflag[qb + k], rl = rotflag[qb + l];
-  if (!rk && !rl) return;
+  const bool measure = offmax && r < kBanded && (r + 1) % pr.rounds == 0;
+  if (!rk && !rl && !measure) return;
   const int n = pr.dim;
   int kp, kq, lp, lq;
   jac_pair(pr.nbp, pr.rounds, r, k, kp, kq);
@@ -1339,18 +1345,34 @@ __global__ void __launch_bounds__(64) k_jac_apply_aR(const JacobiProb* __restric
   }
   // store the tile and its mirror; a diagonal tile stores its upper triangle
   // to both places, so A stays exactly symmetric
+  if (rk || rl)
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int u = 16 * a + k4 + 4 * i, v = 16 * b + i16;
-        const int gu = pair_index(kp, kq, u), gv = pair_index(lp, lq, v);
-        if (gu >= n || gv >= n || (k == l && u > v)) continue;
-        Ap[(long long)gu * n + gv] = o[a][b][i];
-        Ap[(long long)gv * n + gu] = o[a][b][i];
-      }
+        for (int i = 0; i < 4; ++i) {
+          const int u = 16 * a + k4 + 4 * i, v = 16 * b + i16;
+          const int gu = pair_index(kp, kq, u), gv = pair_index(lp, lq, v);
+          if (gu >= n || gv >= n || (k == l && u > v)) continue;
+          Ap[(long long)gu * n + gv] = o[a][b][i];
+          Ap[(long long)gv * n + gu] = o[a][b][i];
+        }
+  if (measure) {
+    double m = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int u = 16 * a + k4 + 4 * i, v = 16 * b + i16;
+          const int gu = pair_index(kp, kq, u), gv = pair_index(lp, lq, v);
+          if (gu < n && gv < n && (k < l || u < v)) m = fmax(m, fabs(o[a][b][i]));
+        }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (lane == 0 && m > 0.0) atomicMax(&offmax[it.x], (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 // W rows I_l, columns of row block rb: Wr[I_l, ...] <- Q_l^T Wr[I_l, ...]
