@@ -178,6 +178,7 @@ class Batch {
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool two_lanes = true;
+  bool lanes_one_stream = false;   // both lanes on stream, interleaved
   int nmol = 0, max_n = 0, max_nsh = 0, max_nao = 0, max_nd4 = 0, max_nbr = 0;
   double intcut = 25.0, neglect = 1e-8, scfconv = 1e-6, qconv = 1e-4;
   std::vector<MolDesc> desc;

@@ -574,7 +574,10 @@ void Batch::init_common() {
   if (const char* r = std::getenv("XTBAMD_BANDED")) banded_passes = std::max(0, std::atoi(r));
   if (const char* r = std::getenv("XTBAMD_JAC_TOL")) jac_tol = std::atof(r);
   trace_rounds = std::getenv("XTBAMD_TRACE_ROUNDS") != nullptr;
-  if (const char* r = std::getenv("XTBAMD_JAC_LANES")) two_lanes = std::atoi(r) != 1;
+  if (const char* r = std::getenv("XTBAMD_JAC_LANES")) {
+    two_lanes = std::atoi(r) != 1;
+    lanes_one_stream = std::atoi(r) == 3;
+  }
   // default priority (as every stream here), no implicit ordering with the null stream
   HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
   HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
@@ -669,7 +672,7 @@ void Batch::allocate(const Capacity& c) {
   JacSet& js = jac_n;
   js.probs.assign(nmol, JacobiProb());
   js.d_probs = alloc<JacobiProb>(nmol);
-  js.offmax = alloc<unsigned long long>(nmol);
+  js.offmax = alloc<unsigned long long>(2LL * nmol);   // two sweep-parity buffers
   js.state = alloc<int>((long long)nmol * kJacState);
   js.scale = alloc<double>(nmol);
   // pivot-factor buffer: at most (nao/64 + 1) pairs per molecule
@@ -1015,7 +1018,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   // two lanes: the problems of lane 1 run their rounds on stream2 beside
   // those of lane 0; the problems of a lane never touch the other lane's
   // data, so the results do not depend on the lanes
-  const bool lanes = rows && !pivot_lds && js.gpairs[1] < js.gpairs[2];
+  const bool lanes = rows && !pivot_lds && js.npairs > 0;
   if (lanes) {
     HIPCHECK(hipEventRecord(ev_fork, stream));
     HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
@@ -1038,11 +1041,8 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
     for (int r = 0; r <= rmax; ++r) sched.push_back(r);
     for (size_t si = 0; si < sched.size(); ++si) {
       const int r = sched[si];
-      // the register tile updates measure the off-diagonal maximum themselves
-      // in the last round of each sweep
-      if (!rows || r == 0)
-        hipLaunchKernelGGL(k_jac_offmax, dim3(js.nto), dim3(256), 0, stream, js.d_probs, js.d_to, (const double*)Am,
-                           (const int*)js.state, js.offmax, r);
+      hipLaunchKernelGGL(k_jac_offmax, dim3(js.nto), dim3(256), 0, stream, js.d_probs, js.d_to, (const double*)Am,
+                         (const int*)js.state, js.offmax, r);
       hipLaunchKernelGGL(k_jac_mark, dim3(1), dim3(256), 0, stream, js.d_probs, nb, (const double*)js.scale,
                          js.state, js.offmax, r, jac_nactive, jac_tol, -1);
       if (r < kBanded && r % js.max_rounds == 0) {
@@ -1059,12 +1059,12 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
                              inner_sweeps, jw);
         else
           hipLaunchKernelGGL(k_jac_pairsR, dim3(js.npairs), dim3(64), 0, stream, js.d_probs, js.d_pairs, r,
-                             (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
-                             inner_sweeps, jw, jac_tol);
+                             (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, js.state, inner_sweeps, jw,
+                             jac_tol, (unsigned long long*)nullptr, 0);
         prof_end(2, nullptr, 0);
         if (rows) {
           hipLaunchKernelGGL(k_jac_apply_aR, dim3(js.nta), dim3(64), 0, stream, js.d_probs, js.d_ta, r, Am,
-                             (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state, js.offmax);
+                             (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
           hipLaunchKernelGGL(k_jac_apply_wR, dim3(js.ntw), dim3(64), 0, stream, js.d_probs, js.d_tw, r, Wm,
                              (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
         } else {
@@ -1103,43 +1103,38 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
 
 // block-Jacobi rounds of the two lanes (cyclic sweeps, register pivots and
 // tile updates).  Lane g's kernels go to its stream with its part of the work
-// lists, so the small kernels and launch gaps of one lane overlap the other
-// lane's work.  Each lane tests its problems at their sweep boundaries; once
-// per max_rounds rounds the host reads both lanes' counts of problems still
-// running and stops a lane that has none left.
+// lists, so the launch gaps of one lane overlap the other lane's work.  A
+// round is two launches per lane: the pivots (which first test the problems
+// at a sweep boundary against the off-diagonal maximum the tile updates
+// measured at the end of the sweep before) and the A and W tile updates.
+// Once per max_rounds rounds, after the pivots, the host reads both lanes'
+// counts of problems still running and stops a lane that has none left.
 void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
-  const hipStream_t ls[2] = {stream, stream2};
+  const hipStream_t ls[2] = {stream, lanes_one_stream ? stream : stream2};
   bool live[2] = {js.gpairs[0] < js.gpairs[1], js.gpairs[1] < js.gpairs[2]};
   unsigned long long* jw = (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr;
-  // the sweep boundary tests: the off-diagonal maxima come from the tile
-  // updates of the round before, except at round 0
-  auto tests = [&](int g, int r) {
+  // the initial off-diagonal maxima go to the buffer the round-0 test reads
+  for (int g = 0; g < 2; ++g) {
     const int nto = js.gto[g + 1] - js.gto[g];
-    if (r == 0 && nto > 0)
+    if (live[g] && nto > 0)
       hipLaunchKernelGGL(k_jac_offmax, dim3(nto), dim3(256), 0, ls[g], js.d_probs, js.d_to + js.gto[g],
-                         (const double*)Am, (const int*)js.state, js.offmax, r);
-    hipLaunchKernelGGL(k_jac_mark, dim3(1), dim3(256), 0, ls[g], js.d_probs, nmol, (const double*)js.scale, js.state,
-                       js.offmax, r, jac_nactive + g, jac_tol, g);
-  };
-  auto pivots = [&](int g, int r) {
-    prof_begin(2, ls[g]);
-    hipLaunchKernelGGL(k_jac_pairsR, dim3(js.gpairs[g + 1] - js.gpairs[g]), dim3(64), 0, ls[g], js.d_probs,
-                       js.d_pairs + js.gpairs[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
-                       (const int*)js.state, inner_sweeps, jw, jac_tol);
-    prof_end(2, nullptr, 0, ls[g]);
-  };
-  auto updates = [&](int g, int r) {
-    hipLaunchKernelGGL(k_jac_apply_aR, dim3(js.gta[g + 1] - js.gta[g]), dim3(64), 0, ls[g], js.d_probs,
-                       js.d_ta + js.gta[g], r, Am, (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state,
-                       js.offmax);
-    hipLaunchKernelGGL(k_jac_apply_wR, dim3(js.gtw[g + 1] - js.gtw[g]), dim3(64), 0, ls[g], js.d_probs,
-                       js.d_tw + js.gtw[g], r, Wm, (const double*)js.qbuf, (const int*)js.rot, (const int*)js.state);
-  };
-  for (int g = 0; g < 2; ++g)
-    if (live[g]) tests(g, 0);
+                         (const double*)Am, (const int*)js.state, js.offmax + nmol, 0);
+  }
   const int rmax = (kMaxSweeps + 1) * js.max_rounds;
   for (int r = 0; r <= rmax; ++r) {
+    for (int g = 0; g < 2; ++g) {
+      if (!live[g]) continue;
+      prof_begin(2, ls[g]);
+      hipLaunchKernelGGL(k_jac_pairsR, dim3(js.gpairs[g + 1] - js.gpairs[g]), dim3(64), 0, ls[g], js.d_probs,
+                         js.d_pairs + js.gpairs[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
+                         js.state, inner_sweeps, jw, jac_tol, js.offmax, nmol);
+      prof_end(2, nullptr, 0, ls[g]);
+    }
     if (r % js.max_rounds == 0) {
+      for (int g = 0; g < 2; ++g)
+        if (live[g])
+          hipLaunchKernelGGL(k_jac_count, dim3(1), dim3(256), 0, ls[g], js.d_probs, nmol, (const int*)js.state, g,
+                             jac_nactive + g);
       HIPCHECK(hipEventRecord(ev_join, stream2));
       HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
       HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -1149,9 +1144,10 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
     }
     for (int g = 0; g < 2; ++g) {
       if (!live[g]) continue;
-      pivots(g, r);
-      updates(g, r);
-      tests(g, r + 1);
+      const int nta = js.gta[g + 1] - js.gta[g], ntw = js.gtw[g + 1] - js.gtw[g];
+      hipLaunchKernelGGL(k_jac_apply_R, dim3(nta + ntw), dim3(64), 0, ls[g], js.d_probs, js.d_ta + js.gta[g], nta,
+                         js.d_tw + js.gtw[g], r, Am, Wm, (const double*)js.qbuf, (const int*)js.rot,
+                         (const int*)js.state, js.offmax, nmol);
     }
     jacobi_rounds_ += 1;
   }

@@ -1151,15 +1151,39 @@ __device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2],
   }
 }
 
+// With offmax given, a problem at a sweep boundary is first tested here (the
+// work of a separate marking pass): its waves read the off-diagonal maximum
+// the tile updates measured at the end of the previous sweep (buffer of that
+// sweep's parity); the wave of pair 0 clears the buffer the current sweep
+// will fill and, for a finished problem, records the outcome in state, which
+// every wave of the problem decides alike (done when max <= tol, failed
+// after kMaxSweeps sweeps).
 __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
                                                    int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
-                                                   const double* __restrict__ scale_in, const int* __restrict__ state,
-                                                   int inner_sweeps, unsigned long long* work, double tolrel) {
+                                                   const double* __restrict__ scale_in, int* state,
+                                                   int inner_sweeps, unsigned long long* work, double tolrel,
+                                                   unsigned long long* offmax, int stride) {
   __shared__ double2 rcs[16];
   __shared__ double red[2];
   const int2 it = list[blockIdx.x];
   const JacobiProb pr = probs[it.x];
-  if (state[it.x * kJacState]) return;
+  int* st = state + it.x * kJacState;
+  if (st[0]) return;
+  if (offmax && r < kBanded && r % pr.rounds == 0) {
+    const int sweep = r / pr.rounds;
+    const double m = __longlong_as_double((long long)offmax[((sweep + 1) & 1) * stride + it.x]);
+    const bool conv = m <= tolrel * scale_in[it.x];
+    const bool fail = !conv && sweep >= kMaxSweeps;
+    if (it.y == 0 && threadIdx.x == 0) {
+      offmax[(sweep & 1) * stride + it.x] = 0ull;
+      if (conv || fail) {
+        st[5] = sweep;
+        st[1] = fail ? 1 : 0;
+        st[0] = 1;
+      }
+    }
+    if (conv || fail) return;
+  }
   const int n = pr.dim;
   int bp, bq;
   jac_pair(pr.nbp, pr.rounds, r, it.y, bp, bq);
@@ -1263,17 +1287,16 @@ __device__ __forceinline__ int pair_index(int bp, int bq, int u) { return u < 16
 // off-diagonal magnitude it leaves (strict upper triangle; the tiles k <= l
 // hold every index pair once) into offmax, the convergence test of the next
 // sweep boundary (order independent: max on the bit patterns).
-__global__ void __launch_bounds__(64) k_jac_apply_aR(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
-                                                     int r, double* A, const double* __restrict__ Qbuf,
-                                                     const int* __restrict__ rotflag, const int* __restrict__ state,
-                                                     unsigned long long* offmax) {
-  const int4 it = list[blockIdx.x];
+__device__ __forceinline__ void jac_tile_a(const JacobiProb* __restrict__ probs, const int4 it, int r, double* A,
+                                           const double* __restrict__ Qbuf, const int* __restrict__ rotflag,
+                                           const int* __restrict__ state, unsigned long long* offmax, int stride) {
   const JacobiProb pr = probs[it.x];
   if (state[it.x * kJacState]) return;
   const int k = it.y, l = it.z;
   const int qb = (int)(pr.q_off / (32 * 32));
   const int rk = rotflag[qb + k], rl = rotflag[qb + l];
   const bool measure = offmax && r < kBanded && (r + 1) % pr.rounds == 0;
+  if (measure) offmax += ((r / pr.rounds) & 1) * stride;
   if (!rk && !rl && !measure) return;
   const int n = pr.dim;
   int kp, kq, lp, lq;
@@ -1376,10 +1399,9 @@ __global__ void __launch_bounds__(64) k_jac_apply_aR(const JacobiProb* __restric
 }
 
 // W rows I_l, columns of row block rb: Wr[I_l, ...] <- Q_l^T Wr[I_l, ...]
-__global__ void __launch_bounds__(64) k_jac_apply_wR(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
-                                                     int r, double* W, const double* __restrict__ Qbuf,
-                                                     const int* __restrict__ rotflag, const int* __restrict__ state) {
-  const int4 it = list[blockIdx.x];
+__device__ __forceinline__ void jac_tile_w(const JacobiProb* __restrict__ probs, const int4 it, int r, double* W,
+                                           const double* __restrict__ Qbuf, const int* __restrict__ rotflag,
+                                           const int* __restrict__ state) {
   const JacobiProb pr = probs[it.x];
   if (state[it.x * kJacState]) return;
   const int l = it.z, rb = it.y;
@@ -1424,8 +1446,36 @@ __global__ void __launch_bounds__(64) k_jac_apply_wR(const JacobiProb* __restric
       }
 }
 
-// initialise W = I and the per-problem scale max|A_ii| and state; molecules
-// that finished their SCC are marked done and skipped
+__global__ void __launch_bounds__(64) k_jac_apply_aR(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
+                                                     int r, double* A, const double* __restrict__ Qbuf,
+                                                     const int* __restrict__ rotflag, const int* __restrict__ state) {
+  jac_tile_a(probs, list[blockIdx.x], r, A, Qbuf, rotflag, state, nullptr, 0);
+}
+
+__global__ void __launch_bounds__(64) k_jac_apply_wR(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
+                                                     int r, double* W, const double* __restrict__ Qbuf,
+                                                     const int* __restrict__ rotflag, const int* __restrict__ state) {
+  jac_tile_w(probs, list[blockIdx.x], r, W, Qbuf, rotflag, state);
+}
+
+// both tile updates of a round in one launch: workgroups below nta update
+// A tiles (and measure the off-diagonal maximum into the buffer of the
+// sweep's parity, offmax[(sweep & 1) * stride + problem]), the rest W tiles
+__global__ void __launch_bounds__(64) k_jac_apply_R(const JacobiProb* __restrict__ probs, const int4* __restrict__ ta,
+                                                    int nta, const int4* __restrict__ tw, int r, double* A, double* W,
+                                                    const double* __restrict__ Qbuf, const int* __restrict__ rotflag,
+                                                    const int* __restrict__ state, unsigned long long* offmax,
+                                                    int stride) {
+  const int b = blockIdx.x;
+  if (b < nta)
+    jac_tile_a(probs, ta[b], r, A, Qbuf, rotflag, state, offmax, stride);
+  else
+    jac_tile_w(probs, tw[b - nta], r, W, Qbuf, rotflag, state);
+}
+
+// initialise W = I and the per-problem scale max|A_ii| and state, clear
+// both off-diagonal maximum buffers (offmax[0, 2 * nprob)); molecules that
+// finished their SCC are marked done and skipped
 __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__ probs, const double* __restrict__ A,
                                                   double* W, double* scale, int* state,
                                                   const MolState* __restrict__ mstate, unsigned long long* offmax,
@@ -1438,6 +1488,7 @@ __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__
       for (int i = 0; i < kJacState; ++i) st[i] = 0;
       st[0] = 1;
       offmax[blockIdx.x] = 0ull;
+      offmax[gridDim.x + blockIdx.x] = 0ull;
     }
     return;
   }
@@ -1453,6 +1504,7 @@ __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__
     scale[blockIdx.x] = fmax(m, 1.0);
     for (int i = 0; i < kJacState; ++i) st[i] = 0;
     offmax[blockIdx.x] = 0ull;
+    offmax[gridDim.x + blockIdx.x] = 0ull;
   }
 }
 
@@ -1513,6 +1565,20 @@ __global__ void __launch_bounds__(256) k_jac_mark(const JacobiProb* __restrict__
       }
     }
     c += 1.0;
+  }
+  c = block_sum<256>(c, sh);
+  if (threadIdx.x == 0) nactive[0] = (int)c;
+}
+
+// problems of a lane (group) still running on the block path
+__global__ void __launch_bounds__(256) k_jac_count(const JacobiProb* __restrict__ probs, int nprob,
+                                                   const int* __restrict__ state, int group, int* nactive) {
+  __shared__ double sh[4];
+  double c = 0.0;
+  for (int p = threadIdx.x; p < nprob; p += blockDim.x) {
+    const JacobiProb pr = probs[p];
+    if (pr.dim <= 64 || pr.group != group) continue;
+    if (!state[p * kJacState]) c += 1.0;
   }
   c = block_sum<256>(c, sh);
   if (threadIdx.x == 0) nactive[0] = (int)c;
