@@ -176,7 +176,8 @@ class Batch {
   // second lane of the block-Jacobi rounds: half of the problems run their
   // rounds on stream2, beside the other half on stream
   hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipStream_t stream3 = nullptr;   // the small (dim <= 64) eigenproblems beside the lanes
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
   bool two_lanes = true;
   bool lanes_one_stream = false;   // both lanes on stream, interleaved
   int nmol = 0, max_n = 0, max_nsh = 0, max_nao = 0, max_nd4 = 0, max_nbr = 0;
@@ -233,6 +234,7 @@ class Batch {
   double *q_in = nullptr, *dq = nullptr, *qlast = nullptr, *dqlast = nullptr;
   double *bdf = nullptr, *bu = nullptr, *ba = nullptr, *bomega = nullptr;
   double *yE = nullptr, *erep = nullptr, *wbo = nullptr;
+  double* gpair = nullptr;   // per atom pair H0 gradient terms: 5 doubles at 5 * (p0 + i n + j)
   // Jacobi bookkeeping
   JacSet jac_n;
   int* jac_nactive = nullptr;

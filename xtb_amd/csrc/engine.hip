@@ -580,8 +580,10 @@ void Batch::init_common() {
   }
   // default priority (as every stream here), no implicit ordering with the null stream
   HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+  HIPCHECK(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
   HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
   HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming));
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
   if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
   intcut = std::max(20.0, 25.0 - 10.0 * std::log10(opt.accuracy));
@@ -642,6 +644,7 @@ void Batch::allocate(const Capacity& c) {
   gab3 = alloc<double>(tpp); gab5 = alloc<double>(tpp);
   c6 = alloc<double>(tpp); dc6 = alloc<double>(tpp); c60 = alloc<double>(tpp); dc60 = alloc<double>(tpp);
   wbo = alloc<double>(tpp);
+  gpair = alloc<double>(5 * tpp);
   E = alloc<double>(std::max(1LL, c.ee));
   xE = alloc<double>(std::max(1LL, c.ne)); yE = alloc<double>(std::max(1LL, c.ne));
   const long long tv = std::max(1LL, c.v);
@@ -1006,7 +1009,9 @@ Batch::~Batch() {
   if (h_pinned) (void)hipHostFree(h_pinned);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
+  if (ev_join3) (void)hipEventDestroy(ev_join3);
   if (stream2) (void)hipStreamDestroy(stream2);
+  if (stream3) (void)hipStreamDestroy(stream3);
 }
 
 // ---------------------------------------------------------------------------
@@ -1019,17 +1024,22 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   // those of lane 0; the problems of a lane never touch the other lane's
   // data, so the results do not depend on the lanes
   const bool lanes = rows && !pivot_lds && js.npairs > 0;
+  // the problems of dimension <= 64 (one workgroup each, k_jac_small) run
+  // on stream3 beside the lanes
   if (lanes) {
     HIPCHECK(hipEventRecord(ev_fork, stream));
     HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
+    HIPCHECK(hipStreamWaitEvent(stream3, ev_fork, 0));
   }
   if (js.nsmall)
-    hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, lanes ? stream2 : stream, js.d_probs, js.d_small, Am,
+    hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, lanes ? stream3 : stream, js.d_probs, js.d_small, Am,
                        Wm, (const double*)js.scale, js.state, keep_w ? 1 : 0, rows ? 1 : 0);
   if (lanes) {
     jacobi_lanes(js, Am, Wm);
     HIPCHECK(hipEventRecord(ev_join, stream2));
     HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
+    HIPCHECK(hipEventRecord(ev_join3, stream3));
+    HIPCHECK(hipStreamWaitEvent(stream, ev_join3, 0));
   } else if (js.npairs) {
     // banded rounds first (neighbouring blocks only), then cyclic sweeps;
     // every round starts with the convergence test of the problems at a
@@ -1537,10 +1547,12 @@ void Batch::gradient() {
   hipLaunchKernelGGL(k_zero_atoms, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, grad,
                      (double*)nullptr);
   const dim3 gat(sub_max_n, ns);
-  hipLaunchKernelGGL(k_grad_h0, gat, dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)selfe,
-                     (const double*)dsedcn, (const double*)P, (const double*)T, (const double*)H0n, (const double*)S,
-                     (const double*)shift, (const double*)vs, (const double*)vd, (const double*)vq, intcut, grad,
-                     dhdcn);
+  hipLaunchKernelGGL(k_grad_h0_pairs, dim3(grid1((long long)sub_max_n * (sub_max_n - 1) / 2, 64), ns), dim3(64), 0,
+                     stream, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)selfe, (const double*)dsedcn,
+                     (const double*)P, (const double*)T, (const double*)H0n, (const double*)S, (const double*)shift,
+                     (const double*)vs, (const double*)vd, (const double*)vq, intcut, gpair);
+  hipLaunchKernelGGL(k_grad_h0, gat, dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)dsedcn,
+                     (const double*)P, (const double*)gpair, grad, dhdcn);
   hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_sub, xyz, q, dipm, qp, radcn, gab3, gab5, grad, tmp2);
   hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, cn_cov, q,
                      zetavec, zerovec, zetadcn, zerodcn);
@@ -1659,7 +1671,7 @@ Capacity server_capacity(double hbm_bytes, int max_atoms, int max_slots, int max
   // bytes per element of each packed array (the buffers allocate() places there)
   const double per[13] = {8.0 * (3 + 4 + 1 + 3 + 6 + 1 + 1 + 3 + 6 + 1 + 3 + 3 + 2 + 4 + 4 * XTB_MAX_REF + 1) + 4.0 * 3,
                           8.0 * 7 + 4.0 * 4, 8.0 * 16 + 4.0 * 3, 8.0 * 4 + 4.0 * 2, 8.0 * 22, 8.0, 8.0,
-                          8.0 * 7, 8.0, 16.0, 32.0, 16.0, 8.0};
+                          8.0 * 12, 8.0, 16.0, 32.0, 16.0, 8.0};
   double bytes = 0.0;
   for (int k = 0; k < 13; ++k) bytes += per[k] * (double)*cap_field(f, k);
   const long long copies = std::max(1LL, (long long)(hbm_bytes / bytes));

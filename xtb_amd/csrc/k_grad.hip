@@ -162,18 +162,87 @@ __device__ void grad_dispatch(int li, int lj, const Element& ei, int is, const E
 #undef GP
 }
 
-// Hamiltonian/overlap/multipole gradient and dE/dCN of the self energies.
-// Block = atom a; thread = partner b.  The pair (i > j) is evaluated with the
-// reference's orientation, contributing +g to i and -g to j.
+// Hamiltonian/overlap/multipole gradient and dE/dCN of the self energies,
+// in two passes.  k_grad_h0_pairs evaluates every atom pair i > j within
+// range once, with the reference's orientation: its gradient g (the term
+// added to i and subtracted from j) and the dE/dCN sums of either side,
+// stored at gpair[5 * (p0 + i n + j)].  k_grad_h0 then gives atom a (one
+// workgroup, thread = partner b) the sum over its pairs in a fixed order.
+__global__ void __launch_bounds__(64) k_grad_h0_pairs(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
+                                                      const int* __restrict__ at, const double* __restrict__ xyz,
+                                                      const int* __restrict__ at_sh0, const int* __restrict__ at_ao0,
+                                                      const double* __restrict__ selfe,
+                                                      const double* __restrict__ dsedcn, const double* __restrict__ P,
+                                                      const double* __restrict__ Pew, const double* __restrict__ H0n,
+                                                      const double* __restrict__ S, const double* __restrict__ shift,
+                                                      const double* __restrict__ vs, const double* __restrict__ vd,
+                                                      const double* __restrict__ vq, double intcut, double* gpair) {
+  const MolDesc md = desc[blockIdx.y];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)md.n * (md.n - 1) / 2) return;
+  int iat, jat;
+  tri_index(t, iat, jat);
+  const int N = md.nao;
+  const double* x = xyz + 3 * md.a0;
+  const double* ri = x + 3 * iat;
+  const double* rj = x + 3 * jat;
+  const double rij[3] = {ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]};
+  const double rij2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+  if (rij2 > 1600.0) return;
+  const Element& ei = el[at[md.a0 + iat]];
+  const Element& ej = el[at[md.a0 + jat]];
+  GradPairIn in;
+  in.P = P + md.m0; in.Pew = Pew + md.m0; in.S = S + md.m0; in.Hn = H0n + md.m0; in.N = N;
+  in.vs_i = vs[md.a0 + iat]; in.vs_j = vs[md.a0 + jat];
+  in.vd_i = vd + 3 * (md.a0 + iat); in.vd_j = vd + 3 * (md.a0 + jat);
+  in.vq_i = vq + 6 * (md.a0 + iat); in.vq_j = vq + 6 * (md.a0 + jat);
+  const double rab = sqrt(rij2), rr = rab / (ei.rad + ej.rad);
+  const double t14 = sqrt(rr), t17 = 1.0 / rij2;
+  double gp[3] = {0, 0, 0}, dcn_i = 0.0, dcn_j = 0.0;
+  int aoi = at_ao0[md.a0 + iat];
+  for (int is = 0; is < ei.nsh; ++is) {
+    const int li = ei.l[is];
+    const int shi = md.s0 + at_sh0[md.a0 + iat] + is;
+    int aoj = at_ao0[md.a0 + jat];
+    for (int js = 0; js < ej.nsh; ++js) {
+      const int lj = ej.l[js];
+      const int shj = md.s0 + at_sh0[md.a0 + jat] + js;
+      const double k1 = ei.shpoly[li] * 0.01, k2 = ej.shpoly[lj] * 0.01;
+      const double t15 = k1 * t14, t23 = k2 * t14;
+      const double shpoly = (1.0 + t15) * (1.0 + k2 * t14);
+      const double dfac = (t15 * (1.0 + t23) + (1.0 + t15) * k2 * t14) * 0.5 * t17;
+      const double hav = 0.5 * (selfe[shi] + selfe[shj]);
+      in.ves_i = shift[shi];
+      in.ves_j = shift[shj];
+      double gx[3] = {0, 0, 0}, hs = 0.0, dcn = 0.0;
+      for (int ii = 0; ii < 2 * li + 1; ++ii)
+        for (int jj = 0; jj < 2 * lj + 1; ++jj) {
+          const long long pos = md.m0 + (long long)(aoj + jj) * N + (aoi + ii);
+          const double hp = H0n[pos] * kEVtoAU * P[pos];
+          hs += 2 * hp * S[pos];
+          dcn += hp / hav * S[pos];
+        }
+      grad_dispatch(li, lj, ei, is, ej, js, ri, rj, intcut, aoi, aoj, in, gx);
+      for (int k = 0; k < 3; ++k) gp[k] += gx[k] + hs * dfac * rij[k] / shpoly;
+      dcn_i += dcn * dsedcn[shi];
+      dcn_j += dcn * dsedcn[shj];
+      aoj += 2 * lj + 1;
+    }
+    aoi += 2 * li + 1;
+  }
+  double* o = gpair + 5 * (md.p0 + (long long)iat * md.n + jat);
+  o[0] = gp[0];
+  o[1] = gp[1];
+  o[2] = gp[2];
+  o[3] = dcn_i;
+  o[4] = dcn_j;
+}
+
 __global__ void __launch_bounds__(64) k_grad_h0(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
                                                 const int* __restrict__ at, const double* __restrict__ xyz,
                                                 const int* __restrict__ at_sh0, const int* __restrict__ at_ao0,
-                                                const double* __restrict__ selfe, const double* __restrict__ dsedcn,
-                                                const double* __restrict__ P, const double* __restrict__ Pew,
-                                                const double* __restrict__ H0n, const double* __restrict__ S,
-                                                const double* __restrict__ shift, const double* __restrict__ vs,
-                                                const double* __restrict__ vd, const double* __restrict__ vq,
-                                                double intcut, double* grad, double* dhdcn) {
+                                                const double* __restrict__ dsedcn, const double* __restrict__ P,
+                                                const double* __restrict__ gpair, double* grad, double* dhdcn) {
   __shared__ double sh[4];
   const MolDesc md = desc[blockIdx.y];
   const int a = blockIdx.x;
@@ -189,49 +258,10 @@ __global__ void __launch_bounds__(64) k_grad_h0(const MolDesc* __restrict__ desc
     const double rij[3] = {ri[0] - rj[0], ri[1] - rj[1], ri[2] - rj[2]};
     const double rij2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
     if (rij2 > 1600.0) continue;
-    const Element& ei = el[at[md.a0 + iat]];
-    const Element& ej = el[at[md.a0 + jat]];
-    GradPairIn in;
-    in.P = P + md.m0; in.Pew = Pew + md.m0; in.S = S + md.m0; in.Hn = H0n + md.m0; in.N = N;
-    in.vs_i = vs[md.a0 + iat]; in.vs_j = vs[md.a0 + jat];
-    in.vd_i = vd + 3 * (md.a0 + iat); in.vd_j = vd + 3 * (md.a0 + jat);
-    in.vq_i = vq + 6 * (md.a0 + iat); in.vq_j = vq + 6 * (md.a0 + jat);
-    const double rab = sqrt(rij2), rr = rab / (ei.rad + ej.rad);
-    const double t14 = sqrt(rr), t17 = 1.0 / rij2;
-    double gp[3] = {0, 0, 0}, dcn_pair = 0.0;
-    int aoi = at_ao0[md.a0 + iat];
-    for (int is = 0; is < ei.nsh; ++is) {
-      const int li = ei.l[is];
-      const int shi = md.s0 + at_sh0[md.a0 + iat] + is;
-      int aoj = at_ao0[md.a0 + jat];
-      for (int js = 0; js < ej.nsh; ++js) {
-        const int lj = ej.l[js];
-        const int shj = md.s0 + at_sh0[md.a0 + jat] + js;
-        const double k1 = ei.shpoly[li] * 0.01, k2 = ej.shpoly[lj] * 0.01;
-        const double t15 = k1 * t14, t23 = k2 * t14;
-        const double shpoly = (1.0 + t15) * (1.0 + k2 * t14);
-        const double dfac = (t15 * (1.0 + t23) + (1.0 + t15) * k2 * t14) * 0.5 * t17;
-        const double hav = 0.5 * (selfe[shi] + selfe[shj]);
-        in.ves_i = shift[shi];
-        in.ves_j = shift[shj];
-        double gx[3] = {0, 0, 0}, hs = 0.0, dcn = 0.0;
-        for (int ii = 0; ii < 2 * li + 1; ++ii)
-          for (int jj = 0; jj < 2 * lj + 1; ++jj) {
-            const long long pos = md.m0 + (long long)(aoj + jj) * N + (aoi + ii);
-            const double hp = H0n[pos] * kEVtoAU * P[pos];
-            hs += 2 * hp * S[pos];
-            dcn += hp / hav * S[pos];
-          }
-        grad_dispatch(li, lj, ei, is, ej, js, ri, rj, intcut, aoi, aoj, in, gx);
-        for (int k = 0; k < 3; ++k) gp[k] += gx[k] + hs * dfac * rij[k] / shpoly;
-        dcn_pair += dcn * (a == iat ? dsedcn[shi] : dsedcn[shj]);
-        aoj += 2 * lj + 1;
-      }
-      aoi += 2 * li + 1;
-    }
+    const double* gp = gpair + 5 * (md.p0 + (long long)iat * md.n + jat);
     const double sgn = (a == iat) ? 1.0 : -1.0;
     for (int k = 0; k < 3; ++k) ga[k] += sgn * gp[k];
-    dcn_a += dcn_pair;
+    dcn_a += (a == iat) ? gp[3] : gp[4];
   }
   for (int k = 0; k < 3; ++k) ga[k] = block_sum<64>(ga[k], sh);
   dcn_a = block_sum<64>(dcn_a, sh);
