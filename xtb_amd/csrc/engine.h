@@ -116,6 +116,11 @@ struct JacSet {
   // the work lists hold lane 0's items first: lane g owns pairs[gpairs[g],
   // gpairs[g+1]) and likewise for the tile lists
   int gpairs[3] = {0, 0, 0}, gta[3] = {0, 0, 0}, gtw[3] = {0, 0, 0}, gto[3] = {0, 0, 0};
+  // XCD-ordered lists of the lanes: pivots (m, k) and tiles (m, y, z, kind:
+  // 0 A tile (k, l), 1 W tile (rb, l), 2 skip); lane g owns [gxp[g], gxp[g+1])
+  int2* d_xp = nullptr;
+  int4* d_xt = nullptr;
+  int gxp[3] = {0, 0, 0}, gxt[3] = {0, 0, 0};
   int* d_big = nullptr;   // problems above 64 (tridiagonal QL path)
   int nbig = 0, big_max_dim = 0;
   unsigned long long* offmax = nullptr;
@@ -197,6 +202,7 @@ class Batch {
   std::vector<HostMol> hmol_;
   std::vector<char> elem_present_;
   long long tiles_cap_ = 0, small_cap_ = 0, pairs_cap_ = 0, ta_cap_ = 0, tw_cap_ = 0, to_cap_ = 0, big_cap_ = 0;
+  long long xp_cap_ = 0, xt_cap_ = 0;
   void allocate(const Capacity& c);
   void init_common();
   void place(int m, const MolDesc& d, const HostMol& hm, const MolState& st, bool upload);

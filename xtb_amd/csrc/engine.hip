@@ -903,6 +903,55 @@ void Batch::rebuild() {
   js.gta[2] = (int)ta.size();
   js.gtw[2] = (int)tw.size();
   js.gto[2] = (int)to.size();
+  // XCD-aware round lists of the lanes: workgroup b of a launch runs on XCD
+  // b mod kXcds, so a problem placed on XCD x has its pivots and its A and W
+  // tiles at positions = x (mod kXcds) and its pivot factors Q are fetched by
+  // one XCD's L2 only.  Problems go to the least loaded XCD (tile count),
+  // largest first; short rows are padded with skip entries (m = -1).
+  std::vector<int2> xp;
+  std::vector<int4> xt;
+  for (int g = 0; g < 2; ++g) {
+    js.gxp[g] = (int)xp.size();
+    js.gxt[g] = (int)xt.size();
+    std::vector<int> pl;
+    for (int m : big)
+      if (js.probs[m].group == g) pl.push_back(m);
+    auto ntile = [&](int m) {
+      const JacobiProb& p = js.probs[m];
+      const long long np = p.nbp / 2, nwb = (p.dim + jac_p - 1) / jac_p;
+      return np * (np + 1) / 2 + nwb * np;
+    };
+    std::stable_sort(pl.begin(), pl.end(), [&](int x, int y) { return ntile(x) > ntile(y); });
+    std::vector<std::vector<int2>> qp(kXcds);
+    std::vector<std::vector<int4>> qt(kXcds);
+    std::vector<long long> xl(kXcds, 0);
+    for (int m : pl) {
+      const int x = (int)(std::min_element(xl.begin(), xl.end()) - xl.begin());
+      const JacobiProb& p = js.probs[m];
+      const int np = p.nbp / 2, nwb = (p.dim + jac_p - 1) / jac_p;
+      for (int k = 0; k < np; ++k) qp[x].push_back(make_int2(m, k));
+      for (int k = 0; k < np; ++k)
+        for (int l = k; l < np; ++l) qt[x].push_back(make_int4(m, k, l, 0));
+      for (int rb = 0; rb < nwb; ++rb)
+        for (int l = 0; l < np; ++l) qt[x].push_back(make_int4(m, rb, l, 1));
+      xl[x] += ntile(m);
+    }
+    size_t mp = 0, mt = 0;
+    for (int x = 0; x < kXcds; ++x) {
+      mp = std::max(mp, qp[x].size());
+      mt = std::max(mt, qt[x].size());
+    }
+    for (size_t j = 0; j < mp; ++j)
+      for (int x = 0; x < kXcds; ++x) xp.push_back(j < qp[x].size() ? qp[x][j] : make_int2(-1, 0));
+    for (size_t j = 0; j < mt; ++j)
+      for (int x = 0; x < kXcds; ++x) xt.push_back(j < qt[x].size() ? qt[x][j] : make_int4(-1, 0, 0, 2));
+  }
+  js.gxp[2] = (int)xp.size();
+  js.gxt[2] = (int)xt.size();
+  grow(js.d_xp, xp_cap_, std::max<long long>(1, (long long)xp.size()));
+  grow(js.d_xt, xt_cap_, std::max<long long>(1, (long long)xt.size()));
+  if (!xp.empty()) HIPCHECK(hipMemcpyAsync(js.d_xp, xp.data(), xp.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+  if (!xt.empty()) HIPCHECK(hipMemcpyAsync(js.d_xt, xt.data(), xt.size() * sizeof(int4), hipMemcpyHostToDevice, stream));
   if (qoff > js.qsize) throw std::runtime_error("Jacobi pivot buffer too small");
   HIPCHECK(hipMemcpyAsync(js.d_probs, js.probs.data(), nmol * sizeof(JacobiProb), hipMemcpyHostToDevice, stream));
   js.nsmall = (int)small.size();
@@ -1135,8 +1184,8 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
     for (int g = 0; g < 2; ++g) {
       if (!live[g]) continue;
       prof_begin(2, ls[g]);
-      hipLaunchKernelGGL(k_jac_pairsR, dim3(js.gpairs[g + 1] - js.gpairs[g]), dim3(64), 0, ls[g], js.d_probs,
-                         js.d_pairs + js.gpairs[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
+      hipLaunchKernelGGL(k_jac_pairsR, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
+                         js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
                          js.state, inner_sweeps, jw, jac_tol, js.offmax, nmol);
       prof_end(2, nullptr, 0, ls[g]);
     }
@@ -1154,9 +1203,8 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
     }
     for (int g = 0; g < 2; ++g) {
       if (!live[g]) continue;
-      const int nta = js.gta[g + 1] - js.gta[g], ntw = js.gtw[g + 1] - js.gtw[g];
-      hipLaunchKernelGGL(k_jac_apply_R, dim3(nta + ntw), dim3(64), 0, ls[g], js.d_probs, js.d_ta + js.gta[g], nta,
-                         js.d_tw + js.gtw[g], r, Am, Wm, (const double*)js.qbuf, (const int*)js.rot,
+      hipLaunchKernelGGL(k_jac_apply_R, dim3(js.gxt[g + 1] - js.gxt[g]), dim3(64), 0, ls[g], js.d_probs,
+                         js.d_xt + js.gxt[g], r, Am, Wm, (const double*)js.qbuf, (const int*)js.rot,
                          (const int*)js.state, js.offmax, nmol);
     }
     jacobi_rounds_ += 1;

@@ -348,6 +348,8 @@ __device__ __forceinline__ bool jac_boundary(int rounds, int r) {
 // state per problem: [0] done, [1] failed, [5] outer sweeps (block path) or
 // inner rounds with rotations (dim <= 64)
 constexpr int kJacState = 8;
+// XCDs of the gfx950 part (MI355X): workgroup b of a launch runs on XCD b mod kXcds
+constexpr int kXcds = 8;
 constexpr int kMaxSweeps = 40;
 
 // problems of dimension <= 64: one workgroup solves to convergence in LDS
@@ -1166,6 +1168,7 @@ __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict_
   __shared__ double2 rcs[16];
   __shared__ double red[2];
   const int2 it = list[blockIdx.x];
+  if (it.x < 0) return;   // skip entry of an XCD-ordered list
   const JacobiProb pr = probs[it.x];
   int* st = state + it.x * kJacState;
   if (st[0]) return;
@@ -1458,19 +1461,19 @@ __global__ void __launch_bounds__(64) k_jac_apply_wR(const JacobiProb* __restric
   jac_tile_w(probs, list[blockIdx.x], r, W, Qbuf, rotflag, state);
 }
 
-// both tile updates of a round in one launch: workgroups below nta update
-// A tiles (and measure the off-diagonal maximum into the buffer of the
-// sweep's parity, offmax[(sweep & 1) * stride + problem]), the rest W tiles
-__global__ void __launch_bounds__(64) k_jac_apply_R(const JacobiProb* __restrict__ probs, const int4* __restrict__ ta,
-                                                    int nta, const int4* __restrict__ tw, int r, double* A, double* W,
-                                                    const double* __restrict__ Qbuf, const int* __restrict__ rotflag,
-                                                    const int* __restrict__ state, unsigned long long* offmax,
-                                                    int stride) {
-  const int b = blockIdx.x;
-  if (b < nta)
-    jac_tile_a(probs, ta[b], r, A, Qbuf, rotflag, state, offmax, stride);
-  else
-    jac_tile_w(probs, tw[b - nta], r, W, Qbuf, rotflag, state);
+// both tile updates of a round in one launch, one list entry (m, y, z, kind)
+// per workgroup: kind 0 updates A tile (y, z) = (k, l) (and measures the
+// off-diagonal maximum into the buffer of the sweep's parity, offmax[(sweep &
+// 1) * stride + problem]), kind 1 W tile (rb, l) = (y, z), kind 2 is a skip
+__global__ void __launch_bounds__(64) k_jac_apply_R(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
+                                                    int r, double* A, double* W, const double* __restrict__ Qbuf,
+                                                    const int* __restrict__ rotflag, const int* __restrict__ state,
+                                                    unsigned long long* offmax, int stride) {
+  const int4 it = list[blockIdx.x];
+  if (it.w == 0)
+    jac_tile_a(probs, it, r, A, Qbuf, rotflag, state, offmax, stride);
+  else if (it.w == 1)
+    jac_tile_w(probs, it, r, W, Qbuf, rotflag, state);
 }
 
 // initialise W = I and the per-problem scale max|A_ii| and state, clear
