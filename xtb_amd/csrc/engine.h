@@ -102,6 +102,8 @@ struct JacobiProb {
   long long a_off, w_off, q_off;
   int ev_off;                 // eigenvalue offset
   int group;                  // lane (stream) of the two-lane round schedule
+  int ld;                     // row stride of the problem's A and W (dim, or dim rounded up to 16)
+  long long src_off;          // the matrix in the caller's packed layout (row stride dim)
 };
 
 // A set of Jacobi problems (one per molecule) with its device work lists.
@@ -122,7 +124,10 @@ struct JacSet {
   int4* d_xt = nullptr;
   int gxp[3] = {0, 0, 0}, gxt[3] = {0, 0, 0};
   int* d_big = nullptr;   // problems above 64 (tridiagonal QL path)
-  int nbig = 0, big_max_dim = 0;
+  // padded working copies of A and W (row stride ld, every row 128-byte aligned)
+  double *ja = nullptr, *jw = nullptr;
+  long long jcap = 0;
+  int nbig = 0, big_max_dim = 0, big_max_ld = 0;   // big_max_ld: largest ld of any problem
   unsigned long long* offmax = nullptr;
   double* qbuf = nullptr;
   int* rot = nullptr;
@@ -184,6 +189,9 @@ class Batch {
   hipStream_t stream3 = nullptr;   // the small (dim <= 64) eigenproblems beside the lanes
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
   bool two_lanes = true;
+  // the block-Jacobi problems work on padded copies (rows 128-byte aligned);
+  // the register path only (no refinement, banded rounds, LDS kernels or QL)
+  bool jac_padded = true;
   bool lanes_one_stream = false;   // both lanes on stream, interleaved
   int nmol = 0, max_n = 0, max_nsh = 0, max_nao = 0, max_nd4 = 0, max_nbr = 0;
   double intcut = 25.0, neglect = 1e-8, scfconv = 1e-6, qconv = 1e-4;

@@ -586,6 +586,7 @@ void Batch::init_common() {
   HIPCHECK(hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming));
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
   if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
+  jac_padded = jac_p == 32 && !apply_lds && !pivot_lds && oa_max_steps == 0 && banded_passes == 0 && !use_ql;
   intcut = std::max(20.0, 25.0 - 10.0 * std::log10(opt.accuracy));
   neglect = 10.0e-9 * opt.accuracy;
   scfconv = 1.e-6 * opt.accuracy;
@@ -845,6 +846,8 @@ void Batch::rebuild() {
   std::vector<int4> ta, tw, to;
   js.max_rounds = 0;
   js.big_max_dim = 0;
+  js.big_max_ld = 0;
+  long long poff = 0;
   for (int m = 0; m < nmol; ++m) {
     JacobiProb& p = js.probs[m];
     std::memset(&p, 0, sizeof(p));
@@ -853,10 +856,14 @@ void Batch::rebuild() {
     p.nb = (p.dim + jac_p / 2 - 1) / (jac_p / 2);
     p.nbp = p.nb + (p.nb & 1);
     p.rounds = std::max(1, p.nbp - 1);
-    p.a_off = desc[m].m0;
+    p.src_off = desc[m].m0;
+    p.ld = jac_padded ? (p.dim + 15) / 16 * 16 : p.dim;
+    p.a_off = jac_padded ? poff : desc[m].m0;
     p.w_off = p.a_off;
+    if (jac_padded) poff += (long long)p.ld * p.ld;
     p.ev_off = desc[m].o0;
     p.q_off = qoff;
+    js.big_max_ld = std::max(js.big_max_ld, p.ld);
     if (p.dim <= 64) {
       small.push_back(m);
       continue;
@@ -953,6 +960,11 @@ void Batch::rebuild() {
   if (!xp.empty()) HIPCHECK(hipMemcpyAsync(js.d_xp, xp.data(), xp.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
   if (!xt.empty()) HIPCHECK(hipMemcpyAsync(js.d_xt, xt.data(), xt.size() * sizeof(int4), hipMemcpyHostToDevice, stream));
   if (qoff > js.qsize) throw std::runtime_error("Jacobi pivot buffer too small");
+  if (jac_padded) {
+    long long c0 = js.jcap;
+    grow(js.ja, c0, std::max<long long>(1, poff));
+    grow(js.jw, js.jcap, std::max<long long>(1, poff));
+  }
   HIPCHECK(hipMemcpyAsync(js.d_probs, js.probs.data(), nmol * sizeof(JacobiProb), hipMemcpyHostToDevice, stream));
   js.nsmall = (int)small.size();
   js.npairs = (int)pairs.size();
@@ -1067,6 +1079,16 @@ Batch::~Batch() {
 bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w, int banded) {
   const int nb = nmol;
   const bool rows = jac_p == 32 && !apply_lds && !keep_w && banded == 0;
+  const MolState* act = check_active ? (const MolState*)d_state : (const MolState*)nullptr;
+  double* Wout = Wm;
+  if (jac_padded) {
+    if (!rows || pivot_lds) throw std::runtime_error("padded Jacobi problems need the register path");
+    // A into the padded working copy; the eigenvectors come back to Wm at the end
+    hipLaunchKernelGGL(k_jac_pad, dim3(grid1((long long)js.big_max_ld * js.big_max_ld, 256), nb), dim3(256), 0,
+                       stream, js.d_probs, (const double*)Am, js.ja, act, 1);
+    Am = js.ja;
+    Wm = js.jw;
+  }
   hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
                      check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0);
   // two lanes: the problems of lane 1 run their rounds on stream2 beside
@@ -1154,6 +1176,9 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
       jacobi_rounds_ += 1;
     }
   }
+  if (jac_padded)
+    hipLaunchKernelGGL(k_jac_pad, dim3(grid1((long long)js.big_max_ld * js.big_max_ld, 256), nb), dim3(256), 0,
+                       stream, js.d_probs, (const double*)Wm, Wout, act, 0);
   if (evout)
     hipLaunchKernelGGL(k_jac_sort, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm,
                        check_active ? (const MolState*)d_state : (const MolState*)nullptr, 1 << 30);

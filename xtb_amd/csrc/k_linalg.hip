@@ -365,12 +365,12 @@ __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict_
   const int pidx = list[blockIdx.x];
   const JacobiProb pr = probs[pidx];
   if (state[pidx * kJacState]) return;
-  const int n = pr.dim;
+  const int n = pr.dim, ld = pr.ld;
   double* Ap = A + pr.a_off;
   double* Wp = W + pr.w_off;
   for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
     const int u = e >> 6, v = e & 63;
-    a[u * kLd + v] = (u < n && v < n) ? Ap[(long long)u * n + v] : 0.0;
+    a[u * kLd + v] = (u < n && v < n) ? Ap[(long long)u * ld + v] : 0.0;
     q[u * kLd + v] = (u == v) ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -379,22 +379,22 @@ __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict_
   __syncthreads();
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
     const int u = e / n, v = e % n;
-    Ap[(long long)u * n + v] = a[u * kLd + v];
-    if (!keep_w) Wp[(long long)u * n + v] = rows_out ? q[v * kLd + u] : q[u * kLd + v];
+    Ap[(long long)u * ld + v] = a[u * kLd + v];
+    if (!keep_w) Wp[(long long)u * ld + v] = rows_out ? q[v * kLd + u] : q[u * kLd + v];
   }
   if (keep_w && nrot > 0) {
     // W <- W Q, with the old W staged in LDS
     __syncthreads();
     for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
       const int u = e >> 6, v = e & 63;
-      a[u * kLd + v] = (u < n && v < n) ? Wp[(long long)u * n + v] : 0.0;
+      a[u * kLd + v] = (u < n && v < n) ? Wp[(long long)u * ld + v] : 0.0;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
       const int u = e / n, v = e % n;
       double acc = 0.0;
       for (int k = 0; k < n; ++k) acc = fma(a[u * kLd + k], q[k * kLd + v], acc);
-      Wp[(long long)u * n + v] = acc;
+      Wp[(long long)u * ld + v] = acc;
     }
   }
   if (threadIdx.x == 0) {
@@ -1187,7 +1187,7 @@ __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict_
     }
     if (conv || fail) return;
   }
-  const int n = pr.dim;
+  const int n = pr.dim, ld = pr.ld;
   int bp, bq;
   jac_pair(pr.nbp, pr.rounds, r, it.y, bp, bq);
   const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
@@ -1198,8 +1198,8 @@ __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict_
 #pragma unroll
     for (int hc = 0; hc < 2; ++hc) {
       const int gt = 16 * bp + 4 * g + i, gb = 16 * bq + 4 * g + i, gv = (hc ? 16 * bq : 16 * bp) + c;
-      T[i][hc] = (gt < n && gv < n) ? Ap[(long long)gt * n + gv] : 0.0;
-      B[i][hc] = (gb < n && gv < n) ? Ap[(long long)gb * n + gv] : 0.0;
+      T[i][hc] = (gt < n && gv < n) ? Ap[(long long)gt * ld + gv] : 0.0;
+      B[i][hc] = (gb < n && gv < n) ? Ap[(long long)gb * ld + gv] : 0.0;
     }
   double QT[4][2], QB[4][2];   // Q^T rows (top, bottom slot of pair 4g + i), columns (c, c + 16)
 #pragma unroll
@@ -1301,7 +1301,7 @@ __device__ __forceinline__ void jac_tile_a(const JacobiProb* __restrict__ probs,
   const bool measure = offmax && r < kBanded && (r + 1) % pr.rounds == 0;
   if (measure) offmax += ((r / pr.rounds) & 1) * stride;
   if (!rk && !rl && !measure) return;
-  const int n = pr.dim;
+  const int n = pr.dim, ld = pr.ld;
   int kp, kq, lp, lq;
   jac_pair(pr.nbp, pr.rounds, r, k, kp, kq);
   jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
@@ -1319,7 +1319,7 @@ __device__ __forceinline__ void jac_tile_a(const JacobiProb* __restrict__ probs,
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int gu = pair_index(kp, kq, 16 * h + i16);
-        av[s][h] = (gw < n && gu < n) ? Ap[(long long)gw * n + gu] : 0.0;
+        av[s][h] = (gw < n && gu < n) ? Ap[(long long)gw * ld + gu] : 0.0;
         bv[s][h] = Ql[w * 32 + 16 * h + i16];
       }
     }
@@ -1341,7 +1341,7 @@ __device__ __forceinline__ void jac_tile_a(const JacobiProb* __restrict__ probs,
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int gu = pair_index(kp, kq, 16 * a + k4 + 4 * i), gv = pair_index(lp, lq, 16 * b + i16);
-          t[a][b][i] = (gu < n && gv < n) ? Ap[(long long)gu * n + gv] : 0.0;
+          t[a][b][i] = (gu < n && gv < n) ? Ap[(long long)gu * ld + gv] : 0.0;
         }
   }
   v4d o[2][2];
@@ -1381,8 +1381,8 @@ __device__ __forceinline__ void jac_tile_a(const JacobiProb* __restrict__ probs,
           const int u = 16 * a + k4 + 4 * i, v = 16 * b + i16;
           const int gu = pair_index(kp, kq, u), gv = pair_index(lp, lq, v);
           if (gu >= n || gv >= n || (k == l && u > v)) continue;
-          Ap[(long long)gu * n + gv] = o[a][b][i];
-          Ap[(long long)gv * n + gu] = o[a][b][i];
+          Ap[(long long)gu * ld + gv] = o[a][b][i];
+          Ap[(long long)gv * ld + gu] = o[a][b][i];
         }
   if (measure) {
     double m = 0.0;
@@ -1410,7 +1410,7 @@ __device__ __forceinline__ void jac_tile_w(const JacobiProb* __restrict__ probs,
   const int l = it.z, rb = it.y;
   const int qb = (int)(pr.q_off / (32 * 32));
   if (!rotflag[qb + l]) return;
-  const int n = pr.dim;
+  const int n = pr.dim, ld = pr.ld;
   int lp, lq;
   jac_pair(pr.nbp, pr.rounds, r, l, lp, lq);
   double* Wp = W + pr.w_off;
@@ -1424,7 +1424,7 @@ __device__ __forceinline__ void jac_tile_w(const JacobiProb* __restrict__ probs,
     for (int h = 0; h < 2; ++h) {
       const int x = 32 * rb + 16 * h + i16;
       qv[s][h] = Ql[w * 32 + 16 * h + i16];
-      wv[s][h] = (gw < n && x < n) ? Wp[(long long)gw * n + x] : 0.0;
+      wv[s][h] = (gw < n && x < n) ? Wp[(long long)gw * ld + x] : 0.0;
     }
   }
   v4d o[2][2];
@@ -1445,7 +1445,7 @@ __device__ __forceinline__ void jac_tile_w(const JacobiProb* __restrict__ probs,
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int gv = pair_index(lp, lq, 16 * a + k4 + 4 * i), x = 32 * rb + 16 * b + i16;
-        if (gv < n && x < n) Wp[(long long)gv * n + x] = o[a][b][i];
+        if (gv < n && x < n) Wp[(long long)gv * ld + x] = o[a][b][i];
       }
 }
 
@@ -1495,13 +1495,14 @@ __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__
     }
     return;
   }
-  const int n = pr.dim;
+  const int n = pr.dim, ld = pr.ld;
   double* Wp = W + pr.w_off;
   const double* Ap = A + pr.a_off;
   double m = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(Ap[(long long)i * n + i]));
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(Ap[(long long)i * ld + i]));
   if (!keep_w)
-    for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x) Wp[e] = (e / n == e % n) ? 1.0 : 0.0;
+    for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x)
+      Wp[(e / n) * ld + e % n] = (e / n == e % n) ? 1.0 : 0.0;
   m = block_max<256>(m, sh);
   if (threadIdx.x == 0) {
     scale[blockIdx.x] = fmax(m, 1.0);
@@ -1509,6 +1510,24 @@ __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__
     offmax[blockIdx.x] = 0ull;
     offmax[gridDim.x + blockIdx.x] = 0ull;
   }
+}
+
+// copies between a problem's matrix in the caller's packed layout (row stride
+// dim at src_off) and its padded working copy (row stride ld at a_off / w_off):
+// in = 1: src packed -> dst padded (A); in = 0: src padded -> dst packed (W)
+__global__ void __launch_bounds__(256) k_jac_pad(const JacobiProb* __restrict__ probs, const double* __restrict__ src,
+                                                 double* dst, const MolState* __restrict__ mstate, int in) {
+  const int p = blockIdx.y;
+  if (mstate && !mstate[p].active) return;
+  const JacobiProb pr = probs[p];
+  const long long n = pr.dim;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  const long long u = e / n, v = e % n;
+  if (in)
+    dst[pr.a_off + u * pr.ld + v] = src[pr.src_off + e];
+  else
+    dst[pr.src_off + e] = src[pr.w_off + u * pr.ld + v];
 }
 
 // largest off-diagonal magnitude of the unfinished block-path problems that
@@ -1523,12 +1542,12 @@ __global__ void __launch_bounds__(256) k_jac_offmax(const JacobiProb* __restrict
   const int4 it = list[blockIdx.x];
   const JacobiProb pr = probs[it.x];
   if (state[it.x * kJacState] || !jac_boundary(pr.rounds, r)) return;
-  const int n = pr.dim;
+  const int n = pr.dim, ld = pr.ld;
   const double* Ap = A + pr.a_off;
   double m = 0.0;
   for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
     const int gu = it.y * 64 + (e >> 6), gv = it.z * 64 + (e & 63);
-    if (gu < gv && gv < n) m = fmax(m, fabs(Ap[(long long)gu * n + gv]));
+    if (gu < gv && gv < n) m = fmax(m, fabs(Ap[(long long)gu * ld + gv]));
   }
   m = block_max<256>(m, sh);
   if (threadIdx.x == 0 && m > 0.0) atomicMax(&offmax[it.x], (unsigned long long)__double_as_longlong(m));
@@ -1905,13 +1924,13 @@ __global__ void k_jac_sort(const JacobiProb* __restrict__ probs, const double* _
   if (mstate && !mstate[blockIdx.x].active) return;
   const JacobiProb pr = probs[blockIdx.x];
   if (pr.dim > max_dim) return;
-  const int n = pr.dim;
+  const int n = pr.dim, ld = pr.ld;
   const double* Ap = A + pr.a_off;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double di = Ap[(long long)i * n + i];
+    const double di = Ap[(long long)i * ld + i];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
-      const double dj = Ap[(long long)j * n + j];
+      const double dj = Ap[(long long)j * ld + j];
       rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
     }
     ev[pr.ev_off + rank] = di;
