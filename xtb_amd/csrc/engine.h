@@ -288,6 +288,7 @@ class Batch {
   int oa_max_steps = 0;     // refinement steps before the Jacobi sweeps (0 = off)
   int inner_sweeps = 1;    // sweeps of each pivot problem per block-Jacobi round
   double jac_tol = 1e-12;   // Jacobi convergence: max |A_ij| <= jac_tol * max(|A_ii|, 1)
+  double jac_tol_mid = 1e-12;   // the same before a molecule's final diagonalization
   bool trace_rounds = false;   // print the Jacobi rounds of each SCC iteration to stderr
   bool pivot_lds = false;  // 32x32 pivots: LDS kernel instead of the register-resident one
   bool apply_lds = false;  // 32x32 pivots: LDS tile updates with W as columns instead of register ones with W as rows

@@ -573,6 +573,8 @@ void Batch::init_common() {
   if (const char* r = std::getenv("XTBAMD_APPLY_KERNEL")) apply_lds = std::strcmp(r, "lds") == 0;
   if (const char* r = std::getenv("XTBAMD_BANDED")) banded_passes = std::max(0, std::atoi(r));
   if (const char* r = std::getenv("XTBAMD_JAC_TOL")) jac_tol = std::atof(r);
+  jac_tol_mid = jac_tol;
+  if (const char* r = std::getenv("XTBAMD_JAC_TOL_MID")) jac_tol_mid = std::max(jac_tol, std::atof(r));
   trace_rounds = std::getenv("XTBAMD_TRACE_ROUNDS") != nullptr;
   if (const char* r = std::getenv("XTBAMD_JAC_LANES")) {
     two_lanes = std::atoi(r) != 1;
@@ -1090,7 +1092,8 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
     Wm = js.jw;
   }
   hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
-                     check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0);
+                     check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0,
+                     jac_tol_mid / jac_tol);
   // two lanes: the problems of lane 1 run their rounds on stream2 beside
   // those of lane 0; the problems of a lane never touch the other lane's
   // data, so the results do not depend on the lanes
@@ -1240,7 +1243,7 @@ void Batch::eigh_rows(JacSet& js, double* Am, double* Wm, double* evout) {
   const MolState* st = d_state;
   // state and scale of the small problems (W is written whole below)
   hipLaunchKernelGGL(k_jac_init, dim3(nmol), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale,
-                     js.state, st, js.offmax, 1);
+                     js.state, st, js.offmax, 1, 1.0);
   if (js.nsmall) {
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
                        (const double*)js.scale, js.state, 0, 1);

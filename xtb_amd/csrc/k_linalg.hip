@@ -101,8 +101,10 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(const int4* __restrict__ tile
                                                    double* C, const double* __restrict__ scale,
                                                    const int* __restrict__ perm, double alpha,
                                                    const int* __restrict__ mask, int mstride) {
-  __shared__ double As[kBK][kTile + 1];
-  __shared__ double Bs[kBK][kTile + 1];
+  // two LDS stages: the global loads of k-block kb + kBK are in flight while
+  // the matrix cores work on k-block kb, one barrier per k-block
+  __shared__ double As[2][kBK][kTile + 1];
+  __shared__ double Bs[2][kBK][kTile + 1];
   const int4 tl = tiles[blockIdx.x];
   if (mask && !mask[(long long)tl.x * mstride]) return;
   const MolDesc md = desc[tl.x];
@@ -115,46 +117,66 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(const int4* __restrict__ tile
   const int row0 = tl.y * kTile, col0 = tl.z * kTile;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = w >> 1, wc = w & 1, i16 = lane & 15, k4 = lane >> 4;
+  // this thread's staging slots: A (ii, kk) and B (k2, jj) for q = 0..3
+  int sa_i[4], sa_k[4], sb_k[4], sb_j[4], cjv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = tid + 256 * q;
+    if (TA) { sa_k[q] = e >> 6; sa_i[q] = e & 63; } else { sa_i[q] = e >> 4; sa_k[q] = e & 15; }
+    if (TB) { sb_j[q] = e >> 4; sb_k[q] = e & 15; } else { sb_k[q] = e >> 6; sb_j[q] = e & 63; }
+    const int gj = col0 + sb_j[q];
+    cjv[q] = (gj < N) ? (PERM ? pm[gj] : gj) : -1;
+  }
+  double ra[4], rb[4];
+  auto fetch = [&](int kb) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int gi = row0 + sa_i[q], gk = kb + sa_k[q];
+      ra[q] = (gi < N && gk < N) ? (TA ? Am[(long long)gk * N + gi] : Am[(long long)gi * N + gk]) : 0.0;
+      const int cj = cjv[q], gk2 = kb + sb_k[q];
+      double x = 0.0;
+      if (cj >= 0 && gk2 < N) {
+        x = TB ? Bm[(long long)cj * N + gk2] : Bm[(long long)gk2 * N + cj];
+        if (SCALE) x *= sc[gk2];
+      }
+      rb[q] = x;
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      As[buf][sa_k[q]][sa_i[q]] = ra[q];
+      Bs[buf][sb_k[q]][sb_j[q]] = rb[q];
+    }
+  };
   v4d acc[2][2];
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int c = 0; c < 2; ++c) acc[r][c] = v4d{0.0, 0.0, 0.0, 0.0};
+  fetch(0);
+  stage(0);
+  __syncthreads();
+  int buf = 0;
   for (int kb = 0; kb < N; kb += kBK) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + 256 * q;
-      int ii, kk;
-      if (TA) { kk = e >> 6; ii = e & 63; } else { ii = e >> 4; kk = e & 15; }
-      const int gi = row0 + ii, gk = kb + kk;
-      double v = 0.0;
-      if (gi < N && gk < N) v = TA ? Am[(long long)gk * N + gi] : Am[(long long)gi * N + gk];
-      As[kk][ii] = v;
-      int jj, k2;
-      if (TB) { jj = e >> 4; k2 = e & 15; } else { k2 = e >> 6; jj = e & 63; }
-      const int gj = col0 + jj, gk2 = kb + k2;
-      double x = 0.0;
-      if (gj < N && gk2 < N) {
-        const int cj = PERM ? pm[gj] : gj;
-        x = TB ? Bm[(long long)cj * N + gk2] : Bm[(long long)gk2 * N + cj];
-        if (SCALE) x *= sc[gk2];
-      }
-      Bs[k2][jj] = x;
-    }
-    __syncthreads();
+    const bool more = kb + kBK < N;
+    if (more) fetch(kb + kBK);
 #pragma unroll
     for (int kk = 0; kk < kBK; kk += 4) {
       double a[2], b[2];
 #pragma unroll
-      for (int r = 0; r < 2; ++r) a[r] = As[kk + k4][32 * wr + 16 * r + i16];
+      for (int r = 0; r < 2; ++r) a[r] = As[buf][kk + k4][32 * wr + 16 * r + i16];
 #pragma unroll
-      for (int c = 0; c < 2; ++c) b[c] = Bs[kk + k4][32 * wc + 16 * c + i16];
+      for (int c = 0; c < 2; ++c) b[c] = Bs[buf][kk + k4][32 * wc + 16 * c + i16];
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
     }
+    // the other stage was last read before the previous barrier
+    if (more) stage(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
   for (int r = 0; r < 2; ++r)
@@ -1482,7 +1504,7 @@ __global__ void __launch_bounds__(64) k_jac_apply_R(const JacobiProb* __restrict
 __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__ probs, const double* __restrict__ A,
                                                   double* W, double* scale, int* state,
                                                   const MolState* __restrict__ mstate, unsigned long long* offmax,
-                                                  int keep_w) {
+                                                  int keep_w, double loose) {
   __shared__ double sh[4];
   const JacobiProb pr = probs[blockIdx.x];
   int* st = state + blockIdx.x * kJacState;
@@ -1505,7 +1527,9 @@ __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__
       Wp[(e / n) * ld + e % n] = (e / n == e % n) ? 1.0 : 0.0;
   m = block_max<256>(m, sh);
   if (threadIdx.x == 0) {
-    scale[blockIdx.x] = fmax(m, 1.0);
+    // a molecule before its final diagonalization converges to a looser
+    // tolerance (loose x the final one)
+    scale[blockIdx.x] = fmax(m, 1.0) * ((mstate && !mstate[blockIdx.x].lastdiag) ? loose : 1.0);
     for (int i = 0; i < kJacState; ++i) st[i] = 0;
     offmax[blockIdx.x] = 0ull;
     offmax[gridDim.x + blockIdx.x] = 0ull;
