@@ -43,4 +43,4 @@ def test_calibrated_table_roundtrip(tmp_path):
 def test_committed_table_has_dominant_kernels():
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     table = profiling.traffic_table(os.path.join(root, "profiles", "batched512_hbm_traffic.md"))
-    assert table["k_jac_pairsR"] > 0 and table["k_gemm_mfma"] > 0
+    assert table["k_jac_pairsR"] > 0 and table["k_jac_apply_R"] > 0 and table["k_gemm_mfma"] > 0

@@ -312,6 +312,7 @@ class Batch {
   int prof_mask = 0;
   std::vector<hipEvent_t> prof_ev_[3];
   int prof_used_[3] = {0, 0, 0};
+  int prof_all_[3] = {0, 0, 0};   // every launch of a family timed on a sample
   unsigned long long* prof_work_ = nullptr;   // [3]
   void prof_begin(int kind, hipStream_t s = nullptr);
   void prof_end(int kind, const int* mask, int mstride, hipStream_t s = nullptr);

@@ -934,16 +934,28 @@ void Batch::rebuild() {
     std::vector<std::vector<int2>> qp(kXcds);
     std::vector<std::vector<int4>> qt(kXcds);
     std::vector<long long> xl(kXcds, 0);
+    long long lane_tiles = 0;
+    for (int m : pl) lane_tiles += ntile(m);
     for (int m : pl) {
-      const int x = (int)(std::min_element(xl.begin(), xl.end()) - xl.begin());
       const JacobiProb& p = js.probs[m];
       const int np = p.nbp / 2, nwb = (p.dim + jac_p - 1) / jac_p;
-      for (int k = 0; k < np; ++k) qp[x].push_back(make_int2(m, k));
+      // a problem with more than half an XCD's share of the lane's tiles is
+      // spread over all XCDs (pivot k and the tiles of pair l on XCD
+      // k, l mod kXcds), the others stay on one XCD each
+      const bool spread = 2 * kXcds * ntile(m) > lane_tiles;
+      const int x = (int)(std::min_element(xl.begin(), xl.end()) - xl.begin());
+      auto home = [&](int l) { return spread ? l % kXcds : x; };
+      for (int k = 0; k < np; ++k) qp[home(k)].push_back(make_int2(m, k));
       for (int k = 0; k < np; ++k)
-        for (int l = k; l < np; ++l) qt[x].push_back(make_int4(m, k, l, 0));
+        for (int l = k; l < np; ++l) {
+          qt[home(l)].push_back(make_int4(m, k, l, 0));
+          xl[home(l)] += 1;
+        }
       for (int rb = 0; rb < nwb; ++rb)
-        for (int l = 0; l < np; ++l) qt[x].push_back(make_int4(m, rb, l, 1));
-      xl[x] += ntile(m);
+        for (int l = 0; l < np; ++l) {
+          qt[home(l)].push_back(make_int4(m, rb, l, 1));
+          xl[home(l)] += 1;
+        }
     }
     size_t mp = 0, mt = 0;
     for (int x = 0; x < kXcds; ++x) {
@@ -1196,6 +1208,8 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
 // measured at the end of the sweep before) and the A and W tile updates.
 // Once per max_rounds rounds, after the pivots, the host reads both lanes'
 // counts of problems still running and stops a lane that has none left.
+constexpr int kProfEvery = 8;
+
 void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   const hipStream_t ls[2] = {stream, lanes_one_stream ? stream : stream2};
   bool live[2] = {js.gpairs[0] < js.gpairs[1], js.gpairs[1] < js.gpairs[2]};
@@ -1211,11 +1225,15 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   for (int r = 0; r <= rmax; ++r) {
     for (int g = 0; g < 2; ++g) {
       if (!live[g]) continue;
-      prof_begin(2, ls[g]);
+      // timing events (and the flop count) on every kProfEvery-th round only:
+      // an event pair per launch would cost more host time than the launch
+      const bool sample = (r % kProfEvery) == 0;
+      if (prof_mask & 4) prof_all_[2] += 1;
+      if (sample) prof_begin(2, ls[g]);
       hipLaunchKernelGGL(k_jac_pairsR, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
                          js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
-                         js.state, inner_sweeps, jw, jac_tol, js.offmax, nmol);
-      prof_end(2, nullptr, 0, ls[g]);
+                         js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol);
+      if (sample) prof_end(2, nullptr, 0, ls[g]);
     }
     if (r % js.max_rounds == 0) {
       for (int g = 0; g < 2; ++g)
@@ -1339,7 +1357,7 @@ void Batch::gemm(int kind, const double* a, const double* b, double* c, const do
 
 void Batch::prof_enable(int mask) {
   prof_mask = mask;
-  for (int k = 0; k < 3; ++k) prof_used_[k] = 0;
+  for (int k = 0; k < 3; ++k) prof_used_[k] = prof_all_[k] = 0;
   if (!prof_work_) prof_work_ = alloc<unsigned long long>(3);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, (double*)prof_work_, 3LL, 0.0);
   HIPCHECK(hipStreamSynchronize(stream));
@@ -1376,9 +1394,14 @@ int Batch::prof_result(int kind, int* launches, double* total_ms, double* work) 
   }
   unsigned long long w = 0;
   if (prof_work_) HIPCHECK(hipMemcpy(&w, prof_work_ + kind, sizeof(w), hipMemcpyDeviceToHost));
-  if (launches) *launches = prof_used_[kind] / 2;
-  if (total_ms) *total_ms = t;
-  if (work) *work = (double)w;
+  // a sampled family (timed on a subset of its launches) is reported for all
+  // of its launches: the sample's time and work scaled by all / sampled
+  const int timed = prof_used_[kind] / 2;
+  double f = 1.0;
+  if (prof_all_[kind] > timed && timed > 0) f = (double)prof_all_[kind] / timed;
+  if (launches) *launches = f > 1.0 ? prof_all_[kind] : timed;
+  if (total_ms) *total_ms = t * f;
+  if (work) *work = (double)w * f;
   return 0;
 }
 
