@@ -1504,14 +1504,18 @@ void Batch::setup() {
                        d_sub, (const SolvDev*)d_solv, xyz, (const double*)sv_brad, (const double*)sv_hbw,
                        (const double*)sv_adet, sv_B);
   }
-  // initial charges: EEQ (default) or SAD
+  // initial charges: EEQ (default) or SAD.  The EEQ solve runs on stream2
+  // beside the overlap integrals and S^(-1/2) (it reads only CN, zsh and
+  // geometry, writes only its own system and the charges); joined below.
+  std::vector<int> big;
   if (opt.guess == 0) {
+    HIPCHECK(hipEventRecord(ev_fork, stream));
+    HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
     const long long ne = sub_max_n + 1;
-    hipLaunchKernelGGL(k_eeq_matrix, dim3(grid1(ne * ne, 256), ns), dim3(256), 0, stream, d_sub, el, at, xyz, cn_eeq,
-                       E, xE);
-    hipLaunchKernelGGL(k_eeq_reset, dim3(grid1(ns, 64)), dim3(64), 0, stream, d_sub, ns, eeq_failed);
+    hipLaunchKernelGGL(k_eeq_matrix, dim3(grid1(ne * ne, 256), ns), dim3(256), 0, stream2, d_sub, el, at, xyz,
+                       cn_eeq, E, xE);
+    hipLaunchKernelGGL(k_eeq_reset, dim3(grid1(ns, 64)), dim3(64), 0, stream2, d_sub, ns, eeq_failed);
     // large systems: one launch pair per column over the whole device
-    std::vector<int> big;
     int big_max = 0;
     for (int k = 0; k < ns; ++k)
       if (desc[sel_[k]].ne >= kEeqBigNe) {
@@ -1519,23 +1523,29 @@ void Batch::setup() {
         big_max = std::max(big_max, desc[sel_[k]].ne);
       }
     if (!big.empty()) {
-      HIPCHECK(hipMemcpyAsync(eeq_big, big.data(), big.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+      HIPCHECK(hipMemcpyAsync(eeq_big, big.data(), big.size() * sizeof(int), hipMemcpyHostToDevice, stream2));
       const int nb = (int)big.size();
       for (int k = 0; k < big_max; ++k) {
-        hipLaunchKernelGGL(k_eeq_lu_pivot, dim3(nb), dim3(256), 0, stream, d_sub, (const int*)eeq_big, k, E, xE,
+        hipLaunchKernelGGL(k_eeq_lu_pivot, dim3(nb), dim3(256), 0, stream2, d_sub, (const int*)eeq_big, k, E, xE,
                            eeq_failed);
         if (big_max - k - 1 > 0)
-          hipLaunchKernelGGL(k_eeq_lu_update, dim3(big_max - k - 1, nb), dim3(256), 0, stream, d_sub,
+          hipLaunchKernelGGL(k_eeq_lu_update, dim3(big_max - k - 1, nb), dim3(256), 0, stream2, d_sub,
                              (const int*)eeq_big, k, E, (const int*)eeq_failed);
       }
     }
-    hipLaunchKernelGGL(k_eeq_gesv, dim3(ns), dim3(256), 0, stream, d_sub, el, at, at_sh0, E, xE, yE, q, qsh,
+    hipLaunchKernelGGL(k_eeq_gesv, dim3(ns), dim3(256), 0, stream2, d_sub, el, at, at_sh0, E, xE, yE, q, qsh,
                        (const double*)zsh, eeq_failed, kEeqBigNe);
   } else {
     hipLaunchKernelGGL(k_sad_guess, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, at_sh0, q,
                        qsh, (const double*)zsh);
   }
   hipLaunchKernelGGL(k_zero_atoms, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, dipm, qp);
+  // S-orthonormal initial orbital basis C = S^(-1/2)
+  orthogonalizer();
+  if (opt.guess == 0) {
+    HIPCHECK(hipEventRecord(ev_join, stream2));
+    HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
+  }
   // restart guesses replace the initial charges (src/api/interface.f90:83-101, src/scf_module.F90:292)
   for (int m : sel_) {
     if (guess_qsh_.empty() || guess_qsh_[m].empty()) continue;
@@ -1550,8 +1560,6 @@ void Batch::setup() {
                             stream));
     HIPCHECK(hipStreamSynchronize(stream));
   }
-  // S-orthonormal initial orbital basis C = S^(-1/2)
-  orthogonalizer();
   HIPCHECK(hipGetLastError());
   setup_done_ = true;
 }
@@ -1646,12 +1654,14 @@ void Batch::gradient() {
   hipLaunchKernelGGL(k_zero_atoms, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, grad,
                      (double*)nullptr);
   const dim3 gat(sub_max_n, ns);
+  // the H0 pair pass (writes only the pair buffer) on stream2 beside the AES
+  // and D4 gradients; its per-atom sums follow them
+  HIPCHECK(hipEventRecord(ev_fork, stream));
+  HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
   hipLaunchKernelGGL(k_grad_h0_pairs, dim3(grid1((long long)sub_max_n * (sub_max_n - 1) / 2, 64), ns), dim3(64), 0,
-                     stream, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)selfe, (const double*)dsedcn,
+                     stream2, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)selfe, (const double*)dsedcn,
                      (const double*)P, (const double*)T, (const double*)H0n, (const double*)S, (const double*)shift,
                      (const double*)vs, (const double*)vd, (const double*)vq, intcut, gpair);
-  hipLaunchKernelGGL(k_grad_h0, gat, dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)dsedcn,
-                     (const double*)P, (const double*)gpair, grad, dhdcn);
   hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_sub, xyz, q, dipm, qp, radcn, gab3, gab5, grad, tmp2);
   hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, cn_cov, q,
                      zetavec, zerovec, zetadcn, zerodcn);
@@ -1661,6 +1671,10 @@ void Batch::gradient() {
                      grad, dedcn);
   hipLaunchKernelGGL(k_d4_atm, gat, dim3(256), 0, stream, d_sub, el, at, xyz, (const double*)c60, (const double*)dc60,
                      grad, dedcn, eat);
+  HIPCHECK(hipEventRecord(ev_join, stream2));
+  HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
+  hipLaunchKernelGGL(k_grad_h0, gat, dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)dsedcn,
+                     (const double*)P, (const double*)gpair, grad, dhdcn);
   hipLaunchKernelGGL(k_grad_pairs, gat, dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0, (const double*)qsh, grad,
                      erep);
   hipLaunchKernelGGL(k_cn_chain, gat, dim3(64), 0, stream, d_sub, el, at, xyz, (const double*)cn,
