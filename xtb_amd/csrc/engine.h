@@ -267,12 +267,12 @@ class Batch {
   int* sv_failed = nullptr;
   double* ns_c = nullptr;
   unsigned long long* ns_res = nullptr;
-  int *ns_stage = nullptr, *ns_act = nullptr, *ns_failed = nullptr;
+  int *ns_stage = nullptr, *ns_act = nullptr, *ns_failed = nullptr, *ns_fin = nullptr;
   int* h_pinned = nullptr;
   long long jacobi_rounds_ = 0;
   long long oa_steps_ = 0;
   const Element* elements_dev = nullptr;
-  int ntiles = 0;
+  int ntiles = 0, ntiles_sym = 0;   // d_tiles: all tiles, then the b >= a tiles of symmetric products
   int4* d_tiles = nullptr;
   // returns true when W comes out as rows (row j = eigenvector j)
   void jacobi_lanes(JacSet& js, double* Am, double* Wm);
@@ -299,8 +299,9 @@ class Batch {
   int* oa_act = nullptr;
   int* oa_ok = nullptr;
   unsigned long long* oa_stat = nullptr;
+  // sym: the product is symmetric (computed on the b >= a tiles, mirrored)
   void gemm(int kind, const double* a, const double* b, double* c, const double* scale, const int* perm,
-            bool check_active, double alpha = 1.0, const int* mask = nullptr);
+            bool check_active, double alpha = 1.0, const int* mask = nullptr, bool sym = false);
   void orthogonalizer();   // C = S^(-1/2) by Newton-Schulz iterations
   void eeq_solve();        // EEQ charges by Gaussian elimination
 

@@ -315,17 +315,20 @@ __global__ void __launch_bounds__(256) k_ns_form(const MolDesc* __restrict__ des
 }
 
 // per-molecule convergence: residual < 1e-10, then one more update (quadratic
-// convergence takes it to rounding level); act = 0 once done
+// convergence takes it to rounding level); act = 0 once done, and fin = the
+// parity of the iteration whose buffers hold the molecule's final Y, Z
 __global__ void __launch_bounds__(256) k_ns_mark(int nmol, int* stage, int* act, unsigned long long* res, int iter,
-                                                 int maxiter, int* failed, int* nactive) {
+                                                 int maxiter, int* failed, int* nactive, int* fin) {
   __shared__ double sh[4];
   double c = 0.0;
   for (int m = threadIdx.x; m < nmol; m += blockDim.x) {
     const double r = __longlong_as_double((long long)res[m]);
     res[m] = 0ull;
+    const int before = stage[m];
     if (stage[m] == 1) stage[m] = 2;
     else if (stage[m] == 0 && r < 1e-10) stage[m] = 1;
     if (stage[m] == 0 && iter >= maxiter) { stage[m] = 2; failed[m] = 1; }
+    if (before < 2 && stage[m] == 2) fin[m] = iter & 1;
     act[m] = stage[m] < 2 ? 1 : 0;
     c += act[m];
   }
@@ -365,12 +368,14 @@ __global__ void k_copy_mask(const MolDesc* __restrict__ desc, const int* __restr
   if (t < n2) b[md.m0 + t] = a[md.m0 + t];
 }
 
-__global__ void k_ns_final(const MolDesc* __restrict__ desc, const double* __restrict__ Z,
+__global__ void k_ns_final(const MolDesc* __restrict__ desc, const double* __restrict__ Z0,
+                           const double* __restrict__ Z1, const int* __restrict__ fin,
                            const double* __restrict__ cbound, double* C) {
   const MolDesc md = desc[blockIdx.y];
   const long long N = md.nao;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N * N) return;
+  const double* Z = fin[md.mol] ? Z1 : Z0;
   C[md.m0 + t] = Z[md.m0 + t] / sqrt(cbound[md.mol]);
 }
 
@@ -672,6 +677,7 @@ void Batch::allocate(const Capacity& c) {
   oa_stat = alloc<unsigned long long>(4LL * nmol);
   ns_res = alloc<unsigned long long>(nmol);
   ns_stage = alloc<int>(nmol);
+  ns_fin = alloc<int>(nmol);
   ns_act = alloc<int>(nmol);
   ns_failed = alloc<int>(nmol);
   // Jacobi set: per-slot problem table and state; work lists grow on demand
@@ -829,15 +835,20 @@ void Batch::rebuild() {
     ptr = alloc<T_>(capv);
   };
   // GEMM tiles over (molecule, 64x64 tile)
-  std::vector<int4> tl;
+  std::vector<int4> tl, ts;
   for (int m = 0; m < nmol; ++m) {
     if (!occupied_[m]) continue;
     const int nt = (desc[m].nao + 63) / 64;
     for (int a = 0; a < nt; ++a)
-      for (int b = 0; b < nt; ++b) tl.push_back(make_int4(m, a, b, 0));
+      for (int b = 0; b < nt; ++b) {
+        tl.push_back(make_int4(m, a, b, 0));
+        if (b >= a) ts.push_back(make_int4(m, a, b, 1));
+      }
   }
   ntiles = (int)tl.size();
-  grow(d_tiles, tiles_cap_, std::max<long long>(1, ntiles));
+  ntiles_sym = (int)ts.size();
+  tl.insert(tl.end(), ts.begin(), ts.end());
+  grow(d_tiles, tiles_cap_, std::max<long long>(1, ntiles + ntiles_sym));
   if (ntiles)
     HIPCHECK(hipMemcpyAsync(d_tiles, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice, stream));
   // Jacobi problems: dim nao, matrices at m0, eigenvalues at o0
@@ -1325,7 +1336,10 @@ void Batch::set_guess(int m, const double* qsh_in, const double* dipm_in, const 
 }
 
 void Batch::gemm(int kind, const double* a, const double* b, double* c, const double* scale_, const int* perm_,
-                 bool check_active, double alpha, const int* mask) {
+                 bool check_active, double alpha, const int* mask, bool sym) {
+  const int4* tiles = sym ? d_tiles + ntiles : d_tiles;
+  const int nt = sym ? ntiles_sym : ntiles;
+  if (nt <= 0) return;
   // SCC-active molecules: the 'active' field of MolState, read with its stride
   const int* m = mask;
   int ms = 1;
@@ -1337,11 +1351,11 @@ void Batch::gemm(int kind, const double* a, const double* b, double* c, const do
 #define XTBAMD_GEMM(K, TA, TB, SC, PM)                                                                              \
   case K:                                                                                                          \
     if (use_mfma)                                                                                                  \
-      hipLaunchKernelGGL((k_gemm_mfma<TA, TB, SC, PM>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b, \
-                         c, scale_, perm_, alpha, m, ms);                                                           \
-    else                                                                                                           \
-      hipLaunchKernelGGL((k_gemm<TA, TB, SC, PM>), dim3(ntiles), dim3(256), 0, stream, d_tiles, d_desc, a, b, c,   \
+      hipLaunchKernelGGL((k_gemm_mfma<TA, TB, SC, PM>), dim3(nt), dim3(256), 0, stream, tiles, d_desc, a, b, c,  \
                          scale_, perm_, alpha, m, ms);                                                              \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_gemm<TA, TB, SC, PM>), dim3(nt), dim3(256), 0, stream, tiles, d_desc, a, b, c, scale_, \
+                         perm_, alpha, m, ms);                                                                      \
     break;
   switch (kind) {
     XTBAMD_GEMM(0, false, false, false, false)
@@ -1441,21 +1455,25 @@ void Batch::orthogonalizer() {
   hipLaunchKernelGGL(k_ns_begin, dim3(grid1(nmol, 64)), dim3(64), 0, stream, nmol, (const int*)phase_act, ns_stage,
                      ns_act, ns_failed, ns_res);
   const int maxit = 60;
+  // Y and Z alternate between two buffers each (A/H and W/T) by iteration
+  // parity; a molecule's final pair is at the parity recorded when it stops
+  double* Yb[2] = {A, H};
+  double* Zb[2] = {W, T};
   for (int it = 0; it <= maxit; ++it) {
-    // Y in A, Z in W
-    gemm(0, W, A, P, nullptr, nullptr, false, 1.0, ns_act);           // P = Z Y
+    double *Y = Yb[it & 1], *Z = Zb[it & 1];
+    // Y and Z are polynomials in S: they commute, and Z Y, Y T and T Z are symmetric
+    gemm(0, Z, Y, P, nullptr, nullptr, false, 1.0, ns_act, true);     // P = Z Y
     hipLaunchKernelGGL(k_ns_form, gmm, dim3(256), 0, stream, d_sub, (const int*)ns_act, P, ns_res);
     hipLaunchKernelGGL(k_ns_mark, dim3(1), dim3(256), 0, stream, nmol, ns_stage, ns_act, ns_res, it, maxit, ns_failed,
-                       jac_nactive);
+                       jac_nactive, ns_fin);
     HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
     if (h_pinned[0] <= 0) break;
-    gemm(0, A, P, H, nullptr, nullptr, false, 1.0, ns_act);           // Y T
-    gemm(0, P, W, T, nullptr, nullptr, false, 1.0, ns_act);           // T Z
-    hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_sub, (const int*)ns_act, (const double*)H, A);
-    hipLaunchKernelGGL(k_copy_mask, gmm, dim3(256), 0, stream, d_sub, (const int*)ns_act, (const double*)T, W);
+    gemm(0, Y, P, Yb[(it + 1) & 1], nullptr, nullptr, false, 1.0, ns_act, true);   // Y T
+    gemm(0, P, Z, Zb[(it + 1) & 1], nullptr, nullptr, false, 1.0, ns_act, true);   // T Z
   }
-  hipLaunchKernelGGL(k_ns_final, gmm, dim3(256), 0, stream, d_sub, (const double*)W, (const double*)ns_c, C);
+  hipLaunchKernelGGL(k_ns_final, gmm, dim3(256), 0, stream, d_sub, (const double*)Zb[0], (const double*)Zb[1],
+                     (const int*)ns_fin, (const double*)ns_c, C);
 }
 
 void Batch::setup() {
@@ -1586,7 +1604,7 @@ bool Batch::scc_iteration() {
                      D, Q, shift, vs, vd, vq, ao_sh, ao_at, thr2, H, (const MolState*)d_state);
   // generalized eigenproblem in the current orbital basis C (S-orthonormal)
   gemm(0, H, C, T, nullptr, nullptr, true);   // T = H C
-  gemm(1, C, T, A, nullptr, nullptr, true);   // A = C^T H C
+  gemm(1, C, T, A, nullptr, nullptr, true, 1.0, nullptr, true);   // A = C^T H C (symmetric)
   // C holds the previous eigenvectors from the second iteration on: A is
   // nearly diagonal, refine W = I towards its eigenvectors with GEMMs first
   const bool warm = iterations_ > 0 && oa_max_steps > 0;
@@ -1608,7 +1626,7 @@ bool Batch::scc_iteration() {
                      d_desc, (const MolState*)d_state, (const double*)T, C);
   hipLaunchKernelGGL(k_occ_fermi, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, (const double*)emo, focca, foccb,
                      focc, opt.etemp, (const double*)qsh, (const double*)dipm, (const double*)qp, q_in);
-  gemm(2, C, C, P, focc, nullptr, true);      // P = C diag(f) C^T
+  gemm(2, C, C, P, focc, nullptr, true, 1.0, nullptr, true);      // P = C diag(f) C^T (symmetric)
   hipLaunchKernelGGL(k_population, dim3(grid1(max_nao, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)P,
                      (const double*)S, (const double*)H0, (const double*)D, (const double*)Q, row_q, row_e, row_d,
                      row_qp, (const MolState*)d_state, 0);
@@ -1648,7 +1666,7 @@ void Batch::gradient() {
   // energy-weighted density into T
   hipLaunchKernelGGL(k_pew_scale, dim3(grid1(sub_max_nao, 64), ns), dim3(64), 0, stream, d_sub, (const double*)focc,
                      (const double*)emo, row_pd);
-  gemm(2, C, C, T, row_pd, nullptr, false, 1.0, phase_act);
+  gemm(2, C, C, T, row_pd, nullptr, false, 1.0, phase_act, true);   // symmetric
   hipLaunchKernelGGL(k_aes_pot, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, q, dipm, qp,
                      gab3, gab5, 0, vs, vd, vq, (const MolState*)nullptr);
   hipLaunchKernelGGL(k_zero_atoms, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, grad,

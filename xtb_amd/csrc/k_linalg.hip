@@ -16,7 +16,11 @@ constexpr int kTile = 64, kBK = 16;
 
 // C = alpha op(A) * op(B) [* diag(scale) on the k index] for square nao x nao
 // matrices; optional column permutation of B (B(:, perm[j])).  Molecules with
-// mask[m * mstride] == 0 are skipped (mask may be null).
+// mask[m * mstride] == 0 are skipped (mask may be null).  A tile entry
+// (m, a, b, sym) with sym = 1 is one of a product known to be symmetric: the
+// list holds only tiles b >= a, each tile writes its elements and their
+// mirrors (a diagonal tile its upper triangle), so C comes out exactly
+// symmetric from half the work.
 template <bool TA, bool TB, bool SCALE, bool PERM>
 __global__ void __launch_bounds__(256) k_gemm(const int4* __restrict__ tiles, const MolDesc* __restrict__ desc,
                                               const double* __restrict__ A, const double* __restrict__ B, double* C,
@@ -76,6 +80,7 @@ __global__ void __launch_bounds__(256) k_gemm(const int4* __restrict__ tiles, co
     }
     __syncthreads();
   }
+  const bool sym = tl.w != 0, dtile = tl.y == tl.z;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int gi = row0 + ty + 16 * r;
@@ -83,7 +88,13 @@ __global__ void __launch_bounds__(256) k_gemm(const int4* __restrict__ tiles, co
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const int gj = col0 + tx + 16 * c;
-      if (gj < N) Cm[(long long)gi * N + gj] = alpha * acc[r][c];
+      if (gj >= N) continue;
+      if (!sym) {
+        Cm[(long long)gi * N + gj] = alpha * acc[r][c];
+      } else if (!dtile || gi <= gj) {
+        Cm[(long long)gi * N + gj] = alpha * acc[r][c];
+        Cm[(long long)gj * N + gi] = alpha * acc[r][c];
+      }
     }
   }
 }
@@ -178,6 +189,7 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(const int4* __restrict__ tile
     __syncthreads();
     buf ^= 1;
   }
+  const bool sym = tl.w != 0, dtile = tl.y == tl.z;
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -186,7 +198,13 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(const int4* __restrict__ tile
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int gi = row0 + 32 * wr + 16 * r + k4 + 4 * i;
-        if (gi < N && gj < N) Cm[(long long)gi * N + gj] = alpha * acc[r][c][i];
+        if (gi >= N || gj >= N) continue;
+        if (!sym) {
+          Cm[(long long)gi * N + gj] = alpha * acc[r][c][i];
+        } else if (!dtile || gi <= gj) {
+          Cm[(long long)gi * N + gj] = alpha * acc[r][c][i];
+          Cm[(long long)gj * N + gi] = alpha * acc[r][c][i];
+        }
       }
     }
 }
