@@ -263,19 +263,21 @@ def test_block_jacobi_batch_same_alone_and_in_mix():
 
 
 def test_jacobi_lanes_do_not_change_results(monkeypatch):
-    """The block-Jacobi rounds on one stream lane and on two (problems split
-    between two streams, XCD-ordered work lists): bitwise the same results."""
+    """The block-Jacobi rounds on one, two and three stream lanes (problems
+    split between the streams, XCD-ordered work lists): bitwise the same
+    results."""
     xa = _engine()
     from xtb_amd import workloads as w
     mols = [w.water_cluster(12, 5), w.alkane(20, 6), w.alkane(45, 8), w.water_cluster(30, 9), w.alkane(33, 3)]
     runs = []
-    for lanes in ("1", "2"):
+    for lanes in ("1", "2", "3"):
         monkeypatch.setenv("XTBAMD_JAC_LANES", lanes)
         b = xa.Batch([(z, x, 0.0, 0) for z, x in mols]).run()
         runs.append(b.results())
         b.close()
-    for k, (r1, r2) in enumerate(zip(*runs)):
+    for k, (r1, r2, r3) in enumerate(zip(*runs)):
         _same(r1, r2, f"mol{k} nao={r1['nao']}")
+        _same(r1, r3, f"mol{k} nao={r1['nao']}")
 
 
 def test_server_results_same_alone_and_in_mix():

@@ -106,6 +106,9 @@ struct JacobiProb {
   long long src_off;          // the matrix in the caller's packed layout (row stride dim)
 };
 
+// stream lanes of the block-Jacobi rounds (at most)
+constexpr int kMaxLanes = 3;
+
 // A set of Jacobi problems (one per molecule) with its device work lists.
 struct JacSet {
   std::vector<JacobiProb> probs;
@@ -117,12 +120,12 @@ struct JacSet {
   int nto = 0;
   // the work lists hold lane 0's items first: lane g owns pairs[gpairs[g],
   // gpairs[g+1]) and likewise for the tile lists
-  int gpairs[3] = {0, 0, 0}, gta[3] = {0, 0, 0}, gtw[3] = {0, 0, 0}, gto[3] = {0, 0, 0};
+  int gpairs[kMaxLanes + 1] = {}, gta[kMaxLanes + 1] = {}, gtw[kMaxLanes + 1] = {}, gto[kMaxLanes + 1] = {};
   // XCD-ordered lists of the lanes: pivots (m, k) and tiles (m, y, z, kind:
   // 0 A tile (k, l), 1 W tile (rb, l), 2 skip); lane g owns [gxp[g], gxp[g+1])
   int2* d_xp = nullptr;
   int4* d_xt = nullptr;
-  int gxp[3] = {0, 0, 0}, gxt[3] = {0, 0, 0};
+  int gxp[kMaxLanes + 1] = {}, gxt[kMaxLanes + 1] = {};
   int* d_big = nullptr;   // problems above 64 (tridiagonal QL path)
   // padded working copies of A and W (row stride ld, every row 128-byte aligned)
   double *ja = nullptr, *jw = nullptr;
@@ -183,16 +186,17 @@ class Batch {
   // --- layout (public for the kernel launchers) ---
   Options opt;
   hipStream_t stream;
-  // second lane of the block-Jacobi rounds: half of the problems run their
-  // rounds on stream2, beside the other half on stream
+  // lanes of the block-Jacobi rounds: the problems are split between
+  // jac_lanes streams (stream, stream2, stream4) that run their rounds side by
+  // side; stream2 also carries other work beside stream outside the rounds
   hipStream_t stream2 = nullptr;
   hipStream_t stream3 = nullptr;   // the small (dim <= 64) eigenproblems beside the lanes
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr;
-  bool two_lanes = true;
+  hipStream_t stream4 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr, ev_join4 = nullptr;
+  int jac_lanes = 3;
   // the block-Jacobi problems work on padded copies (rows 128-byte aligned);
   // the register path only (no refinement, banded rounds, LDS kernels or QL)
   bool jac_padded = true;
-  bool lanes_one_stream = false;   // both lanes on stream, interleaved
   int nmol = 0, max_n = 0, max_nsh = 0, max_nao = 0, max_nd4 = 0, max_nbr = 0;
   double intcut = 25.0, neglect = 1e-8, scfconv = 1e-6, qconv = 1e-4;
   std::vector<MolDesc> desc;

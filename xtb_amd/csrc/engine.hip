@@ -581,16 +581,15 @@ void Batch::init_common() {
   jac_tol_mid = jac_tol;
   if (const char* r = std::getenv("XTBAMD_JAC_TOL_MID")) jac_tol_mid = std::max(jac_tol, std::atof(r));
   trace_rounds = std::getenv("XTBAMD_TRACE_ROUNDS") != nullptr;
-  if (const char* r = std::getenv("XTBAMD_JAC_LANES")) {
-    two_lanes = std::atoi(r) != 1;
-    lanes_one_stream = std::atoi(r) == 3;
-  }
+  if (const char* r = std::getenv("XTBAMD_JAC_LANES")) jac_lanes = std::min(kMaxLanes, std::max(1, std::atoi(r)));
   // default priority (as every stream here), no implicit ordering with the null stream
   HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
   HIPCHECK(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
+  HIPCHECK(hipStreamCreateWithFlags(&stream4, hipStreamNonBlocking));
   HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
   HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   HIPCHECK(hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&ev_join4, hipEventDisableTiming));
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
   if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
   jac_padded = jac_p == 32 && !apply_lds && !pivot_lds && oa_max_steps == 0 && banded_passes == 0 && !use_ql;
@@ -663,7 +662,7 @@ void Batch::allocate(const Capacity& c) {
   perm = alloc<int>(tao);
   Sm = alloc<double>(tmm);
   G = alloc<double>(tmm);
-  jac_nactive = alloc<int>(2);
+  jac_nactive = alloc<int>(kMaxLanes);
   d_state_stage = alloc<MolState>(nmol);
   eeq_failed = alloc<int>(nmol);
   eeq_big = alloc<int>(nmol);
@@ -886,20 +885,22 @@ void Batch::rebuild() {
     qoff += (long long)(p.nbp / 2) * jac_p * jac_p;
     js.max_rounds = std::max(js.max_rounds, p.rounds);
   }
-  // two lanes of equal tile-update bytes per round (dim^2): largest problems
-  // first, each to the lane with less so far
+  // lanes of equal tile-update bytes per round (dim^2): largest problems
+  // first, each to the lane with least so far
   {
     std::vector<int> order(big);
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return js.probs[x].dim > js.probs[y].dim; });
-    double load[2] = {0.0, 0.0};
+    double load[kMaxLanes] = {};
     for (int m : order) {
-      const int g = (two_lanes && load[1] < load[0]) ? 1 : 0;
+      int g = 0;
+      for (int h = 1; h < jac_lanes; ++h)
+        if (load[h] < load[g]) g = h;
       js.probs[m].group = g;
       load[g] += (double)js.probs[m].dim * js.probs[m].dim;
     }
   }
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < kMaxLanes; ++g) {
     js.gpairs[g] = (int)pairs.size();
     js.gta[g] = (int)ta.size();
     js.gtw[g] = (int)tw.size();
@@ -919,10 +920,10 @@ void Batch::rebuild() {
         for (int b = a; b < nrb; ++b) to.push_back(make_int4(m, a, b, 0));
     }
   }
-  js.gpairs[2] = (int)pairs.size();
-  js.gta[2] = (int)ta.size();
-  js.gtw[2] = (int)tw.size();
-  js.gto[2] = (int)to.size();
+  js.gpairs[kMaxLanes] = (int)pairs.size();
+  js.gta[kMaxLanes] = (int)ta.size();
+  js.gtw[kMaxLanes] = (int)tw.size();
+  js.gto[kMaxLanes] = (int)to.size();
   // XCD-aware round lists of the lanes: workgroup b of a launch runs on XCD
   // b mod kXcds, so a problem placed on XCD x has its pivots and its A and W
   // tiles at positions = x (mod kXcds) and its pivot factors Q are fetched by
@@ -930,7 +931,7 @@ void Batch::rebuild() {
   // largest first; short rows are padded with skip entries (m = -1).
   std::vector<int2> xp;
   std::vector<int4> xt;
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < kMaxLanes; ++g) {
     js.gxp[g] = (int)xp.size();
     js.gxt[g] = (int)xt.size();
     std::vector<int> pl;
@@ -978,8 +979,8 @@ void Batch::rebuild() {
     for (size_t j = 0; j < mt; ++j)
       for (int x = 0; x < kXcds; ++x) xt.push_back(j < qt[x].size() ? qt[x][j] : make_int4(-1, 0, 0, 2));
   }
-  js.gxp[2] = (int)xp.size();
-  js.gxt[2] = (int)xt.size();
+  js.gxp[kMaxLanes] = (int)xp.size();
+  js.gxt[kMaxLanes] = (int)xt.size();
   grow(js.d_xp, xp_cap_, std::max<long long>(1, (long long)xp.size()));
   grow(js.d_xt, xt_cap_, std::max<long long>(1, (long long)xt.size()));
   if (!xp.empty()) HIPCHECK(hipMemcpyAsync(js.d_xp, xp.data(), xp.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
@@ -1096,8 +1097,10 @@ Batch::~Batch() {
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (ev_join3) (void)hipEventDestroy(ev_join3);
+  if (ev_join4) (void)hipEventDestroy(ev_join4);
   if (stream2) (void)hipStreamDestroy(stream2);
   if (stream3) (void)hipStreamDestroy(stream3);
+  if (stream4) (void)hipStreamDestroy(stream4);
 }
 
 // ---------------------------------------------------------------------------
@@ -1117,9 +1120,9 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
                      check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0,
                      jac_tol_mid / jac_tol);
-  // two lanes: the problems of lane 1 run their rounds on stream2 beside
-  // those of lane 0; the problems of a lane never touch the other lane's
-  // data, so the results do not depend on the lanes
+  // lanes: the problems of lanes 1 and 2 run their rounds on stream2 and
+  // stream4 beside those of lane 0; the problems of a lane never touch another
+  // lane's data, so the results do not depend on the lanes
   const bool lanes = rows && !pivot_lds && js.npairs > 0;
   // the problems of dimension <= 64 (one workgroup each, k_jac_small) run
   // on stream3 beside the lanes
@@ -1127,6 +1130,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
     HIPCHECK(hipEventRecord(ev_fork, stream));
     HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
     HIPCHECK(hipStreamWaitEvent(stream3, ev_fork, 0));
+    HIPCHECK(hipStreamWaitEvent(stream4, ev_fork, 0));
   }
   if (js.nsmall)
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, lanes ? stream3 : stream, js.d_probs, js.d_small, Am,
@@ -1137,6 +1141,8 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
     HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
     HIPCHECK(hipEventRecord(ev_join3, stream3));
     HIPCHECK(hipStreamWaitEvent(stream, ev_join3, 0));
+    HIPCHECK(hipEventRecord(ev_join4, stream4));
+    HIPCHECK(hipStreamWaitEvent(stream, ev_join4, 0));
   } else if (js.npairs) {
     // banded rounds first (neighbouring blocks only), then cyclic sweeps;
     // every round starts with the convergence test of the problems at a
@@ -1222,11 +1228,12 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
 constexpr int kProfEvery = 8;
 
 void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
-  const hipStream_t ls[2] = {stream, lanes_one_stream ? stream : stream2};
-  bool live[2] = {js.gpairs[0] < js.gpairs[1], js.gpairs[1] < js.gpairs[2]};
+  const hipStream_t ls[kMaxLanes] = {stream, stream2, stream4};
+  bool live[kMaxLanes];
+  for (int g = 0; g < kMaxLanes; ++g) live[g] = js.gpairs[g] < js.gpairs[g + 1];
   unsigned long long* jw = (prof_mask & 4) ? prof_work_ + 2 : (unsigned long long*)nullptr;
   // the initial off-diagonal maxima go to the buffer the round-0 test reads
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < kMaxLanes; ++g) {
     const int nto = js.gto[g + 1] - js.gto[g];
     if (live[g] && nto > 0)
       hipLaunchKernelGGL(k_jac_offmax, dim3(nto), dim3(256), 0, ls[g], js.d_probs, js.d_to + js.gto[g],
@@ -1234,7 +1241,7 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   }
   const int rmax = (kMaxSweeps + 1) * js.max_rounds;
   for (int r = 0; r <= rmax; ++r) {
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < kMaxLanes; ++g) {
       if (!live[g]) continue;
       // timing events (and the flop count) on every kProfEvery-th round only:
       // an event pair per launch would cost more host time than the launch
@@ -1247,18 +1254,24 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       if (sample) prof_end(2, nullptr, 0, ls[g]);
     }
     if (r % js.max_rounds == 0) {
-      for (int g = 0; g < 2; ++g)
+      for (int g = 0; g < kMaxLanes; ++g)
         if (live[g])
           hipLaunchKernelGGL(k_jac_count, dim3(1), dim3(256), 0, ls[g], js.d_probs, nmol, (const int*)js.state, g,
                              jac_nactive + g);
       HIPCHECK(hipEventRecord(ev_join, stream2));
       HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
-      HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHECK(hipEventRecord(ev_join4, stream4));
+      HIPCHECK(hipStreamWaitEvent(stream, ev_join4, 0));
+      HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, kMaxLanes * sizeof(int), hipMemcpyDeviceToHost, stream));
       HIPCHECK(hipStreamSynchronize(stream));
-      for (int g = 0; g < 2; ++g) live[g] = live[g] && h_pinned[g] > 0;
-      if (!live[0] && !live[1]) break;
+      bool any = false;
+      for (int g = 0; g < kMaxLanes; ++g) {
+        live[g] = live[g] && h_pinned[g] > 0;
+        any = any || live[g];
+      }
+      if (!any) break;
     }
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < kMaxLanes; ++g) {
       if (!live[g]) continue;
       hipLaunchKernelGGL(k_jac_apply_R, dim3(js.gxt[g + 1] - js.gxt[g]), dim3(64), 0, ls[g], js.d_probs,
                          js.d_xt + js.gxt[g], r, Am, Wm, (const double*)js.qbuf, (const int*)js.rot,
