@@ -1599,10 +1599,14 @@ bool Batch::scc_iteration() {
   const long long rounds_before = jacobi_rounds_;
   const Element* el = elements_dev;
   const double thr2 = neglect * 1.0e-2;
+  // the AES potentials (read only charges and moments, write only vs, vd,
+  // vq) on stream2 beside the shell and D4 potentials; joined before the Fock build
+  HIPCHECK(hipEventRecord(ev_fork, stream));
+  HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
+  hipLaunchKernelGGL(k_aes_pot, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream2, d_desc, el, at, xyz, q, dipm, qp,
+                     gab3, gab5, 1, vs, vd, vq, (const MolState*)d_state);
   hipLaunchKernelGGL(k_shell_pot, dim3(grid1(max_nsh, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)J,
                      (const double*)qsh, (const double*)shgam, shift, 1, (const MolState*)d_state);
-  hipLaunchKernelGGL(k_aes_pot, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, q, dipm, qp,
-                     gab3, gab5, 1, vs, vd, vq, (const MolState*)d_state);
   hipLaunchKernelGGL(k_d4_zeta, dim3(grid1(max_nd4, 64), nmol), dim3(64), 0, stream, d_desc, el, at, d4_at, d4_ref,
                      gw, q, zv, dzv, (const MolState*)d_state);
   hipLaunchKernelGGL(k_d4_symv, dim3(grid1(max_nd4, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)dispmat,
@@ -1613,6 +1617,8 @@ bool Batch::scc_iteration() {
   hipLaunchKernelGGL(k_shift_final, dim3(grid1(max_nsh, 64), nmol), dim3(64), 0, stream, d_desc, el, at, sh_at,
                      (const double*)d4dum, (const double*)dzv, solv_on_ ? (const double*)sv_pot : nullptr, shift,
                      (const MolState*)d_state);
+  HIPCHECK(hipEventRecord(ev_join, stream2));
+  HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
   hipLaunchKernelGGL(k_fock, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream, d_desc, S, H0,
                      D, Q, shift, vs, vd, vq, ao_sh, ao_at, thr2, H, (const MolState*)d_state);
   // generalized eigenproblem in the current orbital basis C (S-orthonormal)
@@ -1646,10 +1652,13 @@ bool Batch::scc_iteration() {
   hipLaunchKernelGGL(k_pop_reduce, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, at_sh0,
                      at_ao0, (const double*)zsh, (const double*)row_q, (const double*)row_d, (const double*)row_qp,
                      qsh, q, dipm, qp, (const MolState*)d_state, 0);
+  // the AES energy terms on stream2 beside the shell and D4 energy terms
+  HIPCHECK(hipEventRecord(ev_fork, stream));
+  HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
+  hipLaunchKernelGGL(k_aes_energy, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream2, d_desc, el, at, xyz, q, dipm,
+                     qp, gab3, gab5, ae01, (const MolState*)d_state);
   hipLaunchKernelGGL(k_shell_pot, dim3(grid1(max_nsh, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)J,
                      (const double*)qsh, (const double*)shgam, jq, 0, (const MolState*)d_state);
-  hipLaunchKernelGGL(k_aes_energy, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream, d_desc, el, at, xyz, q, dipm,
-                     qp, gab3, gab5, ae01, (const MolState*)d_state);
   hipLaunchKernelGGL(k_d4_zeta, dim3(grid1(max_nd4, 64), nmol), dim3(64), 0, stream, d_desc, el, at, d4_at, d4_ref,
                      gw, q, zv, (double*)nullptr, (const MolState*)d_state);
   hipLaunchKernelGGL(k_d4_symv, dim3(grid1(max_nd4, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)dispmat,
@@ -1657,6 +1666,8 @@ bool Batch::scc_iteration() {
   if (solv_on_)
     hipLaunchKernelGGL(k_solv_energy, dim3(nmol), dim3(256), 0, stream, d_desc, (const double*)sv_B, (const double*)q,
                        sv_qbq, (const MolState*)d_state);
+  HIPCHECK(hipEventRecord(ev_join, stream2));
+  HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
   hipLaunchKernelGGL(k_scc_control, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, (const double*)jq,
                      (const double*)shgam, (const double*)row_e, (const double*)ae01, (const double*)zv,
                      (const double*)d4dum, solv_on_ ? (const double*)sv_qbq : nullptr, (const double*)sv_gsasa,
