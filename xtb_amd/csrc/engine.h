@@ -69,6 +69,9 @@ struct MoleculeInput {
 struct HostMol {
   std::vector<int> at, sh_at, sh_l, sh_ao0, sh_m, at_sh0, at_ao0, ao_at, ao_sh, d4_at, d4_ref;
   std::vector<double> xyz;
+  // atom pairs (i, j), i > j, grouped by element pair (so that a wave of the
+  // pair kernels runs one code path), n (n - 1) / 2 entries
+  std::vector<int2> pairs;
 };
 
 // First-fit allocator of ranges in [0, cap) (one per packed device array).
@@ -252,7 +255,8 @@ class Batch {
   double *q_in = nullptr, *dq = nullptr, *qlast = nullptr, *dqlast = nullptr;
   double *bdf = nullptr, *bu = nullptr, *ba = nullptr, *bomega = nullptr;
   double *yE = nullptr, *erep = nullptr, *wbo = nullptr;
-  double* gpair = nullptr;   // per atom pair H0 gradient terms: 5 doubles at 5 * (p0 + i n + j)
+  double* gpair = nullptr;
+  int2* pair_list = nullptr;   // the molecules' HostMol::pairs at p0   // per atom pair H0 gradient terms: 5 doubles at 5 * (p0 + i n + j)
   // Jacobi bookkeeping
   JacSet jac_n;
   int* jac_nactive = nullptr;

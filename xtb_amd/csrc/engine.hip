@@ -529,6 +529,30 @@ void Batch::describe(const MoleculeInput& mi, int maxiter, MolDesc& d, HostMol& 
     zsum += e.zval;
     for (int k = 0; k < 3; ++k) hm.xyz.push_back(mi.xyz[3 * i + k]);
   }
+  // atom pairs i > j grouped by element pair (groups in the order of their
+  // first pair in lower-triangle order, lower-triangle order inside a group;
+  // a counting sort, linear in the number of pairs)
+  {
+    std::vector<int> group(((size_t)XTB_MAX_Z + 1) * (XTB_MAX_Z + 1), -1);
+    std::vector<long long> count;
+    for (int i = 1; i < d.n; ++i)
+      for (int j = 0; j < i; ++j) {
+        int& g = group[(size_t)hm.at[i] * (XTB_MAX_Z + 1) + hm.at[j]];
+        if (g < 0) {
+          g = (int)count.size();
+          count.push_back(0);
+        }
+        count[g] += 1;
+      }
+    std::vector<long long> start(count.size(), 0);
+    for (size_t g = 1; g < count.size(); ++g) start[g] = start[g - 1] + count[g - 1];
+    hm.pairs.resize((size_t)d.n * (d.n - 1) / 2);
+    for (int i = 1; i < d.n; ++i)
+      for (int j = 0; j < i; ++j) {
+        const int g = group[(size_t)hm.at[i] * (XTB_MAX_Z + 1) + hm.at[j]];
+        hm.pairs[start[g]++] = make_int2(i, j);
+      }
+  }
   d.ne = d.n + 1;
   d.charge = mi.charge;
   d.nel = (int)std::lround(zsum - mi.charge);
@@ -652,6 +676,7 @@ void Batch::allocate(const Capacity& c) {
   c6 = alloc<double>(tpp); dc6 = alloc<double>(tpp); c60 = alloc<double>(tpp); dc60 = alloc<double>(tpp);
   wbo = alloc<double>(tpp);
   gpair = alloc<double>(5 * tpp);
+  pair_list = alloc<int2>(tpp);
   E = alloc<double>(std::max(1LL, c.ee));
   xE = alloc<double>(std::max(1LL, c.ne)); yE = alloc<double>(std::max(1LL, c.ne));
   const long long tv = std::max(1LL, c.v);
@@ -756,6 +781,9 @@ void Batch::place(int m, const MolDesc& d0, const HostMol& hm, const MolState& s
     up_i(d4_at, d.d0, hm.d4_at); up_i(d4_ref, d.d0, hm.d4_ref);
     HIPCHECK(hipMemcpyAsync(xyz + 3LL * d.a0, hm.xyz.data(), hm.xyz.size() * sizeof(double), hipMemcpyHostToDevice,
                             stream));
+    if (!hm.pairs.empty())
+      HIPCHECK(hipMemcpyAsync(pair_list + d.p0, hm.pairs.data(), hm.pairs.size() * sizeof(int2), hipMemcpyHostToDevice,
+                              stream));
     HIPCHECK(hipStreamSynchronize(stream));
   }
   for (int z : hm.at) elem_present_[z] = 1;
@@ -1047,6 +1075,11 @@ Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream
   std::vector<double> xall;
   for (const HostMol& h : hms) xall.insert(xall.end(), h.xyz.begin(), h.xyz.end());
   HIPCHECK(hipMemcpy(xyz, xall.data(), xall.size() * sizeof(double), hipMemcpyHostToDevice));
+  {
+    std::vector<int2> pall((size_t)std::max(1LL, cap_.pp), make_int2(0, 0));
+    for (int m = 0; m < nm; ++m) std::copy(hms[m].pairs.begin(), hms[m].pairs.end(), pall.begin() + desc[m].p0);
+    HIPCHECK(hipMemcpy(pair_list, pall.data(), pall.size() * sizeof(int2), hipMemcpyHostToDevice));
+  }
   update_c6ref();
   rebuild();
 }
@@ -1512,7 +1545,7 @@ void Batch::setup() {
   const long long maxpairs = (long long)sub_max_n * (sub_max_n - 1) / 2;
   if (maxpairs > 0)
     hipLaunchKernelGGL(k_sdqh0, dim3(grid1(maxpairs, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
-                       at_ao0, selfe, intcut, S, H0, H0n, D, Q);
+                       at_ao0, selfe, intcut, (const int2*)pair_list, S, H0, H0n, D, Q);
   hipLaunchKernelGGL(k_sdq_onsite, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
                      at_ao0, selfe, intcut, S, H0, H0n, D, Q);
   hipLaunchKernelGGL(k_coulomb, dim3(grid1((long long)sub_max_nsh * sub_max_nsh, 256), ns), dim3(256), 0, stream,
@@ -1703,7 +1736,7 @@ void Batch::gradient() {
   hipLaunchKernelGGL(k_grad_h0_pairs, dim3(grid1((long long)sub_max_n * (sub_max_n - 1) / 2, 64), ns), dim3(64), 0,
                      stream2, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)selfe, (const double*)dsedcn,
                      (const double*)P, (const double*)T, (const double*)H0n, (const double*)S, (const double*)shift,
-                     (const double*)vs, (const double*)vd, (const double*)vq, intcut, gpair);
+                     (const double*)vs, (const double*)vd, (const double*)vq, intcut, (const int2*)pair_list, gpair);
   hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_sub, xyz, q, dipm, qp, radcn, gab3, gab5, grad, tmp2);
   hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, cn_cov, q,
                      zetavec, zerovec, zetadcn, zerodcn);
@@ -1826,7 +1859,7 @@ Capacity server_capacity(double hbm_bytes, int max_atoms, int max_slots, int max
   // bytes per element of each packed array (the buffers allocate() places there)
   const double per[13] = {8.0 * (3 + 4 + 1 + 3 + 6 + 1 + 1 + 3 + 6 + 1 + 3 + 3 + 2 + 4 + 4 * XTB_MAX_REF + 1) + 4.0 * 3,
                           8.0 * 7 + 4.0 * 4, 8.0 * 16 + 4.0 * 3, 8.0 * 4 + 4.0 * 2, 8.0 * 22, 8.0, 8.0,
-                          8.0 * 12, 8.0, 16.0, 32.0, 16.0, 8.0};
+                          8.0 * 13, 8.0, 16.0, 32.0, 16.0, 8.0};
   double bytes = 0.0;
   for (int k = 0; k < 13; ++k) bytes += per[k] * (double)*cap_field(f, k);
   const long long copies = std::max(1LL, (long long)(hbm_bytes / bytes));

@@ -176,12 +176,13 @@ __global__ void __launch_bounds__(64) k_grad_h0_pairs(const MolDesc* __restrict_
                                                       const double* __restrict__ Pew, const double* __restrict__ H0n,
                                                       const double* __restrict__ S, const double* __restrict__ shift,
                                                       const double* __restrict__ vs, const double* __restrict__ vd,
-                                                      const double* __restrict__ vq, double intcut, double* gpair) {
+                                                      const double* __restrict__ vq, double intcut,
+                                                      const int2* __restrict__ pairs, double* gpair) {
   const MolDesc md = desc[blockIdx.y];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)md.n * (md.n - 1) / 2) return;
-  int iat, jat;
-  tri_index(t, iat, jat);
+  const int2 pr = pairs[md.p0 + t];   // grouped by element pair: one code path per wave
+  const int iat = pr.x, jat = pr.y;
   const int N = md.nao;
   const double* x = xyz + 3 * md.a0;
   const double* ri = x + 3 * iat;

@@ -185,17 +185,18 @@ __device__ void pair_dispatch(int li, int lj, const Element& ei, int is, const E
 #undef PB
 }
 
-// one thread per atom pair (i > j)
+// one thread per atom pair (i > j), pairs taken from the molecule's list
+// grouped by element pair (one code path per wave)
 __global__ void k_sdqh0(const MolDesc* __restrict__ desc, const Element* __restrict__ el, const int* __restrict__ at,
                         const double* __restrict__ xyz, const int* __restrict__ at_sh0, const int* __restrict__ at_ao0,
-                        const double* __restrict__ selfe, double intcut, double* S, double* H0, double* H0n, double* D,
-                        double* Q) {
+                        const double* __restrict__ selfe, double intcut, const int2* __restrict__ pairs, double* S,
+                        double* H0, double* H0n, double* D, double* Q) {
   const MolDesc md = desc[blockIdx.y];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long npair = (long long)md.n * (md.n - 1) / 2;
   if (t >= npair) return;
-  int i, j;
-  tri_index(t, i, j);
+  const int2 pr = pairs[md.p0 + t];
+  const int i = pr.x, j = pr.y;
   const int* a = at + md.a0;
   const double* x = xyz + 3 * md.a0;
   const Element& ei = el[a[i]];
