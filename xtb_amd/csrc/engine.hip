@@ -1258,7 +1258,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
 // measured at the end of the sweep before) and the A and W tile updates.
 // Once per max_rounds rounds, after the pivots, the host reads both lanes'
 // counts of problems still running and stops a lane that has none left.
-constexpr int kProfEvery = 8;
+constexpr int kProfEvery = 16;
 
 void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   const hipStream_t ls[kMaxLanes] = {stream, stream2, stream4};
