@@ -240,12 +240,15 @@ __global__ void k_zero_atoms(const MolDesc* __restrict__ desc, double* v3, doubl
 // Newton-Schulz bookkeeping for the selected molecules: stage 0 (running),
 // the others stay done (stage 2)
 __global__ void k_ns_begin(int nmol, const int* __restrict__ sel, int* stage, int* act, int* failed,
-                           unsigned long long* res) {
+                           unsigned long long* res, double* cbound) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nmol) return;
   stage[m] = sel[m] ? 0 : 2;
   act[m] = sel[m] ? 1 : 0;
-  if (sel[m]) failed[m] = 0;
+  if (sel[m]) {
+    failed[m] = 0;
+    cbound[m] = 0.0;
+  }
   res[m] = 0ull;
 }
 
@@ -269,29 +272,30 @@ __global__ void k_sad_guess(const MolDesc* __restrict__ desc, const Element* __r
 // of Matrices", 2008, eq. 6.35): Y0 = S/c, Z0 = I,
 // T = (3I - Z Y)/2, Y <- Y T, Z <- T Z; Z -> (S/c)^(-1/2).
 // c = max row sum |S_ij| bounds the spectrum of S from above.
-__global__ void __launch_bounds__(256) k_ns_bound(const MolDesc* __restrict__ desc, const double* __restrict__ S,
-                                                  double* cbound) {
-  __shared__ double sh[4];
-  const MolDesc md = desc[blockIdx.x];
+// grid (row chunks of 64, molecules): thread = row; the maximum over the
+// molecule's rows by an atomic max on the bit patterns of cbound (cleared by
+// k_ns_begin; non-negative doubles order like their bit patterns)
+__global__ void __launch_bounds__(64) k_ns_bound(const MolDesc* __restrict__ desc, const double* __restrict__ S,
+                                                 double* cbound) {
+  const MolDesc md = desc[blockIdx.y];
   const int N = md.nao;
-  double m = 0.0;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    double r = 0.0;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double r = 0.0;
+  if (i < N)
     for (int j = 0; j < N; ++j) r += fabs(S[md.m0 + (long long)i * N + j]);
-    m = fmax(m, r);
-  }
-  m = block_max<256>(m, sh);
-  if (threadIdx.x == 0) cbound[md.mol] = m;
+  for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+  if (threadIdx.x == 0 && r > 0.0)
+    atomicMax((unsigned long long*)&cbound[md.mol], (unsigned long long)__double_as_longlong(r));
 }
 
 __global__ void k_ns_init(const MolDesc* __restrict__ desc, const double* __restrict__ S,
                           const double* __restrict__ cbound, double* Y, double* Z) {
   const MolDesc md = desc[blockIdx.y];
   const long long N = md.nao;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N * N) return;
-  Y[md.m0 + t] = S[md.m0 + t] / cbound[md.mol];
-  Z[md.m0 + t] = (t / N == t % N) ? 1.0 : 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N * N; t += (long long)gridDim.x * blockDim.x) {
+    Y[md.m0 + t] = S[md.m0 + t] / cbound[md.mol];
+    Z[md.m0 + t] = (t / N == t % N) ? 1.0 : 0.0;
+  }
 }
 
 // T = (3I - P)/2 with P = Z Y, and max |I - P| per molecule
@@ -302,12 +306,11 @@ __global__ void __launch_bounds__(256) k_ns_form(const MolDesc* __restrict__ des
   if (!act[md.mol]) return;
   const long long N = md.nao;
   if ((long long)blockIdx.x * blockDim.x >= N * N) return;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double r = 0.0;
-  if (t < N * N) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N * N; t += (long long)gridDim.x * blockDim.x) {
     const double d = (t / N == t % N) ? 1.0 : 0.0;
     const double p = P[md.m0 + t];
-    r = fabs(d - p);
+    r = fmax(r, fabs(d - p));
     P[md.m0 + t] = 0.5 * (3.0 * d - p);
   }
   r = block_max<256>(r, sh);
@@ -373,10 +376,8 @@ __global__ void k_ns_final(const MolDesc* __restrict__ desc, const double* __res
                            const double* __restrict__ cbound, double* C) {
   const MolDesc md = desc[blockIdx.y];
   const long long N = md.nao;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N * N) return;
   const double* Z = fin[md.mol] ? Z1 : Z0;
-  C[md.m0 + t] = Z[md.m0 + t] / sqrt(cbound[md.mol]);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N * N; t += (long long)gridDim.x * blockDim.x) C[md.m0 + t] = Z[md.m0 + t] / sqrt(cbound[md.mol]);
 }
 
 // total energy, gradient norm and dipole (src/scf_module.F90:882-913,
@@ -422,6 +423,10 @@ __global__ void __launch_bounds__(256) k_final(const MolDesc* __restrict__ desc,
 }
 
 inline unsigned grid1(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+// blocks per molecule of the grid-stride kernels over n x n matrices (256
+// threads each): a launch needs no more blocks than this however large the
+// largest molecule, so the small molecules' empty blocks cost nothing
+inline unsigned gstride(long long n2) { return (unsigned)std::min<long long>(grid1(n2, 256), 64); }
 
 // Aufbau occupation counts (occu: src/scc_core.f90:1112-1170)
 void host_occu(int ndim, int nel, int nopen, int& ihomoa, int& ihomob) {
@@ -1145,14 +1150,18 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   if (jac_padded) {
     if (!rows || pivot_lds) throw std::runtime_error("padded Jacobi problems need the register path");
     // A into the padded working copy; the eigenvectors come back to Wm at the end
-    hipLaunchKernelGGL(k_jac_pad, dim3(grid1((long long)js.big_max_ld * js.big_max_ld, 256), nb), dim3(256), 0,
-                       stream, js.d_probs, (const double*)Am, js.ja, act, 1);
+    hipLaunchKernelGGL(k_jac_pad, dim3(gstride((long long)js.big_max_ld * js.big_max_ld), nb), dim3(256), 0, stream,
+                       js.d_probs, (const double*)Am, js.ja, act, 1);
     Am = js.ja;
     Wm = js.jw;
   }
+  // W = I over the whole device (k_jac_init only sets the scales and states)
   hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
-                     check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, keep_w ? 1 : 0,
+                     check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, 1,
                      jac_tol_mid / jac_tol);
+  if (!keep_w)
+    hipLaunchKernelGGL(k_jac_ident, dim3(gstride((long long)js.big_max_ld * js.big_max_ld), nb), dim3(256), 0, stream,
+                       js.d_probs, Wm, check_active ? (const MolState*)d_state : (const MolState*)nullptr);
   // lanes: the problems of lanes 1 and 2 run their rounds on stream2 and
   // stream4 beside those of lane 0; the problems of a lane never touch another
   // lane's data, so the results do not depend on the lanes
@@ -1242,10 +1251,11 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
     }
   }
   if (jac_padded)
-    hipLaunchKernelGGL(k_jac_pad, dim3(grid1((long long)js.big_max_ld * js.big_max_ld, 256), nb), dim3(256), 0,
-                       stream, js.d_probs, (const double*)Wm, Wout, act, 0);
+    hipLaunchKernelGGL(k_jac_pad, dim3(gstride((long long)js.big_max_ld * js.big_max_ld), nb), dim3(256), 0, stream,
+                       js.d_probs, (const double*)Wm, Wout, act, 0);
   if (evout)
-    hipLaunchKernelGGL(k_jac_sort, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm,
+    hipLaunchKernelGGL(k_jac_sort, dim3(grid1(js.big_max_ld, 256), nb), dim3(256), 0, stream, js.d_probs,
+                       (const double*)Am, evout, perm,
                        check_active ? (const MolState*)d_state : (const MolState*)nullptr, 1 << 30);
   return rows;
 }
@@ -1322,7 +1332,7 @@ void Batch::eigh_rows(JacSet& js, double* Am, double* Wm, double* evout) {
   if (js.nsmall) {
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
                        (const double*)js.scale, js.state, 0, 1);
-    hipLaunchKernelGGL(k_jac_sort, dim3(nmol), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm, st,
+    hipLaunchKernelGGL(k_jac_sort, dim3(1, nmol), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm, st,
                        64);
   }
   if (js.nbig) {
@@ -1495,11 +1505,12 @@ void Batch::select_all() {
 
 void Batch::orthogonalizer() {
   // C = S^(-1/2) for the selected molecules
-  const dim3 gmm(grid1((long long)sub_max_nao * sub_max_nao, 256), nsub);
-  hipLaunchKernelGGL(k_ns_bound, dim3(nsub), dim3(256), 0, stream, d_sub, (const double*)S, ns_c);
-  hipLaunchKernelGGL(k_ns_init, gmm, dim3(256), 0, stream, d_sub, (const double*)S, (const double*)ns_c, A, W);
+  const dim3 gmm(gstride((long long)sub_max_nao * sub_max_nao), nsub);
   hipLaunchKernelGGL(k_ns_begin, dim3(grid1(nmol, 64)), dim3(64), 0, stream, nmol, (const int*)phase_act, ns_stage,
-                     ns_act, ns_failed, ns_res);
+                     ns_act, ns_failed, ns_res, ns_c);
+  hipLaunchKernelGGL(k_ns_bound, dim3(grid1(sub_max_nao, 64), nsub), dim3(64), 0, stream, d_sub, (const double*)S,
+                     ns_c);
+  hipLaunchKernelGGL(k_ns_init, gmm, dim3(256), 0, stream, d_sub, (const double*)S, (const double*)ns_c, A, W);
   const int maxit = 60;
   // Y and Z alternate between two buffers each (A/H and W/T) by iteration
   // parity; a molecule's final pair is at the parity recorded when it stops
@@ -1652,7 +1663,7 @@ bool Batch::scc_iteration() {
                      (const MolState*)d_state);
   HIPCHECK(hipEventRecord(ev_join, stream2));
   HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
-  hipLaunchKernelGGL(k_fock, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream, d_desc, S, H0,
+  hipLaunchKernelGGL(k_fock, dim3(gstride((long long)max_nao * max_nao), nmol), dim3(256), 0, stream, d_desc, S, H0,
                      D, Q, shift, vs, vd, vq, ao_sh, ao_at, thr2, H, (const MolState*)d_state);
   // generalized eigenproblem in the current orbital basis C (S-orthonormal)
   gemm(0, H, C, T, nullptr, nullptr, true);   // T = H C
@@ -1674,7 +1685,7 @@ bool Batch::scc_iteration() {
     else
       gemm(3, C, W, T, nullptr, perm, true);  // T = C W(:, perm)
   }
-  hipLaunchKernelGGL(k_copy_active, dim3(grid1((long long)max_nao * max_nao, 256), nmol), dim3(256), 0, stream,
+  hipLaunchKernelGGL(k_copy_active, dim3(gstride((long long)max_nao * max_nao), nmol), dim3(256), 0, stream,
                      d_desc, (const MolState*)d_state, (const double*)T, C);
   hipLaunchKernelGGL(k_occ_fermi, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, (const double*)emo, focca, foccb,
                      focc, opt.etemp, (const double*)qsh, (const double*)dipm, (const double*)qp, q_in);

@@ -1588,13 +1588,13 @@ __global__ void __launch_bounds__(256) k_jac_pad(const JacobiProb* __restrict__ 
   if (mstate && !mstate[p].active) return;
   const JacobiProb pr = probs[p];
   const long long n = pr.dim;
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * n) return;
-  const long long u = e / n, v = e % n;
-  if (in)
-    dst[pr.a_off + u * pr.ld + v] = src[pr.src_off + e];
-  else
-    dst[pr.src_off + e] = src[pr.w_off + u * pr.ld + v];
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (long long)gridDim.x * blockDim.x) {
+    const long long u = e / n, v = e % n;
+    if (in)
+      dst[pr.a_off + u * pr.ld + v] = src[pr.src_off + e];
+    else
+      dst[pr.src_off + e] = src[pr.w_off + u * pr.ld + v];
+  }
 }
 
 // largest off-diagonal magnitude of the unfinished block-path problems that
@@ -1986,22 +1986,50 @@ __global__ void __launch_bounds__(256) k_tred_ql(const JacobiProb* __restrict__ 
 }
 
 // eigenvalues ascending: ev[k] = A[perm[k]][perm[k]] (stable rank sort)
-__global__ void k_jac_sort(const JacobiProb* __restrict__ probs, const double* __restrict__ A, double* ev, int* perm,
-                           const MolState* __restrict__ mstate, int max_dim) {
-  if (mstate && !mstate[blockIdx.x].active) return;
-  const JacobiProb pr = probs[blockIdx.x];
+// eigenvalues A_ii in ascending order (ties by index) and their permutation;
+// grid (chunks of 256 eigenvalues, problems), the diagonal staged through
+// LDS 2048 values at a time
+__global__ void __launch_bounds__(256) k_jac_sort(const JacobiProb* __restrict__ probs, const double* __restrict__ A,
+                                                  double* ev, int* perm, const MolState* __restrict__ mstate,
+                                                  int max_dim) {
+  __shared__ double dg[2048];
+  const int p = blockIdx.y;
+  if (mstate && !mstate[p].active) return;
+  const JacobiProb pr = probs[p];
   if (pr.dim > max_dim) return;
   const int n = pr.dim, ld = pr.ld;
+  if (blockIdx.x * blockDim.x >= (unsigned)n) return;
   const double* Ap = A + pr.a_off;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double di = Ap[(long long)i * ld + i];
-    int rank = 0;
-    for (int j = 0; j < n; ++j) {
-      const double dj = Ap[(long long)j * ld + j];
-      rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
-    }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double di = i < n ? Ap[(long long)i * ld + i] : 0.0;
+  int rank = 0;
+  for (int j0 = 0; j0 < n; j0 += 2048) {
+    const int m = min(2048, n - j0);
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) dg[j] = Ap[(long long)(j0 + j) * ld + j0 + j];
+    __syncthreads();
+    if (i < n)
+      for (int j = 0; j < m; ++j) {
+        const double dj = dg[j];
+        rank += (dj < di || (dj == di && j0 + j < i)) ? 1 : 0;
+      }
+  }
+  if (i < n) {
     ev[pr.ev_off + rank] = di;
     perm[pr.ev_off + rank] = i;
+  }
+}
+
+// W = I (row stride ld) of the problems still to solve; grid (chunks, problems)
+__global__ void __launch_bounds__(256) k_jac_ident(const JacobiProb* __restrict__ probs, double* W,
+                                                   const MolState* __restrict__ mstate) {
+  const int p = blockIdx.y;
+  if (mstate && !mstate[p].active) return;
+  const JacobiProb pr = probs[p];
+  const long long n = pr.dim;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (long long)gridDim.x * blockDim.x) {
+    const long long u = e / n, v = e % n;
+    W[pr.w_off + u * pr.ld + v] = (u == v) ? 1.0 : 0.0;
   }
 }
 
@@ -2029,7 +2057,6 @@ __global__ void k_copy_active(const MolDesc* __restrict__ desc, const MolState* 
   const MolDesc md = desc[blockIdx.y];
   if (!st[blockIdx.y].active) return;
   const long long n2 = (long long)md.nao * md.nao;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n2) C[md.m0 + t] = T[md.m0 + t];
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (long long)gridDim.x * blockDim.x) C[md.m0 + t] = T[md.m0 + t];
 }
 }  // namespace xtbamd

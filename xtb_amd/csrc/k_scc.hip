@@ -204,8 +204,7 @@ __global__ void __launch_bounds__(256) k_fock(const MolDesc* __restrict__ desc, 
   const MolDesc md = desc[blockIdx.y];
   if (!state[blockIdx.y].active) return;
   const long long N = md.nao, NN = N * N;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= NN) return;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < NN; t += (long long)gridDim.x * blockDim.x) {
   const int i = (int)(t / N), j = (int)(t % N);
   const long long p = md.m0 + t;
   const double dum = S[p] * kAutoEV * 0.5;
@@ -238,6 +237,7 @@ __global__ void __launch_bounds__(256) k_fock(const MolDesc* __restrict__ desc, 
     h += 0.50 * e1 * kAutoEV;
   }
   H[p] = h;
+  }
 }
 
 // Aufbau occupation of orbital i (occu: src/scc_core.f90:1112-1170)
