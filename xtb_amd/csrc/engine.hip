@@ -1553,6 +1553,12 @@ void Batch::setup() {
   hipLaunchKernelGGL(k_cn, gat, dim3(64), 0, stream, d_sub, el, at, xyz, cn, cn_cov, cn_d4, cn_eeq);
   hipLaunchKernelGGL(k_shell_setup, dim3(grid1(sub_max_nsh, 64), ns), dim3(64), 0, stream, d_sub, el, at, sh_at,
                      sh_m, cn, selfe, dsedcn, shgam, zsh);
+  // stream2 (the EEQ solve below) may start once CN and zsh exist; its
+  // launches are enqueued after the integrals so those start first.
+  if (opt.guess == 0) {
+    HIPCHECK(hipEventRecord(ev_fork, stream));
+    HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
+  }
   const long long maxpairs = (long long)sub_max_n * (sub_max_n - 1) / 2;
   if (maxpairs > 0)
     hipLaunchKernelGGL(k_sdqh0, dim3(grid1(maxpairs, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
@@ -1580,12 +1586,10 @@ void Batch::setup() {
                        (const double*)sv_adet, sv_B);
   }
   // initial charges: EEQ (default) or SAD.  The EEQ solve runs on stream2
-  // beside the overlap integrals and S^(-1/2) (it reads only CN, zsh and
-  // geometry, writes only its own system and the charges); joined below.
+  // beside the integrals and S^(-1/2) (it reads only CN, zsh and geometry,
+  // writes only its own system and the charges); joined below.
   std::vector<int> big;
   if (opt.guess == 0) {
-    HIPCHECK(hipEventRecord(ev_fork, stream));
-    HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
     const long long ne = sub_max_n + 1;
     hipLaunchKernelGGL(k_eeq_matrix, dim3(grid1(ne * ne, 256), ns), dim3(256), 0, stream2, d_sub, el, at, xyz,
                        cn_eeq, E, xE);
