@@ -167,7 +167,7 @@ def main():
     for _ in range(args.warmup):
         b.run()
     times = []
-    b.profile("gemm", "jacobi")   # HIP events around every launch of both kernel families
+    b.profile("gemm", "jacobi")   # HIP events around the GEMM launches and a sample of the pivot launches
     for _ in range(args.steps):
         t = time.perf_counter()
         b.run()

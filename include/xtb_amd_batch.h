@@ -71,12 +71,13 @@ int xtbamd_batch_size(xtbamd_batch b);
 int xtbamd_batch_counters(xtbamd_batch b, int* scc_steps, long long* jacobi_rounds);
 int xtbamd_batch_get(xtbamd_batch b, int m, xtbamd_result* out);
 void xtbamd_batch_free(xtbamd_batch b);
-/* HIP-event timing of kernel families on the batch's stream, from this call on.
- * kernels: bit 0 = the batched FP64 GEMM (k_gemm), bit 1 = the block-Jacobi
- * pivot kernel (k_jac_pairs); 0 = off.  profile_result(kernel = 1 GEMM, 2
- * Jacobi) gives the number of timed launches, their summed duration and their
- * summed algorithmic flops (GEMM: 2 N^3 per molecule processed; Jacobi: 36864
- * per inner round of a 64x64 pivot problem). */
+/* HIP-event timing of kernel families on the streams they run on, from this
+ * call on.  kernels: bit 0 = the batched FP64 GEMM (k_gemm), bit 1 = the
+ * block-Jacobi pivot kernel (k_jac_pairs); 0 = off.  profile_result(kernel =
+ * 1 GEMM, 2 pivots) gives the number of launches, their summed duration and
+ * their summed algorithmic flops (GEMM: 2 N^3 per molecule processed; pivots:
+ * 9216 per inner round of a 32x32 pivot problem).  The pivots are timed on a
+ * sample of the rounds and scaled to all of them. */
 int xtbamd_batch_profile(xtbamd_batch b, int kernels);
 int xtbamd_batch_profile_result(xtbamd_batch b, int kernel, int* launches, double* total_ms, double* work);
 /* ---- continuous batching (serving) ----

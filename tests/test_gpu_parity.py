@@ -280,6 +280,31 @@ def test_jacobi_lanes_do_not_change_results(monkeypatch):
         _same(r1, r3, f"mol{k} nao={r1['nao']}")
 
 
+def test_cross_rounds_agree_with_full_sweeps(monkeypatch):
+    """Pivot problems that rotate only the pairs across their two blocks in
+    most outer rounds (the default) against a full inner sweep in every round
+    (XTBAMD_JAC_CROSS=0): both converge every diagonalisation to the same
+    tolerance, so the results agree far inside the parity tolerances; both
+    match the oracle."""
+    xa = _engine()
+    from xtb_amd import workloads as w
+    mols = [w.water_cluster(12, 5), w.alkane(20, 6), w.alkane(45, 8), w.water_cluster(30, 9), w.alkane(64, 3)]
+    runs = []
+    for cross in ("1", "0"):
+        monkeypatch.setenv("XTBAMD_JAC_CROSS", cross)
+        b = xa.Batch([(z, x, 0.0, 0) for z, x in mols]).run()
+        runs.append(b.results())
+        b.close()
+    for k, (rx, rf) in enumerate(zip(*runs)):
+        tag = f"mol{k} nao={rx['nao']}"
+        assert rx["converged"] and rf["converged"], tag
+        assert abs(rx["energy"] - rf["energy"]) <= 1e-9, tag
+        assert np.abs(rx["gradient"] - rf["gradient"]).max() <= 1e-8, tag
+        assert np.abs(rx["charges"] - rf["charges"]).max() <= 1e-8, tag
+    z, x = mols[-1]
+    _cmp(runs[0][-1], po.singlepoint(z, x, threads=8, lapack=True), "cross rounds, C64H130")
+
+
 def test_server_results_same_alone_and_in_mix():
     """Continuous batching: requests submitted while others iterate give the
     bitwise alone results; every request finishes."""

@@ -297,6 +297,7 @@ class Batch {
   int inner_sweeps = 1;    // sweeps of each pivot problem per block-Jacobi round
   double jac_tol = 1e-12;   // Jacobi convergence: max |A_ij| <= jac_tol * max(|A_ii|, 1)
   double jac_tol_mid = 1e-12;   // the same before a molecule's final diagonalization
+  int jac_cross = 1;   // most outer rounds' pivots rotate only the pairs across their two blocks
   bool trace_rounds = false;   // print the Jacobi rounds of each SCC iteration to stderr
   bool pivot_lds = false;  // 32x32 pivots: LDS kernel instead of the register-resident one
   bool apply_lds = false;  // 32x32 pivots: LDS tile updates with W as columns instead of register ones with W as rows
@@ -322,6 +323,7 @@ class Batch {
   std::vector<hipEvent_t> prof_ev_[3];
   int prof_used_[3] = {0, 0, 0};
   int prof_all_[3] = {0, 0, 0};   // every launch of a family timed on a sample
+  long long prof_round_ = 0;       // block-Jacobi rounds since prof_enable (the sample clock)
   unsigned long long* prof_work_ = nullptr;   // [3]
   void prof_begin(int kind, hipStream_t s = nullptr);
   void prof_end(int kind, const int* mask, int mstride, hipStream_t s = nullptr);

@@ -609,6 +609,7 @@ void Batch::init_common() {
   if (const char* r = std::getenv("XTBAMD_JAC_TOL")) jac_tol = std::atof(r);
   jac_tol_mid = jac_tol;
   if (const char* r = std::getenv("XTBAMD_JAC_TOL_MID")) jac_tol_mid = std::max(jac_tol, std::atof(r));
+  if (const char* r = std::getenv("XTBAMD_JAC_CROSS")) jac_cross = std::atoi(r) != 0;
   trace_rounds = std::getenv("XTBAMD_TRACE_ROUNDS") != nullptr;
   if (const char* r = std::getenv("XTBAMD_JAC_LANES")) jac_lanes = std::min(kMaxLanes, std::max(1, std::atoi(r)));
   // default priority (as every stream here), no implicit ordering with the null stream
@@ -1213,7 +1214,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
                              (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, (const int*)js.state,
                              inner_sweeps, jw);
         else
-          hipLaunchKernelGGL(k_jac_pairsR, dim3(js.npairs), dim3(64), 0, stream, js.d_probs, js.d_pairs, r,
+          hipLaunchKernelGGL(k_jac_pairsR<false>, dim3(js.npairs), dim3(64), 0, stream, js.d_probs, js.d_pairs, r,
                              (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, js.state, inner_sweeps, jw,
                              jac_tol, (unsigned long long*)nullptr, 0);
         prof_end(2, nullptr, 0);
@@ -1268,7 +1269,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
 // measured at the end of the sweep before) and the A and W tile updates.
 // Once per max_rounds rounds, after the pivots, the host reads both lanes'
 // counts of problems still running and stops a lane that has none left.
-constexpr int kProfEvery = 16;
+constexpr int kProfEvery = 17;
 
 void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   const hipStream_t ls[kMaxLanes] = {stream, stream2, stream4};
@@ -1284,16 +1285,23 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   }
   const int rmax = (kMaxSweeps + 1) * js.max_rounds;
   for (int r = 0; r <= rmax; ++r) {
+    // timing events (and the work counts) on every kProfEvery-th round only,
+    // counted over all eigensolves (17 is prime to the cross-round period 8,
+    // so the sample meets every kind of round alike): an event pair per
+    // launch would cost more host time than the launch
+    const bool sample = (prof_round_ % kProfEvery) == 0;
     for (int g = 0; g < kMaxLanes; ++g) {
       if (!live[g]) continue;
-      // timing events (and the flop count) on every kProfEvery-th round only:
-      // an event pair per launch would cost more host time than the launch
-      const bool sample = (r % kProfEvery) == 0;
       if (prof_mask & 4) prof_all_[2] += 1;
       if (sample) prof_begin(2, ls[g]);
-      hipLaunchKernelGGL(k_jac_pairsR, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
-                         js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
-                         js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol);
+      if (jac_cross)
+        hipLaunchKernelGGL(k_jac_pairsR<true>, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
+                           js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
+                           js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol);
+      else
+        hipLaunchKernelGGL(k_jac_pairsR<false>, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
+                           js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
+                           js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol);
       if (sample) prof_end(2, nullptr, 0, ls[g]);
     }
     if (r % js.max_rounds == 0) {
@@ -1321,6 +1329,7 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
                          (const int*)js.state, js.offmax, nmol);
     }
     jacobi_rounds_ += 1;
+    prof_round_ += 1;
   }
 }
 
@@ -1427,7 +1436,16 @@ void Batch::gemm(int kind, const double* a, const double* b, double* c, const do
 
 void Batch::prof_enable(int mask) {
   prof_mask = mask;
-  for (int k = 0; k < 3; ++k) prof_used_[k] = prof_all_[k] = 0;
+  for (int k = 0; k < 3; ++k) {
+    prof_used_[k] = prof_all_[k] = 0;
+    // events created up front: creating them inside a timed loop costs host time
+    if ((mask & (1 << k)) && prof_ev_[k].size() < 4096) {
+      const size_t old = prof_ev_[k].size();
+      prof_ev_[k].resize(4096);
+      for (size_t i = old; i < prof_ev_[k].size(); ++i) HIPCHECK(hipEventCreate(&prof_ev_[k][i]));
+    }
+  }
+  prof_round_ = 0;
   if (!prof_work_) prof_work_ = alloc<unsigned long long>(3);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, (double*)prof_work_, 3LL, 0.0);
   HIPCHECK(hipStreamSynchronize(stream));

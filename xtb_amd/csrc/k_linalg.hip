@@ -1125,6 +1125,29 @@ __device__ __forceinline__ void circle_rows(double (&T)[4][2], double (&B)[4][2]
   }
 }
 
+// Cross rounds rotate only the pairs (index k, index 16 + (k + r) mod 16),
+// r = 0..15: the tops stay in place (top row of pair 4g + i in T[i]) and the
+// bottoms move one pair down cyclically (bottom row of pair 4g + i at phase PH
+// in B[(i + PH) & 3]); 16 rounds meet every pair across the two halves once
+// and restore the slot order.  Row move: only the bottom of pair 4g + 3 is new
+// to its register, taken from the next group's pair 4g + 4 (pair 15 from
+// pair 0).
+template <int PH>
+__device__ __forceinline__ void cross_rows(double (&B)[4][2], int lane) {
+  const int dn = (lane + 16) & 63;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) B[PH & 3][h] = __shfl(B[PH & 3][h], dn, 64);
+}
+
+// column move of a cross round: bottom(c) <- bottom((c + 1) mod 16), the
+// row_ror:15 rotation within each row of 16 lanes
+__device__ __forceinline__ double cross_col(double bot) {
+  const long long b = __double_as_longlong(bot);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned int)(b & 0xffffffffll), 0x12F, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x12F, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
 // 1/x and 1/sqrt(x) for normal positive x: the hardware approximations
 // refined by two Newton steps each (relative error near 1 ulp)
 __device__ __forceinline__ double rcp_refined(double x) {
@@ -1144,7 +1167,7 @@ __device__ __forceinline__ double rsq_refined(double x) {
   return fma(y, e, y);
 }
 
-template <int PH>
+template <int PH, bool CROSS = false>
 __device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2], double (&QT)[4][2],
                                            double (&QB)[4][2], double2* rcs, int lane, int g, int c, int id, bool diag,
                                            double tol, int& total) {
@@ -1152,8 +1175,8 @@ __device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2],
 #pragma unroll
   for (int i = 0; i < 4; ++i)
     if (id == i) {
-      app = T[(i - PH) & 3][0];
-      apq = T[(i - PH) & 3][1];
+      app = T[CROSS ? i : (i - PH) & 3][0];
+      apq = T[CROSS ? i : (i - PH) & 3][1];
       aqq = B[(i + PH) & 3][1];
     }
   double cr = 1.0, sr = 0.0, tr = 0.0;
@@ -1183,7 +1206,7 @@ __device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2],
     const double c2 = col.x, s2 = col.y;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      double* t = T[(i - PH) & 3];
+      double* t = T[CROSS ? i : (i - PH) & 3];
       double* b = B[(i + PH) & 3];
       const double c1 = row[i].x, s1 = row[i].y;
       const double tt = t[0], tb = t[1], bt = b[0], bb = b[1];
@@ -1199,7 +1222,7 @@ __device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2],
         t[1] = 0.0;
         b[0] = 0.0;
       }
-      double* qt = QT[(i - PH) & 3];
+      double* qt = QT[CROSS ? i : (i - PH) & 3];
       double* qb = QB[(i + PH) & 3];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -1209,12 +1232,22 @@ __device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2],
       }
     }
   }
-  circle_rows<PH>(T, B, lane, g);
-  circle_rows<PH>(QT, QB, lane, g);
+  if (CROSS) {
+    cross_rows<PH>(B, lane);
+    cross_rows<PH>(QB, lane);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    circle_cols(T[i][0], T[i][1], c);
-    circle_cols(B[i][0], B[i][1], c);
+    for (int i = 0; i < 4; ++i) {
+      T[i][1] = cross_col(T[i][1]);
+      B[i][1] = cross_col(B[i][1]);
+    }
+  } else {
+    circle_rows<PH>(T, B, lane, g);
+    circle_rows<PH>(QT, QB, lane, g);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      circle_cols(T[i][0], T[i][1], c);
+      circle_cols(B[i][0], B[i][1], c);
+    }
   }
 }
 
@@ -1225,7 +1258,10 @@ __device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2],
 // will fill and, for a finished problem, records the outcome in state, which
 // every wave of the problem decides alike (done when max <= tol, failed
 // after kMaxSweeps sweeps).
-__global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
+constexpr int kCrossPeriod = 8;
+
+template <bool CROSSK>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
                                                    int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
                                                    const double* __restrict__ scale_in, int* state,
                                                    int inner_sweeps, unsigned long long* work, double tolrel,
@@ -1279,6 +1315,13 @@ __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict_
   const bool diag = (c >> 2) == g;   // this lane owns the diagonal block of pair c
   const int id = c & 3;
   int total = 0;
+  // cross mode (CROSSK, a separate instantiation: its second loop costs
+  // registers): outer rounds other than the first and the last of every
+  // kCrossPeriod rotate only the pairs across the two blocks.  The period is
+  // the same for every problem so that the waves of a launch take equally
+  // long; the pairs inside a block are met at least every kCrossPeriod rounds.
+  const int rin = r % kCrossPeriod;
+  const bool cross = CROSSK && r < kBanded && rin != 0 && rin != kCrossPeriod - 1;
   for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
     double m = 0.0;
 #pragma unroll
@@ -1291,6 +1334,16 @@ __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict_
     }
     m = block_max<64>(m, red);
     if (m <= tol) break;
+    if (cross) {
+#pragma unroll 1
+      for (int rr = 0; rr < 16; rr += 4) {
+        jac_roundR<0, true>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+        jac_roundR<1, true>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+        jac_roundR<2, true>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+        jac_roundR<3, true>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
+      }
+      continue;
+    }
     for (int rr = 0; rr < 28; rr += 4) {
       jac_roundR<0>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
       jac_roundR<1>(T, B, QT, QB, rcs, lane, g, c, id, diag, tol, total);
@@ -1336,6 +1389,13 @@ __global__ void __launch_bounds__(64) k_jac_pairsR(const JacobiProb* __restrict_
     if (work && total > 0) atomicAdd(work, (unsigned long long)(9 * 32 * 32) * (unsigned long long)total);
   }
 }
+
+template __global__ void k_jac_pairsR<false>(const JacobiProb*, const int2*, int, const double*, double*, int*,
+                                             const double*, int*, int, unsigned long long*, double,
+                                             unsigned long long*, int);
+template __global__ void k_jac_pairsR<true>(const JacobiProb*, const int2*, int, const double*, double*, int*,
+                                            const double*, int*, int, unsigned long long*, double,
+                                            unsigned long long*, int);
 
 // ---------------------------------------------------------------------------
 // Tile updates of the block-Jacobi rounds with 32 x 32 pivots, one wave per
