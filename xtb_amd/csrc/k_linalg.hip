@@ -1316,12 +1316,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
   const int id = c & 3;
   int total = 0;
   // cross mode (CROSSK, a separate instantiation: its second loop costs
-  // registers): outer rounds other than the first and the last of every
-  // kCrossPeriod rotate only the pairs across the two blocks.  The period is
-  // the same for every problem so that the waves of a launch take equally
-  // long; the pairs inside a block are met at least every kCrossPeriod rounds.
-  const int rin = r % kCrossPeriod;
-  const bool cross = CROSSK && r < kBanded && rin != 0 && rin != kCrossPeriod - 1;
+  // registers): outer rounds other than the first of every kCrossPeriod
+  // rotate only the pairs across the two blocks.  The period is the same for
+  // every problem so that the waves of a launch take equally long; the pairs
+  // inside a block are met at least every kCrossPeriod rounds.
+  const bool cross = CROSSK && r < kBanded && r % kCrossPeriod != 0;
   for (int sweep = 0; sweep < inner_sweeps; ++sweep) {
     double m = 0.0;
 #pragma unroll
