@@ -1433,6 +1433,14 @@ __device__ __forceinline__ void jac_tile_a(const JacobiProb* __restrict__ probs,
   const double* Qk = Qbuf + pr.q_off + (long long)k * 32 * 32;
   const double* Ql = Qbuf + pr.q_off + (long long)l * 32 * 32;
   const int lane = threadIdx.x, i16 = lane & 15, k4 = lane >> 4;
+  // Q_k is loaded with the first product's operands, so that its latency
+  // overlaps them instead of following the first product
+  double qv[8][2];
+  if (rk)
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) qv[s][h] = Qk[(4 * s + k4) * 32 + 16 * h + i16];
   v4d t[2][2];
   if (rl) {
     // T = A_kl Q_l; A_kl(u, w) is read as A(I_l[w], I_k[u]) (coalesced over u)
@@ -1471,11 +1479,6 @@ __device__ __forceinline__ void jac_tile_a(const JacobiProb* __restrict__ probs,
   v4d o[2][2];
   if (rk) {
     // out = Q_k^T T; the A operand Q_k^T(u, w) = Q_k(w, u), the B operand is T
-    double qv[8][2];
-#pragma unroll
-    for (int s = 0; s < 8; ++s)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) qv[s][h] = Qk[(4 * s + k4) * 32 + 16 * h + i16];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
