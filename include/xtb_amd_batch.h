@@ -70,6 +70,11 @@ int xtbamd_batch_size(xtbamd_batch b);
 /* work counters of the last run: SCC steps launched, block-Jacobi rounds */
 int xtbamd_batch_counters(xtbamd_batch b, int* scc_steps, long long* jacobi_rounds);
 int xtbamd_batch_get(xtbamd_batch b, int m, xtbamd_result* out);
+/* current atomic charges [natoms] of molecule m: after xtbamd_batch_setup the
+ * initial guess (EEQ, src/eeq_model.f90:678-1015 via newWavefunction,
+ * src/xtb/calculator.f90:462-480; SAD; or the restart guess), after an
+ * iteration the Mulliken charges of that iteration */
+int xtbamd_batch_charges(xtbamd_batch b, int m, double* q);
 void xtbamd_batch_free(xtbamd_batch b);
 /* HIP-event timing of kernel families on the streams they run on, from this
  * call on.  kernels: bit 0 = the batched FP64 GEMM (k_gemm), bit 1 = the

@@ -2607,8 +2607,24 @@ int om_singlepoint(const om_options *opt_in, int n, const int *at, const double 
       if (res->density) memcpy(res->density, P, sizeof(double) * NN);
       if (res->coeff) memcpy(res->coeff, C, sizeof(double) * NN);
       if (res->wbo) {
-        double *PS = NEW(double, NN);
-        gemm_nn(N, P, S, PS, use_lib);
+        /* closed shell: get_wiberg, sum over the AO pairs of two atoms of
+         * (PS)_km (PS)_mk; open shell: get_unrestricted_wiberg
+         * (src/scc_core.f90:1248-1295), 2 [(Pa S)_km (Pa S)_mk + (Pb S)_km (Pb S)_mk]
+         * with the spin densities Pa = C fa C^T, Pb = C fb C^T (src/scf_module.F90:856-873) */
+        const int nspin = nopen > 0 ? 2 : 1;
+        double *PS = NEW(double, NN * nspin);
+        if (nopen > 0) {
+          const double *fs[2] = {focca, foccb};
+          double *Ps = NEW(double, NN);
+          for (int sp = 0; sp < 2; ++sp) {
+            for (size_t i = 0; i < (size_t)N; ++i) for (int k = 0; k < N; ++k) tmp[i * N + k] = C[i * N + k] * fs[sp][k];
+            gemm_nt(N, C, tmp, Ps, use_lib);
+            gemm_nn(N, Ps, S, PS + (size_t)sp * NN, use_lib);
+          }
+          free(Ps);
+        } else {
+          gemm_nn(N, P, S, PS, use_lib);
+        }
         memset(res->wbo, 0, sizeof(double) * n * n);
         for (int i = 0; i < n; ++i)
           for (int j = 0; j < i; ++j) {
@@ -2618,7 +2634,9 @@ int om_singlepoint(const om_options *opt_in, int n, const int *at, const double 
             if (r2 < 100.0)
               for (int k = b.at_ao0[i]; k < b.at_ao0[i] + b.at_nao[i]; ++k)
                 for (int m = b.at_ao0[j]; m < b.at_ao0[j] + b.at_nao[j]; ++m)
-                  xs += PS[(size_t)k * N + m] * PS[(size_t)m * N + k];
+                  for (int sp = 0; sp < nspin; ++sp)
+                    xs += PS[sp * NN + (size_t)k * N + m] * PS[sp * NN + (size_t)m * N + k];
+            if (nopen > 0) xs *= 2.0;
             res->wbo[i * n + j] = res->wbo[j * n + i] = xs;
           }
         free(PS);

@@ -31,6 +31,7 @@ def _cmp(r, o, tag):
     dq = np.max(np.abs(r["charges"] - o["q"]))
     dg = np.max(np.abs(r["gradient"] - o["gradient"]))
     print(f"{tag}: dE={dE:.2e} dq={dq:.2e} dg={dg:.2e} iters={r['iterations']}/{o['iterations']}", flush=True)
+    assert r["iterations"] == o["iterations"], f"{tag} SCC iterations"
     assert dE <= TOL_E, f"{tag} energy {r['energy']} vs {o['e_total']}"
     assert dq <= TOL_Q, f"{tag} charges"
     assert dg <= TOL_G, f"{tag} gradient"
@@ -58,7 +59,7 @@ def test_golden_single(golden, case):
         assert abs(val - ref) <= max(tol, 1e-7), f"{case} {key}: {val} vs {ref}"
 
 
-SOLV_CASES = ["gbsa_water_api"] + [f"mindless{k:02d}_solv" for k in range(1, 11)]
+SOLV_CASES = ["gbsa_water_api"] + [f"mindless{k:02d}_solv" for k in range(1, 11)] + ["feco5_gbsa", "feco5_alpb"]
 
 
 @pytest.mark.parametrize("case", SOLV_CASES)

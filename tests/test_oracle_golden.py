@@ -9,7 +9,8 @@ import pytest
 
 from oracle import pyoracle as po
 
-SP_CASES = ["h2o_scc", "propyne", "water_tetramer"] + [f"mindless{k:02d}" for k in range(1, 11)]
+SP_CASES = ["h2o_scc", "propyne", "water_tetramer"] + [f"mindless{k:02d}" for k in range(1, 11)] + \
+    ["feco5", "co_cnx6", "fe_cnx6"]
 
 
 def _lookup(res, key):
@@ -110,7 +111,7 @@ def test_overlap_closed_form():
     assert abs(S[0, 0] - 1.0) < 1e-15
 
 
-SOLV_CASES = ["gbsa_water_api"] + [f"mindless{k:02d}_solv" for k in range(1, 11)]
+SOLV_CASES = ["gbsa_water_api"] + [f"mindless{k:02d}_solv" for k in range(1, 11)] + ["feco5_gbsa", "feco5_alpb"]
 
 
 @pytest.mark.parametrize("case", SOLV_CASES)

@@ -107,6 +107,8 @@ struct JacobiProb {
   int group;                  // lane (stream) of the two-lane round schedule
   int ld;                     // row stride of the problem's A and W (dim, or dim rounded up to 16)
   long long src_off;          // the matrix in the caller's packed layout (row stride dim)
+  int maxsw;                  // sweeps before the problem is given up (failed)
+  int pad_;
 };
 
 // stream lanes of the block-Jacobi rounds (at most)
@@ -173,6 +175,8 @@ class Batch {
   void set_solvent(const std::string& name, int kernel, bool alpb, int state = 1, double temperature = 298.15);
   bool solvated() const { return solv_on_; }
   MolResult result(int m, bool matrices = false) const;
+  // current atomic charges of molecule m (after setup(): the initial guess)
+  void charges(int m, double* out) const;
   int size() const { return nmol; }
   // sizes and index maps of one molecule; throws for invalid input
   static void describe(const MoleculeInput& mi, int maxiter, MolDesc& d, HostMol& hm, MolState& st);
@@ -208,6 +212,8 @@ class Batch {
   int nsub = 0;
   int sub_max_n = 0, sub_max_nsh = 0, sub_max_nao = 0, sub_max_nd4 = 0;
   int* phase_act = nullptr;     // 1 for selected molecules (per slot)
+  int* open_act = nullptr;      // 1 for selected open-shell molecules (per slot)
+  int nsub_open = 0;
   std::vector<int> sel_;
   MolState* d_state = nullptr;
   MolState* d_state_stage = nullptr;   // staging for setup()'s one upload of initial states
@@ -297,6 +303,7 @@ class Batch {
   int inner_sweeps = 1;    // sweeps of each pivot problem per block-Jacobi round
   double jac_tol = 1e-12;   // Jacobi convergence: max |A_ij| <= jac_tol * max(|A_ii|, 1)
   double jac_tol_mid = 1e-12;   // the same before a molecule's final diagonalization
+  int jac_max_sweeps = 40;   // sweeps before a Jacobi problem is given up (the molecule fails)
   int jac_cross = 1;   // most outer rounds' pivots rotate only the pairs across their two blocks
   bool trace_rounds = false;   // print the Jacobi rounds of each SCC iteration to stderr
   bool pivot_lds = false;  // 32x32 pivots: LDS kernel instead of the register-resident one

@@ -157,6 +157,24 @@ __global__ void k_eeq_reset(const MolDesc* __restrict__ desc, int ns, int* faile
   if (k < ns) failed[desc[k].mol] = 0;
 }
 
+// Failure codes of MolState::failed (0 = fine).
+constexpr int kFailOrtho = 2;   // S^(-1/2) iterations did not converge
+constexpr int kFailDiag = 3;    // eigensolver gave up ("Diagonalization of Hamiltonian failed")
+constexpr int kFailEeq = 4;     // EEQ system singular
+constexpr int kFailSolv = 7;    // solvent surface
+
+// A molecule whose solver flagged a failure (flags[m * stride] != 0, per
+// slot) gets the code and stops iterating, like the early return of the
+// reference's SCC after a failed diagonalization (src/scc_core.f90:475-480);
+// its results are still filled.  The first failure's code is kept.
+__global__ void k_fail_fold(MolState* st, int nmol, const int* __restrict__ flags, int stride, int code,
+                            const int* __restrict__ sel) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmol || (sel && !sel[m]) || !flags[(long long)m * stride]) return;
+  if (!st[m].failed) st[m].failed = code;
+  st[m].active = 0;
+}
+
 // Column k of the LU factorisation with partial pivoting of large EEQ systems
 // (one workgroup per listed molecule): pivot search with the same tie rule as
 // k_eeq_gesv, row swap, multipliers and the right-hand-side update.
@@ -610,6 +628,9 @@ void Batch::init_common() {
   jac_tol_mid = jac_tol;
   if (const char* r = std::getenv("XTBAMD_JAC_TOL_MID")) jac_tol_mid = std::max(jac_tol, std::atof(r));
   if (const char* r = std::getenv("XTBAMD_JAC_CROSS")) jac_cross = std::atoi(r) != 0;
+  // bound on the sweeps of one diagonalization; a problem still above the
+  // tolerance after it fails its molecule
+  if (const char* r = std::getenv("XTBAMD_JAC_MAX_SWEEPS")) jac_max_sweeps = std::min(40, std::max(1, std::atoi(r)));
   trace_rounds = std::getenv("XTBAMD_TRACE_ROUNDS") != nullptr;
   if (const char* r = std::getenv("XTBAMD_JAC_LANES")) jac_lanes = std::min(kMaxLanes, std::max(1, std::atoi(r)));
   // default priority (as every stream here), no implicit ordering with the null stream
@@ -622,7 +643,10 @@ void Batch::init_common() {
   HIPCHECK(hipEventCreateWithFlags(&ev_join4, hipEventDisableTiming));
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
   if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
-  jac_padded = jac_p == 32 && !apply_lds && !pivot_lds && oa_max_steps == 0 && banded_passes == 0 && !use_ql;
+  // padded working copies on the register path; banded rounds only follow a
+  // warm-start refinement (oa_max_steps > 0), which keeps the packed layout
+  jac_padded = jac_p == 32 && !apply_lds && !pivot_lds && oa_max_steps == 0 && !use_ql;
+  if (const char* r = std::getenv("XTBAMD_JAC_PAD")) jac_padded = jac_padded && std::atoi(r) != 0;
   intcut = std::max(20.0, 25.0 - 10.0 * std::log10(opt.accuracy));
   neglect = 10.0e-9 * opt.accuracy;
   scfconv = 1.e-6 * opt.accuracy;
@@ -701,6 +725,7 @@ void Batch::allocate(const Capacity& c) {
   HIPCHECK(hipMemset(eig_failed, 0, nmol * sizeof(int)));
   d_sub = alloc<MolDesc>(nmol);
   phase_act = alloc<int>(nmol);
+  open_act = alloc<int>(nmol);
   ns_c = alloc<double>(nmol);
   oa_act = alloc<int>(nmol);
   oa_ok = alloc<int>(nmol);
@@ -909,6 +934,7 @@ void Batch::rebuild() {
     if (jac_padded) poff += (long long)p.ld * p.ld;
     p.ev_off = desc[m].o0;
     p.q_off = qoff;
+    p.maxsw = jac_max_sweeps;
     js.big_max_ld = std::max(js.big_max_ld, p.ld);
     if (p.dim <= 64) {
       small.push_back(m);
@@ -1177,7 +1203,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   }
   if (js.nsmall)
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, lanes ? stream3 : stream, js.d_probs, js.d_small, Am,
-                       Wm, (const double*)js.scale, js.state, keep_w ? 1 : 0, rows ? 1 : 0);
+                       Wm, (const double*)js.scale, js.state, keep_w ? 1 : 0, rows ? 1 : 0, jac_tol);
   if (lanes) {
     jacobi_lanes(js, Am, Wm);
     HIPCHECK(hipEventRecord(ev_join, stream2));
@@ -1193,7 +1219,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
     // max_rounds rounds
     std::vector<int> sched;
     for (int b = 0; b < 2 * banded; ++b) sched.push_back(kBanded + b);
-    const int rmax = (kMaxSweeps + 1) * js.max_rounds;
+    const int rmax = (jac_max_sweeps + 1) * js.max_rounds;
     for (int r = 0; r <= rmax; ++r) sched.push_back(r);
     for (size_t si = 0; si < sched.size(); ++si) {
       const int r = sched[si];
@@ -1283,7 +1309,7 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       hipLaunchKernelGGL(k_jac_offmax, dim3(nto), dim3(256), 0, ls[g], js.d_probs, js.d_to + js.gto[g],
                          (const double*)Am, (const int*)js.state, js.offmax + nmol, 0);
   }
-  const int rmax = (kMaxSweeps + 1) * js.max_rounds;
+  const int rmax = (jac_max_sweeps + 1) * js.max_rounds;
   for (int r = 0; r <= rmax; ++r) {
     // timing events (and the work counts) on every kProfEvery-th round only,
     // counted over all eigensolves (17 is prime to the cross-round period 8,
@@ -1340,7 +1366,7 @@ void Batch::eigh_rows(JacSet& js, double* Am, double* Wm, double* evout) {
                      js.state, st, js.offmax, 1, 1.0);
   if (js.nsmall) {
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
-                       (const double*)js.scale, js.state, 0, 1);
+                       (const double*)js.scale, js.state, 0, 1, jac_tol);
     hipLaunchKernelGGL(k_jac_sort, dim3(1, nmol), dim3(256), 0, stream, js.d_probs, (const double*)Am, evout, perm, st,
                        64);
   }
@@ -1497,14 +1523,20 @@ void Batch::select(const std::vector<int>& mols) {
   sel_ = mols;
   nsub = (int)mols.size();
   std::vector<MolDesc> sub(nsub);
-  std::vector<int> flag(nmol, 0);
+  std::vector<int> flag(2 * nmol, 0);
+  nsub_open = 0;
   for (int k = 0; k < nsub; ++k) {
     if (mols[k] < 0 || mols[k] >= nmol) throw std::runtime_error("invalid molecule index");
     sub[k] = desc[mols[k]];
     flag[mols[k]] = 1;
+    if (sub[k].nopen > 0) {
+      flag[nmol + mols[k]] = 1;
+      nsub_open += 1;
+    }
   }
   if (nsub) HIPCHECK(hipMemcpyAsync(d_sub, sub.data(), nsub * sizeof(MolDesc), hipMemcpyHostToDevice, stream));
   HIPCHECK(hipMemcpyAsync(phase_act, flag.data(), nmol * sizeof(int), hipMemcpyHostToDevice, stream));
+  HIPCHECK(hipMemcpyAsync(open_act, flag.data() + nmol, nmol * sizeof(int), hipMemcpyHostToDevice, stream));
   HIPCHECK(hipStreamSynchronize(stream));
   sub_max_n = sub_max_nsh = sub_max_nao = sub_max_nd4 = 0;
   for (const MolDesc& d : sub) {
@@ -1643,6 +1675,13 @@ void Batch::setup() {
     HIPCHECK(hipEventRecord(ev_join, stream2));
     HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
   }
+  // molecules whose S^(-1/2) or EEQ guess failed do not iterate
+  hipLaunchKernelGGL(k_fail_fold, dim3(grid1(nmol, 64)), dim3(64), 0, stream, d_state, nmol, (const int*)ns_failed, 1,
+                     kFailOrtho, (const int*)phase_act);
+  if (opt.guess == 0)
+    hipLaunchKernelGGL(k_fail_fold, dim3(grid1(nmol, 64)), dim3(64), 0, stream, d_state, nmol, (const int*)eeq_failed,
+                       1, kFailEeq, (const int*)phase_act);
+  if (use_ql) HIPCHECK(hipMemsetAsync(eig_failed, 0, nmol * sizeof(int), stream));
   // restart guesses replace the initial charges (src/api/interface.f90:83-101, src/scf_module.F90:292)
   for (int m : sel_) {
     if (guess_qsh_.empty() || guess_qsh_[m].empty()) continue;
@@ -1698,11 +1737,21 @@ bool Batch::scc_iteration() {
                        d_desc, (const MolState*)d_state, W);
     refine(A, W);
   }
+  // a molecule whose eigensolve gave up fails and stops iterating here
+  auto fold_diag = [&]() {
+    hipLaunchKernelGGL(k_fail_fold, dim3(grid1(nmol, 64)), dim3(64), 0, stream, d_state, nmol,
+                       (const int*)jac_n.state + 1, kJacState, kFailDiag, (const int*)nullptr);
+  };
   if (use_ql && !warm) {
     eigh_rows(jac_n, A, W, emo);
+    fold_diag();
+    hipLaunchKernelGGL(k_fail_fold, dim3(grid1(nmol, 64)), dim3(64), 0, stream, d_state, nmol,
+                       (const int*)eig_failed, 1, kFailDiag, (const int*)nullptr);
     gemm(4, C, W, T, nullptr, perm, true);    // T = C W(perm, :)^T
   } else {
-    if (jacobi(jac_n, A, W, emo, true, warm, warm ? banded_passes : 0))
+    const bool rows = jacobi(jac_n, A, W, emo, true, warm, warm ? banded_passes : 0);
+    fold_diag();
+    if (rows)
       gemm(4, C, W, T, nullptr, perm, true);  // T = C W(perm, :)^T
     else
       gemm(3, C, W, T, nullptr, perm, true);  // T = C W(:, perm)
@@ -1800,10 +1849,18 @@ void Batch::gradient() {
     hipLaunchKernelGGL(k_solv_energy, dim3(ns), dim3(256), 0, stream, d_sub, (const double*)sv_B, (const double*)q,
                        sv_qbq, (const MolState*)nullptr);
   }
-  // bond orders from P S
+  // bond orders from P S; open-shell molecules from the spin densities
+  // Pa = C fa C^T and Pb = C fb C^T (src/scf_module.F90:856-873)
   gemm(0, P, S, A, nullptr, nullptr, false, 1.0, phase_act);
   hipLaunchKernelGGL(k_wbo, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub, xyz,
-                     at_ao0, at, el, (const double*)A, wbo);
+                     at_ao0, at, el, (const double*)A, wbo, 0, (const int*)nullptr);
+  if (nsub_open > 0)
+    for (int spin = 0; spin < 2; ++spin) {
+      gemm(2, C, C, T, spin ? foccb : focca, nullptr, false, 1.0, open_act, true);   // spin density
+      gemm(0, T, S, A, nullptr, nullptr, false, 1.0, open_act);
+      hipLaunchKernelGGL(k_wbo, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub,
+                         xyz, at_ao0, at, el, (const double*)A, wbo, 1 + spin, (const int*)open_act);
+    }
   hipLaunchKernelGGL(k_final, dim3(ns), dim3(256), 0, stream, d_sub, d_state, el, at, xyz, (const double*)grad,
                      (const double*)erep, (const double*)eat, (const double*)row_d);
   HIPCHECK(hipGetLastError());
@@ -1818,6 +1875,11 @@ void Batch::run() {
     if (++guard > opt.maxiter + 2) break;
   }
   gradient();
+}
+
+void Batch::charges(int m, double* out) const {
+  HIPCHECK(hipStreamSynchronize(stream));
+  HIPCHECK(hipMemcpy(out, q + desc[m].a0, desc[m].n * sizeof(double), hipMemcpyDeviceToHost));
 }
 
 MolResult Batch::result(int m, bool matrices) const {
@@ -1849,7 +1911,7 @@ MolResult Batch::result(int m, bool matrices) const {
     HIPCHECK(hipMemcpy(&gs, sv_gsasa + m, sizeof(double), hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(&bad, sv_failed + m, sizeof(int), hipMemcpyDeviceToHost));
     r.e_solv = (qbq + gs) + sv_gshift;
-    if (bad && !r.failed) r.failed = 7;
+    if (bad && !r.failed) r.failed = kFailSolv;
   }
   for (int k = 0; k < 3; ++k) r.dipole[k] = st.dipole[k];
   auto get = [&](std::vector<double>& v, const double* src, long long off, long long cnt) {
@@ -1916,6 +1978,15 @@ long long Server::submit(const MoleculeInput& mi) {
   HostMol hm;
   MolState st;
   Batch::describe(mi, batch.opt.maxiter, d, hm, st);
+  // a request that can never fit the packed arrays is refused here instead of
+  // waiting in the queue for room that will not come
+  const Capacity f = Batch::footprint(d);
+  const long long need[13] = {f.n, f.nsh, f.nao, f.nd4, f.mm, f.jj, f.xx, f.pp, f.ee, f.ne, f.v, f.h, f.g};
+  const long long have[13] = {batch.cap_.n, batch.cap_.nsh, batch.cap_.nao, batch.cap_.nd4, batch.cap_.mm,
+                              batch.cap_.jj, batch.cap_.xx, batch.cap_.pp, batch.cap_.ee, batch.cap_.ne,
+                              batch.cap_.v, batch.cap_.h, batch.cap_.g};
+  for (int k = 0; k < 13; ++k)
+    if (need[k] > have[k]) throw std::runtime_error("request larger than the HBM budget of the server");
   const long long id = next_id_++;
   queue_.emplace_back(id, mi);
   return id;
@@ -2170,6 +2241,7 @@ int xtbamd_batch_profile_result(xtbamd_batch b, int kernel, int* launches, doubl
 
 struct xtbamd_server_s {
   xtbamd::Server* server = nullptr;
+  std::vector<long long> finished;   // ids finished but not yet returned by xtbamd_server_finished
 };
 
 xtbamd_server xtbamd_server_new(double hbm_gb, int max_atoms, int max_slots, double accuracy, double etemp,
@@ -2216,7 +2288,7 @@ int xtbamd_server_step(xtbamd_server s) {
 int xtbamd_server_finished(xtbamd_server s, long long* ids, int max_ids) {
   try {
     if (!s) throw std::runtime_error("invalid server");
-    static thread_local std::vector<long long> pending;
+    std::vector<long long>& pending = s->finished;
     std::vector<long long> f = s->server->take_finished();
     pending.insert(pending.end(), f.begin(), f.end());
     const int n = std::min<int>(max_ids, (int)pending.size());
@@ -2272,6 +2344,15 @@ int xtbamd_batch_get(xtbamd_batch b, int m, xtbamd_result* out) {
     const bool mats = out->coeff || out->density;
     xtbamd::MolResult r = b->batch->result(m, mats);
     copy_result(r, out);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_charges(xtbamd_batch b, int m, double* q) {
+  try {
+    if (!b || !q || m < 0 || m >= b->batch->size() || !b->batch->occupied(m))
+      throw std::runtime_error("invalid charge request");
+    b->batch->charges(m, q);
     return 0;
   } catch (const std::exception& e) { return fail(e); }
 }

@@ -655,10 +655,14 @@ __global__ void k_pew_scale(const MolDesc* __restrict__ desc, const double* __re
 }
 
 // Wiberg/Mayer bond orders from PS (get_wiberg: src/scc_core.f90:1208-1245)
+// mode 0: wbo = sum (closed shell, src/scc_core.f90:1208-1245); mode 1:
+// wbo = 2 sum over (Pa S) and mode 2: wbo += 2 sum over (Pb S) (open shell,
+// src/scc_core.f90:1248-1295), only for molecules with mask[slot] set.
 __global__ void k_wbo(const MolDesc* __restrict__ desc, const double* __restrict__ xyz, const int* __restrict__ at_ao0,
                       const int* __restrict__ at, const Element* __restrict__ el, const double* __restrict__ PS,
-                      double* wbo) {
+                      double* wbo, int mode, const int* __restrict__ mask) {
   const MolDesc md = desc[blockIdx.y];
+  if (mask && !mask[md.mol]) return;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n = md.n;
   if (t >= n * n) return;
@@ -684,7 +688,9 @@ __global__ void k_wbo(const MolDesc* __restrict__ desc, const double* __restrict
           xs += PS[md.m0 + (long long)k * N + m] * PS[md.m0 + (long long)m * N + k];
     }
   }
-  wbo[md.p0 + t] = xs;
+  if (mode == 0) wbo[md.p0 + t] = xs;
+  else if (mode == 1) wbo[md.p0 + t] = 2.0 * xs;
+  else wbo[md.p0 + t] += 2.0 * xs;
 }
 
 }  // namespace xtbamd

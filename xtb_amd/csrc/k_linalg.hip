@@ -390,12 +390,11 @@ __device__ __forceinline__ bool jac_boundary(int rounds, int r) {
 constexpr int kJacState = 8;
 // XCDs of the gfx950 part (MI355X): workgroup b of a launch runs on XCD b mod kXcds
 constexpr int kXcds = 8;
-constexpr int kMaxSweeps = 40;
 
 // problems of dimension <= 64: one workgroup solves to convergence in LDS
 __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict__ probs, const int* __restrict__ list,
                                                    double* A, double* W, const double* __restrict__ scale_in,
-                                                   int* state, int keep_w, int rows_out) {
+                                                   int* state, int keep_w, int rows_out, double tolrel) {
   __shared__ double a[64 * kLd];
   __shared__ double q[64 * kLd];
   __shared__ double cs[96];
@@ -414,9 +413,17 @@ __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict_
     q[u * kLd + v] = (u == v) ? 1.0 : 0.0;
   }
   __syncthreads();
-  const double tol = 1e-12 * scale_in[pidx];
-  const int nrot = jacobi64(a, q, tol, kMaxSweeps, cs, pq, flags, red);
+  const double tol = tolrel * scale_in[pidx];
+  const int nrot = jacobi64(a, q, tol, pr.maxsw, cs, pq, flags, red);
   __syncthreads();
+  // the sweeps stop at convergence or after maxsw of them: given up when the
+  // largest off-diagonal magnitude is still above the tolerance
+  double offm = 0.0;
+  for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+    const int u = e >> 6, v = e & 63;
+    if (u < v) offm = fmax(offm, fabs(a[u * kLd + v]));
+  }
+  offm = block_max<256>(offm, red);
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
     const int u = e / n, v = e % n;
     Ap[(long long)u * ld + v] = a[u * kLd + v];
@@ -439,7 +446,7 @@ __global__ void __launch_bounds__(256) k_jac_small(const JacobiProb* __restrict_
   }
   if (threadIdx.x == 0) {
     state[pidx * kJacState + 0] = 1;
-    state[pidx * kJacState + 1] = 0;
+    state[pidx * kJacState + 1] = offm > tol ? 1 : 0;
     state[pidx * kJacState + 5] = nrot;
   }
 }
@@ -1257,7 +1264,7 @@ __device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2],
 // sweep's parity); the wave of pair 0 clears the buffer the current sweep
 // will fill and, for a finished problem, records the outcome in state, which
 // every wave of the problem decides alike (done when max <= tol, failed
-// after kMaxSweeps sweeps).
+// after the problem's maxsw sweeps).
 constexpr int kCrossPeriod = 8;
 
 template <bool CROSSK>
@@ -1277,7 +1284,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const int sweep = r / pr.rounds;
     const double m = __longlong_as_double((long long)offmax[((sweep + 1) & 1) * stride + it.x]);
     const bool conv = m <= tolrel * scale_in[it.x];
-    const bool fail = !conv && sweep >= kMaxSweeps;
+    const bool fail = !conv && sweep >= pr.maxsw;
     if (it.y == 0 && threadIdx.x == 0) {
       offmax[(sweep & 1) * stride + it.x] = 0ull;
       if (conv || fail) {
@@ -1684,7 +1691,7 @@ __global__ void __launch_bounds__(256) k_jac_offmax(const JacobiProb* __restrict
 
 // convergence test of the problems at a sweep boundary (per problem, so a
 // problem's rounds never depend on the other problems of the batch): done
-// when every off-diagonal element is <= tol, failed after kMaxSweeps sweeps;
+// when every off-diagonal element is <= tol, failed after maxsw sweeps;
 // counts the problems still running into nactive.  group >= 0 restricts the
 // test to the problems of that round lane.
 __global__ void __launch_bounds__(256) k_jac_mark(const JacobiProb* __restrict__ probs, int nprob,
@@ -1708,7 +1715,7 @@ __global__ void __launch_bounds__(256) k_jac_mark(const JacobiProb* __restrict__
         st[5] = sweep;
         continue;
       }
-      if (sweep >= kMaxSweeps) {
+      if (sweep >= pr.maxsw) {
         st[0] = 1;
         st[1] = 1;
         st[5] = sweep;

@@ -47,6 +47,8 @@ def library():
     lib.xtbamd_batch_set_guess.restype = ctypes.c_int
     lib.xtbamd_batch_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Result)]
     lib.xtbamd_batch_get.restype = ctypes.c_int
+    lib.xtbamd_batch_charges.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    lib.xtbamd_batch_charges.restype = ctypes.c_int
     lib.xtbamd_batch_free.argtypes = [ctypes.c_void_p]
     lib.xtbamd_batch_free.restype = None
     lib.xtbamd_device_count.restype = ctypes.c_int
@@ -173,6 +175,12 @@ class Batch:
                                                           ctypes.byref(t), ctypes.byref(w)))
         return n.value, t.value, w.value
 
+    def charges(self, m):
+        """Current atomic charges of molecule m (after setup(): the initial guess)."""
+        q = np.zeros(int(self.natoms[m]))
+        self._check(self._lib.xtbamd_batch_charges(self._h, m, _ptr(q)))
+        return q
+
     def __len__(self):
         return len(self.natoms)
 
@@ -290,13 +298,31 @@ class Server:
             pass
 
 
+# MolState failure codes (xtbamd_result.failed)
+FAILURES = {2: "orthonormalization of the overlap matrix failed", 3: "diagonalization of Hamiltonian failed",
+            4: "EEQ guess charges failed", 7: "solvation surface setup failed"}
+
+
+class SolverFailure(RuntimeError):
+    """A solver of the single point gave up (result["failed"] != 0); the
+    result is attached as .result."""
+
+    def __init__(self, result):
+        super().__init__(FAILURES.get(result["failed"], f"single point failed (code {result['failed']})"))
+        self.result = result
+
+
 def singlepoint(numbers, positions, charge=0.0, uhf=0, accuracy=1.0, etemp=300.0, maxiter=250, guess="eeq",
                 matrices=False, solvent=None, solv_kernel="still", alpb=False):
+    """One single point; raises SolverFailure when a solver gave up."""
     b = Batch([(numbers, positions, charge, uhf)], accuracy, etemp, maxiter, guess)
     try:
         if solvent:
             b.set_solvent(solvent, solv_kernel, alpb)
         b.run()
-        return b.result(0, matrices)
+        r = b.result(0, matrices)
     finally:
         b.close()
+    if r["failed"]:
+        raise SolverFailure(r)
+    return r

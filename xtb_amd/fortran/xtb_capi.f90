@@ -832,10 +832,32 @@ contains
          write(env%unit, '(a, f20.12, a, f12.6, a, i0)') "  TOTAL ENERGY ", er%energy, " Eh   GAP ", &
             & er%hl_gap_ev, " eV   SCC iterations ", er%iterations
       end if
-      if (er%converged == 0) then
+      if (er%failed /= 0) then
+         call env_error(env, failure_message(er%failed), 'scc')
+      else if (er%converged == 0) then
          call env_error(env, "Self consistent charge iterator did not converge", 'scf')
       end if
    end subroutine singlepoint_api
+
+   !> message of an engine failure code (xtbamd_result%failed): the solver
+   !> that gave up, in the reference's wording where it has one
+   !> (src/scc_core.f90:478, src/scf_module.F90:642)
+   function failure_message(code) result(msg)
+      integer(c_int), intent(in) :: code
+      character(len=:), allocatable :: msg
+      select case (code)
+      case (2)
+         msg = "Orthonormalization of the overlap matrix failed"
+      case (3)
+         msg = "Diagonalization of Hamiltonian failed"
+      case (4)
+         msg = "EEQ guess charges failed"
+      case (7)
+         msg = "Solvation surface setup failed"
+      case default
+         msg = "Self consistent charge iterator terminated"
+      end select
+   end function failure_message
 
    !> pass the calculator's implicit solvent to an engine batch; frees the
    !> batch and logs the error on failure
@@ -1029,7 +1051,7 @@ contains
          er%gradient = c_loc(grad(1, 1, m))
          rc = engine_get(handle, int(m - 1, c_int), er)
          dip(:, m) = er%dipole
-         conv = conv .and. er%converged /= 0
+         conv = conv .and. er%converged /= 0 .and. er%failed == 0
       end do
       call engine_free(handle)
 
