@@ -41,6 +41,17 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
   return r;
 }
 
+// D4 charge scaling of a reference with charge qref at the effective charge
+// qmod (both including the effective nuclear charge) and its qmod derivative
+__device__ __forceinline__ double zeta_d(double a, double c, double qref, double qmod) {
+  if (qmod < 0.0) return exp(a);
+  return exp(a * (1.0 - exp(c * (1.0 - qref / qmod))));
+}
+__device__ __forceinline__ double dzeta_d(double a, double c, double qref, double qmod) {
+  if (qmod < 0.0) return 0.0;
+  return -a * c * exp(c * (1.0 - qref / qmod)) * zeta_d(a, c, qref, qmod) * qref / (qmod * qmod);
+}
+
 // index t of the strictly lower triangle -> (i, j), i > j
 __device__ __forceinline__ void tri_index(long long t, int& i, int& j) {
   double d = sqrt(8.0 * (double)t + 1.0);

@@ -16,6 +16,8 @@
 #include "engine.h"
 #include "gfn2_common.h"
 // kernels (single translation unit)
+#include "k_aes.hip"
+#include "k_d4.hip"
 #include "k_grad.hip"
 #include "k_linalg.hip"
 #include "k_scc.hip"
@@ -678,7 +680,7 @@ void Batch::allocate(const Capacity& c) {
   q = alloc<double>(tn); dipm = alloc<double>(3 * tn); qp = alloc<double>(6 * tn); radcn = alloc<double>(tn);
   vs = alloc<double>(tn); vd = alloc<double>(3 * tn); vq = alloc<double>(6 * tn); ashift = alloc<double>(tn);
   grad = alloc<double>(3 * tn); dhdcn = alloc<double>(tn); dedcn = alloc<double>(tn);
-  tmp2 = alloc<double>(tn); eat = alloc<double>(tn); ae01 = alloc<double>(4 * tn);
+  tmp2 = alloc<double>(tn); eat = alloc<double>(tn); ae01 = alloc<double>(2 * tn);
   sh_at = alloc<int>(tsh); sh_l = alloc<int>(tsh); sh_ao0 = alloc<int>(tsh); sh_m = alloc<int>(tsh);
   at_sh0 = alloc<int>(tn); at_ao0 = alloc<int>(tn);
   qsh = alloc<double>(tsh); zsh = alloc<double>(tsh); shift = alloc<double>(tsh);
@@ -1617,7 +1619,8 @@ void Batch::setup() {
                      at_ao0, selfe, intcut, S, H0, H0n, D, Q);
   hipLaunchKernelGGL(k_coulomb, dim3(grid1((long long)sub_max_nsh * sub_max_nsh, 256), ns), dim3(256), 0, stream,
                      d_sub, el, at, xyz, sh_at, sh_m, J);
-  hipLaunchKernelGGL(k_d4_weights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, cn_d4, gw);
+  hipLaunchKernelGGL(k_d4_weights, dim3(grid1(sub_max_nd4, 64), ns), dim3(64), 0, stream, d_sub, el, at, d4_at, d4_ref,
+                     cn_d4, gw);
   hipLaunchKernelGGL(k_dispmat, dim3(grid1((long long)sub_max_nd4 * sub_max_nd4, 256), ns), dim3(256), 0, stream,
                      d_sub, el, at, xyz, d4_at, d4_ref, gw, c6ref, dispmat);
   hipLaunchKernelGGL(k_aes_setup, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub,
@@ -1708,7 +1711,7 @@ bool Batch::scc_iteration() {
   // vq) on stream2 beside the shell and D4 potentials; joined before the Fock build
   HIPCHECK(hipEventRecord(ev_fork, stream));
   HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
-  hipLaunchKernelGGL(k_aes_pot, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream2, d_desc, el, at, xyz, q, dipm, qp,
+  hipLaunchKernelGGL(k_aes_pot, dim3(std::max(1, max_n), nmol), dim3(64), 0, stream2, d_desc, el, at, xyz, q, dipm, qp,
                      gab3, gab5, 1, vs, vd, vq, (const MolState*)d_state);
   hipLaunchKernelGGL(k_shell_pot, dim3(grid1(max_nsh, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)J,
                      (const double*)qsh, (const double*)shgam, shift, 1, (const MolState*)d_state);
@@ -1770,7 +1773,7 @@ bool Batch::scc_iteration() {
   // the AES energy terms on stream2 beside the shell and D4 energy terms
   HIPCHECK(hipEventRecord(ev_fork, stream));
   HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
-  hipLaunchKernelGGL(k_aes_energy, dim3(grid1(max_n, 64), nmol), dim3(64), 0, stream2, d_desc, el, at, xyz, q, dipm,
+  hipLaunchKernelGGL(k_aes_energy, dim3(std::max(1, max_n), nmol), dim3(64), 0, stream2, d_desc, el, at, xyz, q, dipm,
                      qp, gab3, gab5, ae01, (const MolState*)d_state);
   hipLaunchKernelGGL(k_shell_pot, dim3(grid1(max_nsh, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)J,
                      (const double*)qsh, (const double*)shgam, jq, 0, (const MolState*)d_state);
@@ -1806,7 +1809,7 @@ void Batch::gradient() {
   hipLaunchKernelGGL(k_pew_scale, dim3(grid1(sub_max_nao, 64), ns), dim3(64), 0, stream, d_sub, (const double*)focc,
                      (const double*)emo, row_pd);
   gemm(2, C, C, T, row_pd, nullptr, false, 1.0, phase_act, true);   // symmetric
-  hipLaunchKernelGGL(k_aes_pot, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, q, dipm, qp,
+  hipLaunchKernelGGL(k_aes_pot, dim3(std::max(1, sub_max_n), ns), dim3(64), 0, stream, d_sub, el, at, xyz, q, dipm, qp,
                      gab3, gab5, 0, vs, vd, vq, (const MolState*)nullptr);
   hipLaunchKernelGGL(k_zero_atoms, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, grad,
                      (double*)nullptr);
@@ -1819,7 +1822,7 @@ void Batch::gradient() {
                      stream2, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)selfe, (const double*)dsedcn,
                      (const double*)P, (const double*)T, (const double*)H0n, (const double*)S, (const double*)shift,
                      (const double*)vs, (const double*)vd, (const double*)vq, intcut, (const int2*)pair_list, gpair);
-  hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_sub, xyz, q, dipm, qp, radcn, gab3, gab5, grad, tmp2);
+  hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_sub, xyz, q, dipm, qp, radcn, grad, tmp2);
   hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, cn_cov, q,
                      zetavec, zerovec, zetadcn, zerodcn);
   hipLaunchKernelGGL(k_d4_c6, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub, el,

@@ -280,135 +280,6 @@ __global__ void __launch_bounds__(64) k_grad_h0(const MolDesc* __restrict__ desc
   }
 }
 
-// AES gradient (aniso_grad: src/aespot.F90:1070-1165) and the CN-radius weight tmp2
-__global__ void __launch_bounds__(64) k_aes_grad(const MolDesc* __restrict__ desc, const double* __restrict__ xyz,
-                                                 const double* __restrict__ q, const double* __restrict__ dipm,
-                                                 const double* __restrict__ qp, const double* __restrict__ radcn,
-                                                 const double* __restrict__ gab3, const double* __restrict__ gab5,
-                                                 double* grad, double* tmp2out) {
-  __shared__ double sh[4];
-  const MolDesc md = desc[blockIdx.y];
-  const int i = blockIdx.x;
-  if (i >= md.n) return;
-  const int n = md.n;
-  const double* x = xyz + 3 * md.a0;
-  const double* qa = q + md.a0;
-  const double* dm = dipm + 3 * md.a0;
-  const double* qm = qp + 6 * md.a0;
-  const int lin[3][3] = {{0, 1, 3}, {1, 2, 4}, {3, 4, 5}};
-  const double kdmp3 = 3.0, kdmp5 = 4.0;
-  const double q1 = qa[i];
-  const double* rr = x + 3 * i;
-  const double* dip = dm + 3 * i;
-  const double* qp1 = qm + 6 * i;
-  double g[3] = {0, 0, 0}, tmp2 = 0.0;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    if (j == i) continue;
-    const double rij[3] = {x[3 * j] - rr[0], x[3 * j + 1] - rr[1], x[3 * j + 2] - rr[2]};
-    const double r2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
-    const double rabi = 1.0 / sqrt(r2);
-    const double rco = 0.5 * (radcn[md.a0 + i] + radcn[md.a0 + j]);
-    const double damp1 = 1.0 / (1.0 + 6.0 * pow(rco * rabi, kdmp3));
-    const double ddamp1 = -kdmp3 * rabi * (damp1 * damp1 - damp1);
-    const double dgab3 = (-3.0 * rabi * pow(rabi, 3.0)) * damp1 + ddamp1 * pow(rabi, 3.0);
-    const double damp2 = 1.0 / (1.0 + 6.0 * pow(rco * rabi, kdmp5));
-    const double ddamp2 = -kdmp5 * rabi * (damp2 * damp2 - damp2);
-    const double dgab5 = (-5.0 * rabi * pow(rabi, 5.0)) * damp2 + ddamp2 * pow(rabi, 5.0);
-    const double g3 = gab3[md.p0 + (long long)i * n + j], g5 = gab5[md.p0 + (long long)i * n + j];
-    double ed = 0, edd = 0, eq = 0;
-    for (int k = 0; k < 3; ++k) {
-      ed = ed + qa[j] * dip[k] * rij[k];
-      ed = ed - dm[3 * j + k] * q1 * rij[k];
-      const double tt = q1 * dm[3 * j + k] - qa[j] * dip[k];
-      g[k] += g3 * tt;
-      double ddm2 = 0, ddm3a = 0, ddm3b = 0, qqa = 0, qqb = 0;
-      for (int l = 0; l < 3; ++l) {
-        const int kl = lin[l][k];
-        const double t = rij[l] * rij[k], t3 = 3.0 * t;
-        eq = eq + qa[j] * qp1[kl] * t;
-        eq = eq + qm[6 * j + kl] * q1 * t;
-        edd = edd - dm[3 * j + k] * dip[l] * t3;
-        ddm2 = ddm2 + dm[3 * j + l] * dip[l];
-        ddm3a = ddm3a + dip[l] * rij[l];
-        ddm3b = ddm3b + dm[3 * j + l] * rij[l];
-        qqa = qqa + rij[l] * qm[6 * j + kl];
-        qqb = qqb + rij[l] * qp1[kl];
-      }
-      edd = edd + dm[3 * j + k] * dip[k] * r2;
-      g[k] += -2.0 * g5 * ddm2 * rij[k];
-      g[k] += 3.0 * g5 * ddm3a * dm[3 * j + k];
-      g[k] += 3.0 * g5 * ddm3b * dip[k];
-      g[k] += -2.0 * g5 * qqa * q1;
-      g[k] += -2.0 * g5 * qqb * qa[j];
-    }
-    for (int k = 0; k < 3; ++k) {
-      const double dxi = rij[k] * rabi;
-      g[k] -= ed * dgab3 * dxi;
-      g[k] -= (eq + edd) * dgab5 * dxi;
-    }
-    const double rab = 0.50 * (radcn[md.a0 + i] + radcn[md.a0 + j]);
-    tmp2 += ed * kdmp3 * g3 * (damp1 / rab) * pow(rab * rabi, kdmp3);
-    tmp2 += (eq + edd) * kdmp5 * g5 * (damp2 / rab) * pow(rab * rabi, kdmp5);
-  }
-  for (int k = 0; k < 3; ++k) g[k] = block_sum<64>(g[k], sh);
-  tmp2 = block_sum<64>(tmp2, sh);
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < 3; ++k) grad[3 * (md.a0 + i) + k] += g[k];
-    tmp2out[md.a0 + i] = 3.0 * tmp2;
-  }
-}
-
-// D4 reference weights at the covalent CN (weight_references: src/disp/dftd4.F90:948-1065)
-__global__ void k_d4_refweights(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
-                                const int* __restrict__ at, const double* __restrict__ cn, const double* __restrict__ q,
-                                double* zetavec, double* zerovec, double* zetadcn, double* zerodcn) {
-  const MolDesc md = desc[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= md.n) return;
-  const Element& e = el[at[md.a0 + i]];
-  const double zi = e.d4zeff, gi = kD4gc * e.d4gam, c = cn[md.a0 + i], qi = q[md.a0 + i];
-  double norm = 0.0, dnorm = 0.0;
-  for (int r = 0; r < e.nref; ++r)
-    for (int k = 1; k <= e.ncount[r]; ++k) {
-      const double twf = k * kD4wf;
-      const double val = -twf * (c - e.refcn[r]) * (c - e.refcn[r]);
-      const double gw = (val < -200.0) ? 0.0 : exp(val);
-      norm += gw;
-      dnorm += 2 * twf * (e.refcn[r] - c) * gw;
-    }
-  norm = (norm > 1e-80) ? 1.0 / norm : 0.0;
-  double maxcn = -1e300;
-  for (int r = 0; r < e.nref; ++r) maxcn = fmax(maxcn, e.refcn[r]);
-  const int base = (md.a0 + i) * XTB_MAX_REF;
-  for (int r = 0; r < XTB_MAX_REF; ++r) {
-    double z1 = 0, z0 = 0, d1 = 0, d0 = 0;
-    if (r < e.nref) {
-      double expw = 0.0, expd = 0.0;
-      for (int k = 1; k <= e.ncount[r]; ++k) {
-        const double twf = k * kD4wf;
-        const double val = -twf * (c - e.refcn[r]) * (c - e.refcn[r]);
-        const double gw = (val < -200.0) ? 0.0 : exp(val);
-        expw += gw;
-        expd += 2 * twf * (e.refcn[r] - c) * gw;
-      }
-      double gwk = expw * norm;
-      if (norm == 0.0) gwk = (fabs(maxcn - e.refcn[r]) < 1e-12) ? 1.0 : 0.0;
-      const double dgwk = expd * norm - expw * dnorm * norm * norm;
-      const double qr = e.refq[r] + zi;
-      const double za = (qi + zi < 0.0) ? exp(kD4ga) : exp(kD4ga * (1.0 - exp(gi * (1.0 - qr / (qi + zi)))));
-      const double zb = (zi < 0.0) ? exp(kD4ga) : exp(kD4ga * (1.0 - exp(gi * (1.0 - qr / zi))));
-      z1 = za * gwk;
-      z0 = zb * gwk;
-      d1 = za * dgwk;
-      d0 = zb * dgwk;
-    }
-    zetavec[base + r] = z1;
-    zerovec[base + r] = z0;
-    zetadcn[base + r] = d1;
-    zerodcn[base + r] = d0;
-  }
-}
-
 // atomic C6 and dC6/dCN for atom pairs (get_atomic_c6: src/disp/dftd4.F90:1069-1147),
 // for the charge-scaled (c6, dc6) and the zero-charge (c60, dc60) weights
 __global__ void k_d4_c6(const MolDesc* __restrict__ desc, const Element* __restrict__ el, const int* __restrict__ at,

@@ -28,113 +28,6 @@ __global__ void __launch_bounds__(256) k_shell_pot(const MolDesc* __restrict__ d
   }
 }
 
-// AES potentials with (setvsdq: src/aespot.F90:727-848) or without
-// (setdvsdq: :861-952) the moment-origin shift terms; one thread per atom
-__global__ void k_aes_pot(const MolDesc* __restrict__ desc, const Element* __restrict__ el, const int* __restrict__ at,
-                          const double* __restrict__ xyz, const double* __restrict__ q, const double* __restrict__ dipm,
-                          const double* __restrict__ qp, const double* __restrict__ gab3, const double* __restrict__ gab5,
-                          int with_shift, double* vs, double* vd, double* vq, const MolState* __restrict__ mst) {
-  const MolDesc md = desc[blockIdx.y];
-  if (mst && !mst[blockIdx.y].active) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= md.n) return;
-  const int n = md.n;
-  const double* x = xyz + 3 * md.a0;
-  const double* qa = q + md.a0;
-  const double* dm = dipm + 3 * md.a0;
-  const double* qm = qp + 6 * md.a0;
-  const double* g3r = gab3 + md.p0 + (long long)i * n;
-  const double* g5r = gab5 + md.p0 + (long long)i * n;
-  const double ra[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
-  double stmp = 0.0, dtmp[3] = {0, 0, 0}, qtmp[6] = {0, 0, 0, 0, 0, 0};
-  const int lin[3][3] = {{0, 1, 3}, {1, 2, 4}, {3, 4, 5}};
-  for (int j = 0; j < n; ++j) {
-    if (j == i) continue;
-    const double g3 = g3r[j], g5 = g5r[j];
-    const double dra[3] = {ra[0] - x[3 * j], ra[1] - x[3 * j + 1], ra[2] - x[3 * j + 2]};
-    const double qj = qa[j];
-    double dum5a = 0.0, r2a = 0.0, r2ab = 0.0, t1a = 0.0, t2a = 0.0, t3a = 0.0;
-    for (int l1 = 0; l1 < 3; ++l1) {
-      r2a += ra[l1] * ra[l1];
-      r2ab += dra[l1] * dra[l1];
-      t1a += ra[l1] * dra[l1];
-      t2a += dm[3 * j + l1] * dra[l1];
-      t3a += ra[l1] * dm[3 * j + l1];
-      for (int l2 = 0; l2 < 3; ++l2) {
-        dum5a = dum5a - qm[6 * j + lin[l1][l2]] * dra[l1] * dra[l2];
-        if (with_shift) dum5a = dum5a - 1.50 * qj * dra[l1] * dra[l2] * ra[l1] * ra[l2];
-        if (l2 >= l1) continue;
-        const int ki = l1 + l2 + 2;  // xy, xz, yz in integral order
-        qtmp[ki] = qtmp[ki] - 3.0 * qj * g5 * dra[l2] * dra[l1];
-      }
-      qtmp[l1] = qtmp[l1] - 1.50 * qj * g5 * dra[l1] * dra[l1];
-    }
-    double dum3a;
-    if (with_shift) {
-      dum3a = -t1a * qj - t2a;
-      dum5a = dum5a + t3a * r2ab - 3.0 * t1a * t2a + 0.50 * qj * r2a * r2ab;
-    } else {
-      dum3a = -t2a;
-    }
-    stmp = stmp + dum5a * g5 + dum3a * g3;
-    for (int l1 = 0; l1 < 3; ++l1) {
-      const double d3a = dra[l1] * qj;
-      double d5a = 3.0 * dra[l1] * t2a - r2ab * dm[3 * j + l1];
-      if (with_shift) d5a = d5a - qj * r2ab * ra[l1] + 3.0 * qj * dra[l1] * t1a;
-      dtmp[l1] = dtmp[l1] + d3a * g3 + d5a * g5;
-      qtmp[l1] = qtmp[l1] + 0.50 * r2ab * qj * g5;
-    }
-  }
-  double vsi = stmp, vdi[3] = {dtmp[0], dtmp[1], dtmp[2]};
-  double vqi[6] = {qtmp[0], qtmp[1], qtmp[2], qtmp[3], qtmp[4], qtmp[5]};
-  const Element& e = el[at[md.a0 + i]];
-  const double qs1 = e.dipk * 2.0, qs2 = e.quadk * 6.0;
-  const double* dmi = dm + 3 * i;
-  const double* qmi = qm + 6 * i;
-  double t3a = 0.0, t2a = 0.0;
-  for (int l1 = 0; l1 < 3; ++l1) {
-    if (with_shift) t3a = t3a + ra[l1] * dmi[l1] * qs1;
-    vdi[l1] = vdi[l1] - qs1 * dmi[l1];
-    for (int l2 = 0; l2 < l1; ++l2) {
-      const int ll = lin[l1][l2], ki = l1 + l2 + 2;
-      vqi[ki] = vqi[ki] - qmi[ll] * qs2;
-      if (with_shift) {
-        t3a = t3a - ra[l1] * ra[l2] * qmi[ll] * qs2;
-        vdi[l1] = vdi[l1] + ra[l2] * qmi[ll] * qs2;
-        vdi[l2] = vdi[l2] + ra[l1] * qmi[ll] * qs2;
-      }
-    }
-    const int ll = lin[l1][l1];
-    vqi[l1] = vqi[l1] - qmi[ll] * qs2 * 0.50;
-    if (with_shift) {
-      t3a = t3a - ra[l1] * ra[l1] * qmi[ll] * qs2 * 0.50;
-      vdi[l1] = vdi[l1] + ra[l1] * qmi[ll] * qs2;
-    }
-    t2a = t2a + qmi[ll];
-  }
-  if (with_shift) vsi = vsi + t3a;
-  t2a = t2a * e.quadk;
-  for (int l1 = 0; l1 < 3; ++l1) {
-    vqi[l1] = vqi[l1] + t2a;
-    if (with_shift) {
-      vdi[l1] = vdi[l1] - 2.0 * ra[l1] * t2a;
-      vsi = vsi + t2a * ra[l1] * ra[l1];
-    }
-  }
-  vs[md.a0 + i] = vsi;
-  for (int k = 0; k < 3; ++k) vd[3 * (md.a0 + i) + k] = vdi[k];
-  for (int k = 0; k < 6; ++k) vq[6 * (md.a0 + i) + k] = vqi[k];
-}
-
-__device__ __forceinline__ double zeta_d(double a, double c, double qref, double qmod) {
-  if (qmod < 0.0) return exp(a);
-  return exp(a * (1.0 - exp(c * (1.0 - qref / qmod))));
-}
-__device__ __forceinline__ double dzeta_d(double a, double c, double qref, double qmod) {
-  if (qmod < 0.0) return 0.0;
-  return -a * c * exp(c * (1.0 - qref / qmod)) * zeta_d(a, c, qref, qmod) * qref / (qmod * qmod);
-}
-
 // zeta scaling of the D4 references (disppot / edisp_scc: src/disp/dftd4.F90:735-864)
 __global__ void k_d4_zeta(const MolDesc* __restrict__ desc, const Element* __restrict__ el, const int* __restrict__ at,
                           const int* __restrict__ d4_at, const int* __restrict__ d4_ref, const double* __restrict__ gw,
@@ -441,59 +334,6 @@ __global__ void k_pop_reduce(const MolDesc* __restrict__ desc, const Element* __
   for (int c = 0; c < 6; ++c) qp[6 * (md.a0 + i) + c] = qq[c];
 }
 
-// AES energy partials of atom i (aniso_electro_cpu: src/aespot.F90:569-662):
-// e01, e02, e11 over j < i and the XC-kernel term of atom i
-__global__ void k_aes_energy(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
-                             const int* __restrict__ at, const double* __restrict__ xyz, const double* __restrict__ q,
-                             const double* __restrict__ dipm, const double* __restrict__ qp,
-                             const double* __restrict__ gab3, const double* __restrict__ gab5, double* ae,
-                             const MolState* __restrict__ state) {
-  const MolDesc md = desc[blockIdx.y];
-  if (!state[blockIdx.y].active) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= md.n) return;
-  const int n = md.n;
-  const double* x = xyz + 3 * md.a0;
-  const double* qa = q + md.a0;
-  const double* dm = dipm + 3 * md.a0;
-  const double* qm = qp + 6 * md.a0;
-  const int idx[3][3] = {{0, 1, 3}, {1, 2, 4}, {3, 4, 5}};
-  const double* dp1 = dm + 3 * i;
-  const double* qp1 = qm + 6 * i;
-  const double q1 = qa[i];
-  double e01 = 0, e02 = 0, e11 = 0;
-  for (int j = 0; j < i; ++j) {
-    const double rij[3] = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
-    const double r2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
-    double ed = 0, eq = 0, edd = 0;
-    for (int k = 0; k < 3; ++k) {
-      ed = ed + qa[j] * dp1[k] * rij[k];
-      ed = ed - dm[3 * j + k] * q1 * rij[k];
-      for (int l = 0; l < 3; ++l) {
-        const int kl = idx[l][k];
-        const double tt = rij[l] * rij[k], tt3 = 3.0 * tt;
-        eq = eq + qa[j] * qp1[kl] * tt;
-        eq = eq + qm[6 * j + kl] * q1 * tt;
-        edd = edd - dm[3 * j + k] * dp1[l] * tt3;
-      }
-      edd = edd + dm[3 * j + k] * dp1[k] * r2;
-    }
-    e01 += ed * gab3[md.p0 + (long long)i * n + j];
-    e02 += eq * gab5[md.p0 + (long long)i * n + j];
-    e11 += edd * gab5[md.p0 + (long long)i * n + j];
-  }
-  const Element& e = el[at[md.a0 + i]];
-  const double tt = dp1[0] * dp1[0] + dp1[1] * dp1[1] + dp1[2] * dp1[2];
-  double tt3 = 0.0;
-  for (int k = 0; k < 3; ++k)
-    for (int l = 0; l < 3; ++l) tt3 += qp1[idx[l][k]] * qp1[idx[l][k]];
-  const double ep = e.dipk * tt + tt3 * e.quadk;
-  ae[4 * (md.a0 + i) + 0] = e01;
-  ae[4 * (md.a0 + i) + 1] = e02;
-  ae[4 * (md.a0 + i) + 2] = e11;
-  ae[4 * (md.a0 + i) + 3] = ep;
-}
-
 // Energy, convergence test and charge mixing of one molecule (one workgroup):
 // src/scc_core.f90:523-632 with broyden (src/broyden.f90:37-153).
 __global__ void __launch_bounds__(256) k_scc_control(
@@ -521,23 +361,18 @@ __global__ void __launch_bounds__(256) k_scc_control(
   double eh = 0.0;
   for (int i = tid; i < md.nao; i += blockDim.x) eh += row_e[md.o0 + i];
   eh = block_sum<256>(eh, sh);
-  double a01 = 0, a02 = 0, a11 = 0, apol = 0;
+  double eaes = 0, apol = 0;
   for (int i = tid; i < md.n; i += blockDim.x) {
-    a01 += ae[4 * (md.a0 + i)];
-    a02 += ae[4 * (md.a0 + i) + 1];
-    a11 += ae[4 * (md.a0 + i) + 2];
-    apol += ae[4 * (md.a0 + i) + 3];
+    eaes += ae[2 * (md.a0 + i)];
+    apol += ae[2 * (md.a0 + i) + 1];
   }
-  a01 = block_sum<256>(a01, sh);
-  a02 = block_sum<256>(a02, sh);
-  a11 = block_sum<256>(a11, sh);
+  eaes = block_sum<256>(eaes, sh);
   apol = block_sum<256>(apol, sh);
   double edisp = 0.0;
   for (int k = tid; k < md.nd4; k += blockDim.x) edisp += zv[md.d0 + k] * (0.5 * d4dum[md.d0 + k]);
   edisp = block_sum<256>(edisp, sh);
   const double ees = 0.5 * ej + e3;
   double eel = ees + eh * kEVtoAU;
-  const double eaes = a01 + a02 + a11;
   eel = eel + eaes + apol;
   eel = eel + edisp;
   // solvation free energy (src/scc_core.f90:544-548, src/solv/gbsa.f90:525-548)

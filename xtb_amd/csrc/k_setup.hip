@@ -316,36 +316,6 @@ __global__ void k_coulomb(const MolDesc* __restrict__ desc, const Element* __res
   J[md.j0 + t] = v;
 }
 
-// D4 Gaussian CN weights of the reference systems (d4: src/disp/dftd4.F90:588-657)
-__global__ void k_d4_weights(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
-                             const int* __restrict__ at, const double* __restrict__ cn_d4, double* gw) {
-  const MolDesc md = desc[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= md.n) return;
-  const Element& e = el[at[md.a0 + i]];
-  int k0 = 0;
-  for (int a = 0; a < i; ++a) k0 += el[at[md.a0 + a]].nref;
-  const double c = cn_d4[md.a0 + i];
-  double norm = 0.0;
-  for (int r = 0; r < e.nref; ++r)
-    for (int k = 1; k <= e.ncount[r]; ++k) {
-      const double val = -(k * kD4wf) * (c - e.refcn[r]) * (c - e.refcn[r]);
-      norm += (val < -200.0) ? 0.0 : exp(val);
-    }
-  norm = (norm > 1e-80) ? 1.0 / norm : 0.0;
-  double maxcn = -1e300;
-  for (int r = 0; r < e.nref; ++r) maxcn = fmax(maxcn, e.refcn[r]);
-  for (int r = 0; r < e.nref; ++r) {
-    double g = 0.0;
-    for (int k = 1; k <= e.ncount[r]; ++k) {
-      const double val = -(k * kD4wf) * (c - e.refcn[r]) * (c - e.refcn[r]);
-      g += ((val < -200.0) ? 0.0 : exp(val)) * norm;
-    }
-    if (norm == 0.0) g = (fabs(maxcn - e.refcn[r]) < 1e-12) ? 1.0 : 0.0;
-    gw[md.d0 + k0 + r] = g;
-  }
-}
-
 // weighted dispersion matrix (build_wdispmat: src/disp/dftd4.F90:660-732)
 __global__ void k_dispmat(const MolDesc* __restrict__ desc, const Element* __restrict__ el, const int* __restrict__ at,
                           const double* __restrict__ xyz, const int* __restrict__ d4_at, const int* __restrict__ d4_ref,
