@@ -112,6 +112,23 @@ int xtbamd_server_set_solvent(xtbamd_server s, const char* solvent, int kernel, 
                               double temperature);
 void xtbamd_server_free(xtbamd_server s);
 
+/* ---- single point that allocates nothing ----
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Computes what xtbamd_batch_new (one molecule, EEQ
+ * guess) + xtbamd_batch_run + xtbamd_batch_get compute, with every device
+ * array carved from a 64 MiB scratch area that the library holds on each GPU
+ * it runs on until the process ends (molecules of up to about 450 AOs; a
+ * larger one returns -1 with a message naming the limit).  No hipMalloc,
+ * hipFree or allocating library call is made; the first call on a device
+ * sets up the scratch address and four events; the work runs on the null
+ * stream.  Calls on one device share its scratch area: calls from threads of
+ * one process are serialised, and the area keeps the last call's contents.  Inputs and
+ * outputs are host arrays as for the batch interface (out's array pointers
+ * may be NULL). */
+int xtbamd_singlepoint_noalloc(int natoms, const int* numbers, const double* positions, double charge, int uhf,
+                               double accuracy, double etemp, int maxiter, xtbamd_result* out);
+
 /* copies `bytes` bytes device to device with one plain copy kernel (k_copy):
  * the reference point for hardware byte counters */
 int xtbamd_copy_probe(long long bytes);

@@ -147,10 +147,24 @@ struct JacSet {
 
 struct SolvDev;   // implicit-solvent model on the device (k_solv.hip)
 
+// Device memory, streams, events and a host word buffer lent to a Batch: a
+// batch built on a loan calls no allocator (its device arrays are carved from
+// [base, base + bytes) in 256-byte aligned pieces) and releases nothing.
+struct Loan {
+  char* base = nullptr;
+  size_t bytes = 0;
+  hipStream_t streams[4] = {};   // main stream, then the second, small-problem and third-lane streams
+  hipEvent_t events[4] = {};     // fork, join, join3, join4 (timing disabled)
+  int* host_words = nullptr;     // 64 ints of host memory for the active-count reads
+};
+
 class Batch {
  public:
   // a batch of fixed molecules (packed layout)
   Batch(const std::vector<MoleculeInput>& mols, const Options& opt, hipStream_t stream = nullptr);
+  // the same on lent memory and streams (allocates nothing; throws when the
+  // molecules do not fit loan.bytes)
+  Batch(const std::vector<MoleculeInput>& mols, const Options& opt, const Loan& loan);
   // an empty batch of cap.slots slots for continuous batching (admit/release)
   Batch(const Capacity& cap, const Options& opt, hipStream_t stream = nullptr);
   ~Batch();
@@ -337,6 +351,10 @@ class Batch {
   int prof_result(int kind, int* launches, double* total_ms, double* work);
   void prof_enable(int mask);
   std::vector<void*> allocs_;
+  Loan loan_;                  // lent memory and streams (loan_.base != nullptr)
+  size_t loan_used_ = 0;       // bytes of the loan carved so far
+  bool on_loan() const { return loan_.base != nullptr; }
+  void build_fixed(const std::vector<MoleculeInput>& mols);
   std::vector<MolState> h_state0_;   // initial SCC state, restored by every setup()
   std::vector<MolState> state_stage_;
   std::vector<std::vector<double>> guess_qsh_, guess_dipm_, guess_qp_;

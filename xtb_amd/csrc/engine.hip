@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cctype>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -475,8 +476,18 @@ void host_occu(int ndim, int nel, int nopen, int& ihomoa, int& ihomob) {
 
 template <class T_>
 T_* Batch::alloc(size_t n) {
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T_);
+  if (on_loan()) {
+    // carve from the lent area, every piece 256-byte aligned
+    const size_t off = (loan_used_ + 255) / 256 * 256;
+    if (off + bytes > loan_.bytes)
+      throw std::runtime_error("molecule needs more device memory than the " +
+                               std::to_string(loan_.bytes >> 20) + " MiB scratch area holds");
+    loan_used_ = off + bytes;
+    return (T_*)(loan_.base + off);
+  }
   void* p = nullptr;
-  HIPCHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T_)));
+  HIPCHECK(hipMalloc(&p, bytes));
   allocs_.push_back(p);
   return (T_*)p;
 }
@@ -635,14 +646,26 @@ void Batch::init_common() {
   if (const char* r = std::getenv("XTBAMD_JAC_MAX_SWEEPS")) jac_max_sweeps = std::min(40, std::max(1, std::atoi(r)));
   trace_rounds = std::getenv("XTBAMD_TRACE_ROUNDS") != nullptr;
   if (const char* r = std::getenv("XTBAMD_JAC_LANES")) jac_lanes = std::min(kMaxLanes, std::max(1, std::atoi(r)));
-  // default priority (as every stream here), no implicit ordering with the null stream
-  HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
-  HIPCHECK(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
-  HIPCHECK(hipStreamCreateWithFlags(&stream4, hipStreamNonBlocking));
-  HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-  HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-  HIPCHECK(hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming));
-  HIPCHECK(hipEventCreateWithFlags(&ev_join4, hipEventDisableTiming));
+  if (on_loan()) {
+    // the lender's streams and events (created and kept by the lender)
+    stream = loan_.streams[0];
+    stream2 = loan_.streams[1];
+    stream3 = loan_.streams[2];
+    stream4 = loan_.streams[3];
+    ev_fork = loan_.events[0];
+    ev_join = loan_.events[1];
+    ev_join3 = loan_.events[2];
+    ev_join4 = loan_.events[3];
+  } else {
+    // default priority (as every stream here), no implicit ordering with the null stream
+    HIPCHECK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&stream4, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&ev_join4, hipEventDisableTiming));
+  }
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
   if (const char* r = std::getenv("XTBAMD_EIG")) use_ql = std::string(r) == "ql";
   // padded working copies on the register path; banded rounds only follow a
@@ -754,7 +777,8 @@ void Batch::allocate(const Capacity& c) {
   elements_dev = d_el;
   // every slot starts empty and inactive
   HIPCHECK(hipMemset(d_state, 0, nmol * sizeof(MolState)));
-  HIPCHECK(hipHostMalloc((void**)&h_pinned, 64 * sizeof(int), hipHostMallocDefault));
+  if (on_loan()) h_pinned = loan_.host_words;
+  else HIPCHECK(hipHostMalloc((void**)&h_pinned, 64 * sizeof(int), hipHostMallocDefault));
 }
 
 // reference C6 of all element pairs present (D4 reference tables)
@@ -889,7 +913,7 @@ void Batch::rebuild() {
       HIPCHECK(hipStreamSynchronize(stream));
       auto it = std::find(allocs_.begin(), allocs_.end(), (void*)ptr);
       if (it != allocs_.end()) allocs_.erase(it);
-      HIPCHECK(hipFree(ptr));
+      if (!on_loan()) HIPCHECK(hipFree(ptr));   // a loan's pieces are never released
     }
     capv = std::max<long long>(need, 2 * capv);
     ptr = alloc<T_>(capv);
@@ -1079,6 +1103,17 @@ void Batch::rebuild() {
 
 Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, hipStream_t s) : opt(o), stream(s) {
   init_common();
+  build_fixed(mols);
+}
+
+Batch::Batch(const std::vector<MoleculeInput>& mols, const Options& o, const Loan& loan)
+    : opt(o), stream(loan.streams[0]), loan_(loan) {
+  if (!loan.base || !loan.host_words) throw std::runtime_error("invalid loan");
+  init_common();
+  build_fixed(mols);
+}
+
+void Batch::build_fixed(const std::vector<MoleculeInput>& mols) {
   if (mols.empty()) throw std::runtime_error("empty batch");
   const int nm = (int)mols.size();
   std::vector<MolDesc> ds(nm);
@@ -1159,6 +1194,14 @@ void Batch::set_solvent(const std::string& name, int kernel, bool alpb, int stat
 Batch::~Batch() {
   for (auto& v : prof_ev_)
     for (hipEvent_t e : v) (void)hipEventDestroy(e);
+  if (on_loan()) {
+    // the lender owns the memory, streams and events; only wait for the work
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamSynchronize(stream2);
+    (void)hipStreamSynchronize(stream3);
+    (void)hipStreamSynchronize(stream4);
+    return;
+  }
   for (void* p : allocs_) (void)hipFree(p);
   if (h_pinned) (void)hipHostFree(h_pinned);
   if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -2079,6 +2122,21 @@ int fail(const std::exception& e) {
   return -1;
 }
 
+// Scratch area of xtbamd_singlepoint_noalloc: every device the library runs
+// on holds its own copy, for the life of the process.  The name is only a
+// host-side handle; its device address comes from hipGetSymbolAddress.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kNoAllocDevices = 64;
+struct NoAllocDevice {
+  std::mutex mu;          // calls on one device run one at a time
+  bool ready = false;
+  xtbamd::Loan loan;
+  int words[64] = {};     // host words of the active-count reads
+};
+NoAllocDevice g_noalloc[kNoAllocDevices];
+
 void copy_result(const xtbamd::MolResult& r, xtbamd_result* out) {
   out->natoms = r.n;
   out->nshells = r.nsh;
@@ -2336,6 +2394,50 @@ void xtbamd_server_free(xtbamd_server s) {
   if (!s) return;
   delete s->server;
   delete s;
+}
+
+int xtbamd_singlepoint_noalloc(int natoms, const int* numbers, const double* positions, double charge, int uhf,
+                               double accuracy, double etemp, int maxiter, xtbamd_result* out) {
+  try {
+    if (natoms <= 0 || !numbers || !positions || !out || maxiter <= 0)
+      throw std::runtime_error("invalid single point arguments");
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) throw std::runtime_error("device ordinal out of range");
+    NoAllocDevice& nd = g_noalloc[dev];
+    std::lock_guard<std::mutex> lock(nd.mu);
+    if (!nd.ready) {
+      // set-up, once per device with the device current: the device address
+      // of this device's copy of the scratch array and the fork/join events
+      // (kept until the process ends).  Every stream of the lent batch is
+      // the null stream: one molecule gains little from stream concurrency,
+      // and streams of their own would take hardware queues from the
+      // caller's other work.
+      void* ptr = nullptr;
+      HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      if (!ptr) throw std::runtime_error("scratch area has no device address");
+      nd.loan.base = (char*)ptr;
+      nd.loan.bytes = kScratchBytes;
+      for (int k = 0; k < 4; ++k) nd.loan.streams[k] = nullptr;
+      for (int k = 0; k < 4; ++k) HIPCHECK(hipEventCreateWithFlags(&nd.loan.events[k], hipEventDisableTiming));
+      nd.loan.host_words = nd.words;
+      nd.ready = true;
+    }
+    std::vector<xtbamd::MoleculeInput> mols(1);
+    mols[0].numbers.assign(numbers, numbers + natoms);
+    mols[0].xyz.assign(positions, positions + 3 * (size_t)natoms);
+    mols[0].charge = charge;
+    mols[0].uhf = uhf;
+    xtbamd::Options o;
+    o.accuracy = accuracy;
+    o.etemp = etemp;
+    o.maxiter = maxiter;
+    o.guess = 0;
+    xtbamd::Batch b(mols, o, nd.loan);
+    b.run();
+    copy_result(b.result(0, out->coeff || out->density), out);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
 }
 
 int xtbamd_batch_size(xtbamd_batch b) { return b ? b->batch->size() : 0; }

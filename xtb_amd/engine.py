@@ -79,6 +79,10 @@ def library():
     lib.xtbamd_batch_profile_result.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.xtbamd_batch_profile_result.restype = ctypes.c_int
+    lib.xtbamd_singlepoint_noalloc.argtypes = [ctypes.c_int, ip, _dp, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                               ctypes.POINTER(Result)]
+    lib.xtbamd_singlepoint_noalloc.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -326,3 +330,36 @@ def singlepoint(numbers, positions, charge=0.0, uhf=0, accuracy=1.0, etemp=300.0
     if r["failed"]:
         raise SolverFailure(r)
     return r
+
+
+def singlepoint_noalloc(numbers, positions, charge=0.0, uhf=0, accuracy=1.0, etemp=300.0, maxiter=250):
+    """One single point through xtbamd_singlepoint_noalloc (no device
+    allocation: the library's per-device scratch area); the same result as
+    singlepoint() with the EEQ guess.  Raises SolverFailure when a solver gave up."""
+    lib = library()
+    num = np.ascontiguousarray(numbers, dtype=np.int32)
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    n = len(num)
+    # shells and AOs per atom are at most 3 and 9 (s, p, d): arrays sized for
+    # that bound, trimmed to the reported sizes
+    out = {"gradient": np.zeros((n, 3)), "charges": np.zeros(n), "qsh": np.zeros(3 * n),
+           "dipm": np.zeros((n, 3)), "qp": np.zeros((n, 6)), "wbo": np.zeros((n, n)),
+           "emo": np.zeros(9 * n), "focc": np.zeros(9 * n)}
+    r = Result()
+    for k, v in out.items():
+        setattr(r, k, _ptr(v))
+    rc = lib.xtbamd_singlepoint_noalloc(n, num.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _ptr(pos),
+                                        float(charge), int(uhf), accuracy, etemp, maxiter, ctypes.byref(r))
+    if rc < 0:
+        raise RuntimeError(_err(lib))
+    for k in ("natoms", "nshells", "nao", "iterations", "converged", "failed", "ihomo", "energy", "gnorm",
+              "hl_gap_ev", "e_elec", "e_es", "e_aes", "e_axc", "e_disp", "e_rep", "e_atm", "e_solv"):
+        out[k] = getattr(r, k)
+    out["dipole"] = np.array(r.dipole[:])
+    out["qsh"] = out["qsh"][:r.nshells].copy()
+    out["emo"] = out["emo"][:r.nao].copy()
+    out["focc"] = out["focc"][:r.nao].copy()
+    if out["failed"]:
+        raise SolverFailure(out)
+    return out
+
