@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from xtb_amd import workloads  # noqa: E402
-from xtb_amd.profiling import traffic_table  # noqa: E402
+from xtb_amd.profiling import gpu_time_shares, traffic_table  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
@@ -40,27 +40,42 @@ def cpu_baseline(mols, budget_s, procs):
     import multiprocessing as mp
     ctx = mp.get_context("fork")   # runs before anything touches the GPU
     t0 = time.perf_counter()
-    energies, done = [], 0
+    energies, done, busy = [], 0, 0.0
     with ctx.Pool(procs) as pool:
         it = pool.imap(_cpu_worker, mols, chunksize=1)
-        for e, _ in it:
+        for e, dt in it:
             energies.append(e)
+            busy += dt
             done += 1
             if time.perf_counter() - t0 > budget_s and done % procs == 0:
                 pool.terminate()
                 break
     wall = time.perf_counter() - t0
-    return done, wall, energies
+    return done, wall, energies, busy
 
 
-def serve(args):
-    """Serving workload: Poisson arrivals of seeded water clusters / alkanes of
-    20-400 atoms (log-uniform), continuous batching under an HBM budget;
-    latency = finish - arrival."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def serve(args, reqs, qps):
+    """Serving workload: arrivals of seeded water clusters / alkanes of 20-400
+    atoms (log-uniform), continuous batching under an HBM budget; Poisson
+    arrivals at qps requests/s (qps = inf: every request at t = 0, the
+    saturated capacity); latency = finish - arrival."""
     import xtb_amd
     rng = np.random.default_rng(args.seed + 1)
-    reqs = workloads.mixed_batch(args.serve_requests, seed=args.seed + 2, lo=20, hi=400)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_qps, len(reqs)))
+    if np.isinf(qps):
+        arrivals = np.zeros(len(reqs))
+    else:
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, len(reqs)))
     srv = xtb_amd.Server(hbm_gb=args.serve_hbm_gb, max_atoms=400, max_slots=1024)
     # warm-up: a few requests through the server before the clock starts
     for z, x in reqs[:4]:
@@ -87,10 +102,9 @@ def serve(args):
     lat = np.array([fin[i] - arrivals[i] for i in range(len(reqs))])
     span = max(fin.values()) - arrivals[0]
     srv.close()
-    return {"requests": len(reqs), "offered_qps": args.serve_qps, "value": len(reqs) / span,
+    return {"requests": len(reqs), "offered_qps": None if np.isinf(qps) else qps, "value": len(reqs) / span,
             "unit": "single-points/s", "p50_ms": 1e3 * float(np.percentile(lat, 50)),
-            "p99_ms": 1e3 * float(np.percentile(lat, 99)), "hbm_gb": args.serve_hbm_gb,
-            "workload": "Poisson arrivals, seeded water clusters / alkanes, 20-400 atoms log-uniform"}
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)), "hbm_gb": args.serve_hbm_gb}
 
 
 def single_instances(xtb_amd):
@@ -121,12 +135,15 @@ def main():
     ap.add_argument("--nmol", type=int, default=512)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--plain", type=int, default=512, help="molecules timed on the plain one-at-a-time path")
+    ap.add_argument("--plain-alloc", type=int, default=64,
+                    help="molecules timed one at a time through a new (allocating) batch per call")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=600)
-    ap.add_argument("--serve-qps", type=float, default=40.0)
-    ap.add_argument("--serve-hbm-gb", type=float, default=16.0)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-loads", type=float, nargs="*", default=[0.5, 0.9],
+                    help="Poisson offered loads as fractions of the saturated serving capacity")
+    ap.add_argument("--serve-hbm-gb", type=float, default=64.0)
     ap.add_argument("--no-singles", action="store_true", help="skip the one-molecule timings (C0, C1, C4 sizes)")
     ap.add_argument("--no-solvent", action="store_true", help="skip the batched step in implicit water")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
@@ -142,37 +159,50 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        done, wall, ecpu = cpu_baseline(mols, args.cpu_budget, args.cpu_procs)
+        done, wall, ecpu, busy = cpu_baseline(mols, args.cpu_budget, args.cpu_procs)
         cpu = {"value": done / wall, "unit": "single-points/s", "cores": args.cpu_procs, "kind": "port",
-               "sample": f"first {done} molecules of the workload, one per process, FP64 C oracle with "
-                         f"MKL dsygvd/dgemm when loadable, {wall:.1f} s wall"}
+               "single_thread_value": done / busy, "cpu_model": cpu_model(),
+               "sample": f"first {done} molecules of the workload, one single-threaded process per core, FP64 C "
+                         f"oracle with MKL dsygvd/dgemm when loadable, {wall:.1f} s wall; single_thread_value = "
+                         f"molecules / summed per-molecule time"}
 
     import xtb_amd  # first GPU use
     batch_in = [(z, x, 0.0, 0) for z, x in mols]
     if args.copy_probe_mb > 0:
         xtb_amd.library().xtbamd_copy_probe(int(args.copy_probe_mb * 1048576))
 
-    # plain path: one molecule per batch, synchronous
+    # plain path: one molecule at a time, synchronous, through the
+    # single-molecule entry point that allocates nothing (the library's
+    # per-device scratch area), so no call pays for device allocation
     lat = []
     eplain = []
     for m in range(min(args.plain, len(mols))):
         t = time.perf_counter()
-        r = xtb_amd.singlepoint(mols[m][0], mols[m][1])
+        r = xtb_amd.singlepoint_noalloc(mols[m][0], mols[m][1])
         lat.append(time.perf_counter() - t)
         eplain.append(r["energy"])
     lat = np.array(lat)
+    # the same through a new batch per molecule (device allocation and
+    # release on every call), on the first molecules
+    lat_alloc = []
+    for m in range(min(args.plain_alloc, len(mols))):
+        t = time.perf_counter()
+        xtb_amd.singlepoint(mols[m][0], mols[m][1])
+        lat_alloc.append(time.perf_counter() - t)
 
     # batched path
     b = xtb_amd.Batch(batch_in)
     for _ in range(args.warmup):
         b.run()
     times = []
-    b.profile("gemm", "jacobi")   # HIP events around the GEMM launches and a sample of the pivot launches
+    # HIP events around the GEMM launches and a sample of the Jacobi rounds'
+    # pivot and tile-update launches, on the streams they run on
+    b.profile("gemm", "jacobi", "apply")
     for _ in range(args.steps):
         t = time.perf_counter()
         b.run()
         times.append(time.perf_counter() - t)
-    prof = {k: b.profile_result(k) for k in ("gemm", "jacobi")}
+    prof = {k: b.profile_result(k) for k in ("gemm", "jacobi", "apply")}
     res = b.results()
     # every timed step is a complete single point of every molecule: the
     # results of the last step equal the plain path's and iterate from scratch
@@ -201,42 +231,76 @@ def main():
                                f"10-200 atoms log-uniform, seed {args.seed}), energy+gradient, batched",
                    "atoms": int(natoms.sum()), "aos": int(naos.sum()), "max_ao": int(naos.max()),
                    "parallelism": "single GPU"},
-        "batched": {"p50_ms": ms, "p99_ms": 1e3 * float(np.max(times)), "converged": conv,
+        "batched": {"mean_ms": ms, "p50_ms": 1e3 * float(np.percentile(times, 50)),
+                    "p99_ms": 1e3 * float(np.percentile(times, 99)), "converged": conv,
                     "scc_iterations_mean": float(iters.mean())},
         "plain": {"molecules": len(lat), "value": len(lat) / float(lat.sum()) if len(lat) else None,
                   "p50_ms": 1e3 * float(np.percentile(lat, 50)) if len(lat) else None,
-                  "p99_ms": 1e3 * float(np.percentile(lat, 99)) if len(lat) else None},
+                  "p99_ms": 1e3 * float(np.percentile(lat, 99)) if len(lat) else None,
+                  "path": "one molecule per call, xtbamd_singlepoint_noalloc (no device allocation)"},
+        "plain_allocating": {"molecules": len(lat_alloc),
+                             "value": len(lat_alloc) / float(np.sum(lat_alloc)) if lat_alloc else None,
+                             "path": "one molecule per call, new batch (allocation) per call"},
         "parity": {"max_dE_batched_vs_plain": dE_plain, "max_dE_vs_oracle": dE_cpu},
         "roofline": None,
         "cpu_baseline": cpu,
     }
     ql = os.environ.get("XTBAMD_EIG") == "ql"
     lds = os.environ.get("XTBAMD_PIVOT_KERNEL") == "lds"
-    names = {"gemm": "k_gemm (batched FP64 GEMM on the matrix cores)",
+    names = {"gemm": "k_gemm_mfma (batched FP64 GEMM on the matrix cores)",
              "jacobi": ql and "k_tred_ql (tridiagonalisation + QL eigensolver, one workgroup per molecule)" or
              lds and "k_jac_pairsP (32x32 pivot problems of the block-Jacobi eigensolver, in LDS)" or
-             "k_jac_pairsR (register-resident Jacobi sweep of each 32x32 pivot problem, one wave)"}
-    short = {"gemm": "k_gemm_mfma", "jacobi": ql and "k_tred_ql" or lds and "k_jac_pairsP" or "k_jac_pairsR"}
+             "k_jac_pairsR (register-resident Jacobi sweep of each 32x32 pivot problem, one wave)",
+             "apply": "k_jac_apply_R (A and W tile updates of a block-Jacobi round, FP64 MFMA, one wave per "
+                      "32x32 tile)"}
+    short = {"gemm": "k_gemm_mfma", "jacobi": ql and "k_tred_ql" or lds and "k_jac_pairsP" or "k_jac_pairsR",
+             "apply": "k_jac_apply_R"}
     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs of this
-    # workload (copy-calibrated); only valid for the default workload and schedule
-    table = {}
-    if args.nmol == 512 and args.seed == 2024 and not any(
-            k.startswith("XTBAMD_") for k in os.environ):
+    # workload (copy-calibrated) and the GPU-time shares of its kernel-trace
+    # run; only valid for the default workload and schedule
+    table, shares = {}, {}
+    default_run = args.nmol == 512 and args.seed == 2024 and not any(k.startswith("XTBAMD_") for k in os.environ)
+    if default_run:
         table = traffic_table(os.path.join(ROOT, "profiles", "batched512_hbm_traffic.md"))
+        shares = gpu_time_shares(os.path.join(ROOT, "profiles", "batched512_kernel_stats.md"))
     lines = {}
     for k, (n, tms, work) in prof.items():
         if not n:
             continue
-        ach = work / (tms * 1e-3) / 1e12
-        lines[k] = {"kernel": names[k], "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": table.get(short[k]),
-                    "launches": n,
-                    "avg_us": 1e3 * tms / n, "flops_per_launch": work / n,
-                    "share_of_step": tms / (args.steps * ms)}
+        sec = tms * 1e-3
+        if k == "apply":
+            ach = work / sec / 1e9
+            line = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                    "bytes_per_launch": work / n,
+                    "work": "compulsory HBM bytes: each element of A and W a rotation touches read once and "
+                            "written once (A through one of its mirrored tiles), counted on the device"}
+        else:
+            ach = work / sec / 1e12
+            line = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / FP64_PEAK_TFLOPS, "flops_per_launch": work / n}
+            if k == "jacobi":
+                line["note"] = ("FP64 VALU work (the FP64 vector and matrix peaks are both 78.6 TFLOP/s); bound by "
+                                "the latency of its dependent inner rounds")
+        line.update({"kernel": names[k], "traffic": table.get(short[k]), "launches": n, "avg_us": 1e3 * tms / n,
+                     "share_of_step_events": tms / (args.steps * ms),
+                     "share_of_gpu_time_rocprof": shares.get(short[k])})
+        lines[k] = line
     if lines:
-        dom = max(lines, key=lambda k: lines[k]["share_of_step"])
+        # the dominant kernel: largest share of GPU time in the committed
+        # kernel-trace run of this workload, else of the event-timed time
+        if shares and all(lines[k]["share_of_gpu_time_rocprof"] is not None for k in lines):
+            dom = max(lines, key=lambda k: lines[k]["share_of_gpu_time_rocprof"])
+        else:
+            dom = max(lines, key=lambda k: lines[k]["share_of_step_events"])
         out["roofline"] = lines[dom]
         out["roofline_other"] = [v for k, v in lines.items() if k != dom]
+    # the whole step against the FP64 peak: F_SP = it (22/3) N^3 + (13/3) N^3
+    # flops per single point (it = its SCC iterations, N = its AOs)
+    nao_r = np.array([r["nao"] for r in res], dtype=np.float64)
+    fsp = float(np.sum(iters * (22.0 / 3.0) * nao_r ** 3 + (13.0 / 3.0) * nao_r ** 3))
+    out["step_roofline"] = {"flops_per_step": fsp, "achieved": fsp / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                            "unit": "TFLOP/s", "frac": fsp / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                            "model": "it * 22/3 N^3 + 13/3 N^3 FP64 flops per single point"}
     out["counters"] = {"scc_steps": steps_run, "jacobi_rounds": jac_rounds}
     if out["plain"]["value"]:
         out["speedup_batched_vs_plain"] = out["value"] / out["plain"]["value"]
@@ -254,7 +318,16 @@ def main():
                            "e_solv_mean": float(np.mean([r["e_solv"] for r in rs]))}
     b.close()
     if args.serve_requests > 0:
-        out["serving"] = serve(args)
+        # C3: saturated capacity of the serving loop on its 20-400-atom mix,
+        # then Poisson arrivals at fractions of it
+        reqs = workloads.mixed_batch(args.serve_requests, seed=args.seed + 2, lo=20, hi=400)
+        sat = serve(args, reqs, float("inf"))
+        out["serving"] = {"workload": "seeded water clusters / alkanes, 20-400 atoms log-uniform, energy+gradient, "
+                                      "continuous batching under an HBM budget",
+                          "saturated": sat,
+                          "c2_capacity_single_points_per_s": out["value"]}
+        for f in args.serve_loads:
+            out["serving"][f"poisson_{f:g}x"] = serve(args, reqs, f * sat["value"])
     if not args.no_singles:
         out["single_instances"] = single_instances(xtb_amd)
     print(json.dumps(out), flush=True)

@@ -49,6 +49,16 @@ const char* xtbamd_last_error(void);
  * atomic dipoles [natoms][3], quadrupoles [natoms][6] of a previous result;
  * mirrors the wavefunction reuse of xtb_singlepoint (src/api/interface.f90:83-101) */
 int xtbamd_batch_set_guess(xtbamd_batch b, int m, const double* qsh, const double* dipm, const double* qp);
+/* new positions [natoms][3] (bohr) of molecule m, same atoms: the device
+ * copy of xtb_updateMolecule (src/api/molecule.f90:114-159) for a batch that
+ * is kept across single points; the next setup/run recomputes everything
+ * that depends on them */
+int xtbamd_batch_set_positions(xtbamd_batch b, int m, const double* positions);
+/* on != 0: every run after the first starts each molecule from the shell
+ * charges, atomic dipoles and quadrupoles the previous run left on the device
+ * (no EEQ guess), the resident form of the wavefunction reuse of a results
+ * object (src/api/interface.f90:83-101) */
+int xtbamd_batch_keep_wavefunction(xtbamd_batch b, int on);
 /* implicit solvent for every molecule of the batch (call before setup/run;
  * NULL or "" returns to the gas phase).  kernel 1: Still kernel with the
  * GFN2/GBSA parameter sets; 2: P16 kernel with the GFN2/ALPB sets; alpb adds
@@ -78,11 +88,14 @@ int xtbamd_batch_charges(xtbamd_batch b, int m, double* q);
 void xtbamd_batch_free(xtbamd_batch b);
 /* HIP-event timing of kernel families on the streams they run on, from this
  * call on.  kernels: bit 0 = the batched FP64 GEMM (k_gemm), bit 1 = the
- * block-Jacobi pivot kernel (k_jac_pairs); 0 = off.  profile_result(kernel =
- * 1 GEMM, 2 pivots) gives the number of launches, their summed duration and
- * their summed algorithmic flops (GEMM: 2 N^3 per molecule processed; pivots:
- * 9216 per inner round of a 32x32 pivot problem).  The pivots are timed on a
- * sample of the rounds and scaled to all of them. */
+ * block-Jacobi pivot kernel (k_jac_pairsR), bit 2 = the block-Jacobi tile
+ * updates (k_jac_apply_R); 0 = off.  profile_result(kernel = 1 GEMM, 2
+ * pivots, 3 tile updates) gives the number of launches, their summed
+ * duration and their summed algorithmic work (GEMM: 2 N^3 flops per molecule
+ * processed; pivots: 9216 flops per inner round of a 32x32 pivot problem;
+ * tile updates: compulsory HBM bytes, each touched element of A and W read
+ * and written once).  The Jacobi families are timed on a sample of the
+ * rounds and scaled to all of them. */
 int xtbamd_batch_profile(xtbamd_batch b, int kernels);
 int xtbamd_batch_profile_result(xtbamd_batch b, int kernel, int* launches, double* total_ms, double* work);
 /* ---- continuous batching (serving) ----
@@ -128,6 +141,18 @@ void xtbamd_server_free(xtbamd_server s);
  * may be NULL). */
 int xtbamd_singlepoint_noalloc(int natoms, const int* numbers, const double* positions, double charge, int uhf,
                                double accuracy, double etemp, int maxiter, xtbamd_result* out);
+
+/* Symmetric eigen-decomposition of nprob matrices by the blocked FP64 solver
+ * that the engine uses for its largest eigenproblems (Householder
+ * tridiagonalisation, divide and conquer, back-transformation: the steps of
+ * dsyevd, called by the reference at src/mctc/lapack/eigensolve.F90:218-246
+ * through src/scc_core.f90:475).  a: the matrices, row-major, dimension n[k]
+ * (>= 2), concatenated; evals: eigenvalues ascending, concatenated; evecs:
+ * eigenvectors as rows (row j of problem k = eigenvector j), concatenated;
+ * tri (may be NULL): per problem the diagonal (n[k]) and the off-diagonal
+ * (n[k], last entry 0) of the tridiagonal form.  Allocates and frees its own
+ * device memory. */
+int xtbamd_eigh(int nprob, const int* n, const double* a, double* evals, double* evecs, double* tri);
 
 /* copies `bytes` bytes device to device with one plain copy kernel (k_copy):
  * the reference point for hardware byte counters */

@@ -146,6 +146,43 @@ struct JacSet {
 };
 
 struct SolvDev;   // implicit-solvent model on the device (k_solv.hip)
+struct EigProb;   // blocked eigensolver (k_eig.hip)
+struct DcNode;
+struct GemmJob;
+
+// The problems of a batch solved by the blocked eigensolver (dimension at
+// least Batch::eig_blk_min): work arrays, the divide and conquer trees and
+// the product job lists, built by Batch::rebuild().
+struct EigSet {
+  int np = 0, maxn = 0, npanel = 0, maxdepth = 0;
+  std::vector<int> mols;                 // batch slot of each problem
+  std::vector<int> zfin;                 // buffer (0/1) of each problem's final eigenvectors
+  std::vector<long long> v_off;          // each problem's vector block in vec
+  std::vector<int> ld;                   // each problem's row stride
+  EigProb* d_probs = nullptr;
+  int* act = nullptr;                    // per problem: solved in this call
+  int* zf = nullptr;                     // zfin on the device
+  double *Aw = nullptr, *Vh = nullptr, *Z0 = nullptr, *Z1 = nullptr, *U = nullptr;
+  double *X = nullptr, *P = nullptr, *P2 = nullptr, *Tf = nullptr, *vec = nullptr;
+  int* ivec = nullptr;
+  long long cap[18] = {};     // capacities (elements) of the arrays above and below
+  long long nmat = 0, nx = 0;   // elements in use of the n x ld arrays and of X
+  // product launches: jobs [job0, job0 + njob), tiles [tile0, tile0 + ntile)
+  struct Launch {
+    int job0 = 0, njob = 0;
+    long long tile0 = 0, ntile = 0;
+  };
+  std::vector<Launch> upd, bt1, bt2, bt3, dcg;   // per panel (upd, bt*), per tree level (dcg)
+  GemmJob* d_jobs = nullptr;
+  int2* d_tiles = nullptr;
+  // divide and conquer: leaves (prob, start, size, -), all merges (for the
+  // splitting), merges by level (height 1..maxdepth)
+  int4* d_leaves = nullptr;
+  int nleaves = 0;
+  DcNode* d_nodes = nullptr;
+  int nnodes = 0;
+  std::vector<int> lvl0, lvln, lvlmax;   // per level: first node, node count, largest node
+};
 
 // Device memory, streams, events and a host word buffer lent to a Batch: a
 // batch built on a loan calls no allocator (its device arrays are carved from
@@ -157,6 +194,23 @@ struct Loan {
   hipEvent_t events[4] = {};     // fork, join, join3, join4 (timing disabled)
   int* host_words = nullptr;     // 64 ints of host memory for the active-count reads
 };
+
+// device memory for an EigSet's arrays
+struct EigMem {
+  virtual void* get(size_t bytes) = 0;
+  virtual void put(void* p) = 0;
+  virtual ~EigMem() = default;
+};
+// work arrays, trees and product job lists of es's problems (es.mols set,
+// dims their dimensions)
+void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem);
+// the blocked eigensolver on es's problems: problem k's matrix is read from
+// Am at jprobs[es.mols[k]] (a_off, ld), its eigenvalues are written to Am's
+// diagonal and its eigenvectors as rows of Wm (w_off, ld); jstate gets the
+// done / failed flags (kJacState ints per problem); mstate (optional) skips
+// molecules that no longer iterate
+void eig_set_run(EigSet& es, const JacobiProb* jprobs, int* jstate, const MolState* mstate, double* Am, double* Wm,
+                 hipStream_t s);
 
 class Batch {
  public:
@@ -182,6 +236,13 @@ class Batch {
   // restart guess for molecule m: shell charges, atomic dipoles [n][3] and
   // quadrupoles [n][6] of a previous wavefunction (used by the next setup)
   void set_guess(int m, const double* qsh_in, const double* dipm_in, const double* qp_in);
+  // new positions [n][3] (bohr) of molecule m, same atoms; the next run
+  // recomputes everything that depends on them
+  void set_positions(int m, const double* xyz);
+  // on: every run after the first starts each molecule from the wavefunction
+  // (shell charges, atomic dipoles and quadrupoles) the previous run left on
+  // the device instead of an initial guess
+  void keep_wavefunction(bool on) { keep_wfn_ = on; }
   // implicit solvent for every molecule of the batch (empty name: gas phase);
   // kernel 1 Still (GBSA parameter sets) or 2 P16 (ALPB parameter sets), alpb
   // adds the ALPB shape term, state 1 gsolv / 2 reference / 3 1 bar.  Throws
@@ -302,6 +363,16 @@ class Batch {
   const Element* elements_dev = nullptr;
   int ntiles = 0, ntiles_sym = 0;   // d_tiles: all tiles, then the b >= a tiles of symmetric products
   int4* d_tiles = nullptr;
+  // problems of at least this dimension go to the blocked eigensolver
+  // (tridiagonalisation, divide and conquer, back-transformation) instead of
+  // the block-Jacobi rounds; register path only.  Off unless the tuning file
+  // or XTBAMD_EIG_BLK_MIN sets it.
+  int eig_blk_min = 1 << 30;
+  EigSet eig_;
+  void eig_build();
+  // the blocked eigensolver on eig_'s problems: A from Am (Jacobi problem
+  // layout), eigenvalues onto Am's diagonal, eigenvector rows into Wm
+  void eig_blocked(const JacSet& js, double* Am, double* Wm, bool check_active, hipStream_t s);
   // returns true when W comes out as rows (row j = eigenvector j)
   void jacobi_lanes(JacSet& js, double* Am, double* Wm);
   bool jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check_active, bool keep_w = false,
@@ -336,16 +407,21 @@ class Batch {
   void eeq_solve();        // EEQ charges by Gaussian elimination
 
   // HIP-event timing of kernel families over a run (roofline measurement):
-  // bit 1 = k_gemm, bit 2 = k_jac_pairs.  Algorithmic work is counted on the
-  // device: 2 N^3 flops per molecule a GEMM launch processes; 36864 flops per
-  // inner Jacobi round of a 64x64 pivot problem (4096 elements rotated on
-  // both sides and in Q, 3 flops each).
+  // bit 1 = k_gemm, bit 2 = the block-Jacobi pivots, bit 3 = the block-Jacobi
+  // tile updates (k_jac_apply_R).  Algorithmic work is counted on the device:
+  // 2 N^3 flops per molecule a GEMM launch processes; 9216 flops per inner
+  // round of a 32x32 pivot problem (2048 elements rotated on both sides of A
+  // and 1024 in Q, 3 flops each); the tile updates' compulsory HBM bytes
+  // (each touched element of A and W read once and written once, A through
+  // one of its mirrored tiles).  The Jacobi families are timed on a sample of
+  // the rounds.
+  static constexpr int kProfKinds = 4;   // 1 GEMM, 2 Jacobi pivots, 3 Jacobi tile updates
   int prof_mask = 0;
-  std::vector<hipEvent_t> prof_ev_[3];
-  int prof_used_[3] = {0, 0, 0};
-  int prof_all_[3] = {0, 0, 0};   // every launch of a family timed on a sample
+  std::vector<hipEvent_t> prof_ev_[kProfKinds];
+  int prof_used_[kProfKinds] = {};
+  int prof_all_[kProfKinds] = {};   // every launch of a family timed on a sample
   long long prof_round_ = 0;       // block-Jacobi rounds since prof_enable (the sample clock)
-  unsigned long long* prof_work_ = nullptr;   // [3]
+  unsigned long long* prof_work_ = nullptr;   // [kProfKinds]
   void prof_begin(int kind, hipStream_t s = nullptr);
   void prof_end(int kind, const int* mask, int mstride, hipStream_t s = nullptr);
   int prof_result(int kind, int* launches, double* total_ms, double* work);
@@ -359,6 +435,8 @@ class Batch {
   std::vector<MolState> state_stage_;
   std::vector<std::vector<double>> guess_qsh_, guess_dipm_, guess_qp_;
   bool setup_done_ = false;
+  bool keep_wfn_ = false;    // start from the resident wavefunction when there is one
+  bool wfn_valid_ = false;   // a completed run left its wavefunction on the device
   int iterations_ = 0;
   template <class T_>
   T_* alloc(size_t n);

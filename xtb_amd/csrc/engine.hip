@@ -1,5 +1,6 @@
 // Host side of the batched GFN2-xTB engine: layout, work lists, the SCC
 // driver and the C ABI of the batch interface (include/xtb_amd_batch.h).
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +22,7 @@
 #include "k_d4.hip"
 #include "k_grad.hip"
 #include "k_linalg.hip"
+#include "k_eig.hip"
 #include "k_scc.hip"
 #include "k_setup.hip"
 #include "k_solv.hip"
@@ -256,6 +258,19 @@ __global__ void k_zero_atoms(const MolDesc* __restrict__ desc, double* v3, doubl
   for (int k = 0; k < 3; ++k) v3[3 * (md.a0 + i) + k] = 0.0;
   if (v6)
     for (int k = 0; k < 6; ++k) v6[6 * (md.a0 + i) + k] = 0.0;
+}
+
+// atomic charges from shell charges, each atom's shells summed in ascending
+// order from zero (the order of the host-side restart guess)
+__global__ void k_q_from_qsh(const MolDesc* __restrict__ desc, const int* __restrict__ sh_at,
+                             const double* __restrict__ qsh, double* q) {
+  const MolDesc md = desc[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= md.n) return;
+  double acc = 0.0;
+  for (int s = 0; s < md.nsh; ++s)
+    if (sh_at[md.s0 + s] == i) acc += qsh[md.s0 + s];
+  q[md.a0 + i] = acc;
 }
 
 // Newton-Schulz bookkeeping for the selected molecules: stage 0 (running),
@@ -629,7 +644,55 @@ static long long* cap_field(Capacity& c, int k) {
   return f[k];
 }
 
+// The project autotuner's choices (python -m xtb_amd.tune writes them to
+// tuning_gfx950.txt next to this library; XTBAMD_TUNING names another file,
+// "off" none): "key value" lines, '#' comments.  Read once per process.
+namespace {
+struct Tuning {
+  int eig_blk_min = -1, jac_lanes = -1, jac_cross = -1, gemm_valu = -1;
+};
+const Tuning& tuning() {
+  static const Tuning t = [] {
+    Tuning r;
+    std::string path;
+    if (const char* e = std::getenv("XTBAMD_TUNING")) {
+      if (std::string(e) == "off") return r;
+      path = e;
+    } else {
+      Dl_info info;
+      if (dladdr((void*)&tuning, &info) && info.dli_fname) {
+        std::string f(info.dli_fname);
+        const size_t k = f.rfind('/');
+        path = (k == std::string::npos ? std::string(".") : f.substr(0, k)) + "/tuning_gfx950.txt";
+      }
+    }
+    FILE* fp = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
+    if (!fp) return r;
+    char key[64];
+    char val[64];
+    char line[256];
+    while (std::fgets(line, sizeof(line), fp)) {
+      if (line[0] == '#' || std::sscanf(line, "%63s %63s", key, val) != 2) continue;
+      const std::string k(key), v(val);
+      if (k == "eig_blk_min") r.eig_blk_min = std::atoi(val);
+      else if (k == "jac_lanes") r.jac_lanes = std::atoi(val);
+      else if (k == "jac_cross") r.jac_cross = std::atoi(val);
+      else if (k == "gemm") r.gemm_valu = v == "valu" ? 1 : 0;
+    }
+    std::fclose(fp);
+    return r;
+  }();
+  return t;
+}
+}  // namespace
+
 void Batch::init_common() {
+  // the autotuner's choices first; environment variables override them
+  const Tuning& tu = tuning();
+  if (tu.eig_blk_min > 0) eig_blk_min = std::max(65, tu.eig_blk_min);
+  if (tu.jac_lanes > 0) jac_lanes = std::min(kMaxLanes, tu.jac_lanes);
+  if (tu.jac_cross >= 0) jac_cross = tu.jac_cross != 0;
+  if (tu.gemm_valu >= 0) use_mfma = tu.gemm_valu == 0;
   if (const char* g = std::getenv("XTBAMD_GEMM")) use_mfma = std::string(g) != "valu";
   if (const char* r = std::getenv("XTBAMD_REFINE_STEPS")) oa_max_steps = std::atoi(r);
   if (const char* r = std::getenv("XTBAMD_INNER_SWEEPS")) inner_sweeps = std::max(1, std::atoi(r));
@@ -646,6 +709,7 @@ void Batch::init_common() {
   if (const char* r = std::getenv("XTBAMD_JAC_MAX_SWEEPS")) jac_max_sweeps = std::min(40, std::max(1, std::atoi(r)));
   trace_rounds = std::getenv("XTBAMD_TRACE_ROUNDS") != nullptr;
   if (const char* r = std::getenv("XTBAMD_JAC_LANES")) jac_lanes = std::min(kMaxLanes, std::max(1, std::atoi(r)));
+  if (const char* r = std::getenv("XTBAMD_EIG_BLK_MIN")) eig_blk_min = std::max(65, std::atoi(r));
   if (on_loan()) {
     // the lender's streams and events (created and kept by the lender)
     stream = loan_.streams[0];
@@ -938,9 +1002,11 @@ void Batch::rebuild() {
   // Jacobi problems: dim nao, matrices at m0, eigenvalues at o0
   JacSet& js = jac_n;
   long long qoff = 0;
-  std::vector<int> small, big;
+  std::vector<int> small, big, blk;
   std::vector<int2> pairs;
   std::vector<int4> ta, tw, to;
+  // the blocked eigensolver takes the largest problems on the register path
+  const bool blk_path = jac_padded && jac_p == 32 && !use_ql;
   js.max_rounds = 0;
   js.big_max_dim = 0;
   js.big_max_ld = 0;
@@ -964,6 +1030,12 @@ void Batch::rebuild() {
     js.big_max_ld = std::max(js.big_max_ld, p.ld);
     if (p.dim <= 64) {
       small.push_back(m);
+      continue;
+    }
+    if (blk_path && p.dim >= eig_blk_min) {
+      // the blocked eigensolver (no lane: group -1 is counted by none)
+      p.group = -1;
+      blk.push_back(m);
       continue;
     }
     big.push_back(m);
@@ -1088,6 +1160,8 @@ void Batch::rebuild() {
   grow(js.d_ta, ta_cap_, std::max<long long>(1, js.nta));
   grow(js.d_tw, tw_cap_, std::max<long long>(1, js.ntw));
   grow(js.d_to, to_cap_, std::max<long long>(1, js.nto));
+  eig_.mols = blk;
+  eig_build();
   js.nbig = (int)big.size();
   grow(js.d_big, big_cap_, std::max<long long>(1, js.nbig));
   if (!big.empty()) HIPCHECK(hipMemcpyAsync(js.d_big, big.data(), big.size() * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -1238,19 +1312,24 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   // stream4 beside those of lane 0; the problems of a lane never touch another
   // lane's data, so the results do not depend on the lanes
   const bool lanes = rows && !pivot_lds && js.npairs > 0;
+  // the largest problems go to the blocked eigensolver on stream3 (after the
+  // small ones), beside the lanes
+  const bool blk = rows && !pivot_lds && eig_.np > 0;
+  const bool forked = lanes || blk;
   // the problems of dimension <= 64 (one workgroup each, k_jac_small) run
   // on stream3 beside the lanes
-  if (lanes) {
+  if (forked) {
     HIPCHECK(hipEventRecord(ev_fork, stream));
     HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
     HIPCHECK(hipStreamWaitEvent(stream3, ev_fork, 0));
     HIPCHECK(hipStreamWaitEvent(stream4, ev_fork, 0));
   }
   if (js.nsmall)
-    hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, lanes ? stream3 : stream, js.d_probs, js.d_small, Am,
-                       Wm, (const double*)js.scale, js.state, keep_w ? 1 : 0, rows ? 1 : 0, jac_tol);
-  if (lanes) {
-    jacobi_lanes(js, Am, Wm);
+    hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, forked ? stream3 : stream, js.d_probs, js.d_small,
+                       Am, Wm, (const double*)js.scale, js.state, keep_w ? 1 : 0, rows ? 1 : 0, jac_tol);
+  if (blk) eig_blocked(js, Am, Wm, check_active, stream3);
+  if (forked) {
+    if (lanes) jacobi_lanes(js, Am, Wm);
     HIPCHECK(hipEventRecord(ev_join, stream2));
     HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
     HIPCHECK(hipEventRecord(ev_join3, stream3));
@@ -1332,6 +1411,302 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   return rows;
 }
 
+// ---------------------------------------------------------------------------
+// blocked eigensolver: work arrays, trees and product job lists of eig_'s
+// problems (built once per rebuild; the device addresses are fixed)
+void Batch::eig_build() {
+  // device memory of the batch (a loan's pieces are never released)
+  struct BatchMem : EigMem {
+    Batch* b;
+    explicit BatchMem(Batch* bb) : b(bb) {}
+    void* get(size_t bytes) override { return b->alloc<char>(bytes); }
+    void put(void* p) override {
+      auto it = std::find(b->allocs_.begin(), b->allocs_.end(), p);
+      if (it != b->allocs_.end()) b->allocs_.erase(it);
+      if (!b->on_loan()) HIPCHECK(hipFree(p));
+    }
+  } mem(this);
+  std::vector<int> dims;
+  for (int m : eig_.mols) dims.push_back(desc[m].nao);
+  eig_set_build(eig_, dims, mem);
+}
+
+void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
+  es.np = (int)es.mols.size();
+  if (es.np == 0) return;
+  auto grow = [&](auto*& ptr, long long& capv, long long need) {
+    using T_ = typename std::remove_reference<decltype(*ptr)>::type;
+    if (need <= capv && ptr) return;
+    if (ptr) {
+      HIPCHECK(hipDeviceSynchronize());
+      mem.put((void*)ptr);
+    }
+    capv = std::max<long long>(need, capv);
+    ptr = (T_*)mem.get((size_t)std::max(1LL, capv) * sizeof(T_));
+  };
+  std::vector<EigProb> probs(es.np);
+  long long mat = 0, vv = 0, iv = 0, xx = 0, pp = 0;
+  es.v_off.clear();
+  es.ld.clear();
+  es.maxn = 0;
+  es.maxdepth = 0;
+  for (int k = 0; k < es.np; ++k) {
+    EigProb& e = probs[k];
+    std::memset(&e, 0, sizeof(e));
+    e.n = dims[k];
+    e.ld = (e.n + 15) / 16 * 16;
+    e.mol = es.mols[k];
+    e.m_off = mat;
+    e.v_off = vv;
+    es.v_off.push_back(vv);
+    es.ld.push_back(e.ld);
+    e.i_off = iv;
+    e.x_off = xx;
+    e.p_off = pp;
+    mat += (long long)e.n * e.ld;
+    vv += (long long)kEigVecs * e.ld;
+    iv += (long long)kEigInts * e.ld;
+    xx += 4LL * kEigNb * e.ld;
+    pp += (long long)e.n * kEigNb;
+    int depth = 0;
+    while (((e.n + (1 << depth) - 1) >> depth) > kDcLeaf) ++depth;
+    e.depth = depth;
+    es.maxn = std::max(es.maxn, e.n);
+    es.maxdepth = std::max(es.maxdepth, depth);
+  }
+  es.npanel = (es.maxn - 1 + kEigNb - 1) / kEigNb;
+  grow(es.Aw, es.cap[0], mat);
+  grow(es.Vh, es.cap[1], mat);
+  grow(es.Z0, es.cap[2], mat);
+  grow(es.Z1, es.cap[3], mat);
+  grow(es.U, es.cap[4], mat);
+  grow(es.vec, es.cap[5], vv);
+  grow(es.ivec, es.cap[6], iv);
+  grow(es.X, es.cap[7], xx);
+  grow(es.P, es.cap[8], pp);
+  grow(es.P2, es.cap[9], pp);
+  grow(es.Tf, es.cap[10], (long long)es.np * std::max(1, es.npanel) * kEigNb * kEigNb);
+  grow(es.d_probs, es.cap[11], es.np);
+  grow(es.act, es.cap[12], es.np);
+  grow(es.zf, es.cap[13], es.np);
+  es.nmat = mat;
+  es.nx = xx;
+  // divide and conquer trees: every leaf at the problem's depth
+  std::vector<int4> leaves;
+  std::vector<std::vector<DcNode>> lv(es.maxdepth + 1);
+  es.zfin.assign(es.np, 0);
+  for (int k = 0; k < es.np; ++k) {
+    const int D = probs[k].depth;
+    es.zfin[k] = D & 1;
+    struct Item { int start, size, depth; };
+    std::vector<Item> st{{0, probs[k].n, 0}};
+    while (!st.empty()) {
+      const Item it = st.back();
+      st.pop_back();
+      if (it.depth == D) {
+        leaves.push_back(make_int4(k, it.start, it.size, 0));
+        continue;
+      }
+      const int n1 = it.size / 2;
+      DcNode nd;
+      nd.prob = k;
+      nd.start = it.start;
+      nd.n1 = n1;
+      nd.n2 = it.size - n1;
+      lv[D - it.depth].push_back(nd);
+      st.push_back({it.start, n1, it.depth + 1});
+      st.push_back({it.start + n1, it.size - n1, it.depth + 1});
+    }
+  }
+  std::vector<DcNode> nodes;
+  es.lvl0.assign(es.maxdepth + 1, 0);
+  es.lvln.assign(es.maxdepth + 1, 0);
+  es.lvlmax.assign(es.maxdepth + 1, 0);
+  for (int h = 1; h <= es.maxdepth; ++h) {
+    es.lvl0[h] = (int)nodes.size();
+    es.lvln[h] = (int)lv[h].size();
+    for (const DcNode& nd : lv[h]) {
+      nodes.push_back(nd);
+      es.lvlmax[h] = std::max(es.lvlmax[h], nd.n1 + nd.n2);
+    }
+  }
+  es.nleaves = (int)leaves.size();
+  es.nnodes = (int)nodes.size();
+  grow(es.d_leaves, es.cap[14], std::max(1, es.nleaves));
+  grow(es.d_nodes, es.cap[15], std::max(1, es.nnodes));
+  // product jobs
+  std::vector<GemmJob> jobs;
+  std::vector<int2> tiles;
+  auto add = [&](std::vector<EigSet::Launch>& lst, const std::vector<GemmJob>& js_) {
+    EigSet::Launch L;
+    L.job0 = (int)jobs.size();
+    L.tile0 = (long long)tiles.size();
+    for (GemmJob g : js_) {
+      g.tm = (g.M + 63) / 64;
+      g.tn = (g.N + 63) / 64;
+      g.tile0 = (long long)tiles.size() - L.tile0;
+      const int jid = (int)jobs.size() - L.job0;
+      for (int t = 0; t < g.tm * g.tn; ++t) tiles.push_back(make_int2(jid, t));
+      jobs.push_back(g);
+    }
+    L.njob = (int)jobs.size() - L.job0;
+    L.ntile = (long long)tiles.size() - L.tile0;
+    lst.push_back(L);
+  };
+  auto job = [](int M, int N, int K, const double* A, long long lda, const double* B, long long ldb, double* C,
+                long long ldc, double alpha, double beta, const int* act) {
+    GemmJob g;
+    std::memset(&g, 0, sizeof(g));
+    g.M = M; g.N = N; g.K = K;
+    g.A = A; g.B = B; g.C = C;
+    g.lda = lda; g.ldb = ldb; g.ldc = ldc;
+    g.alpha = alpha; g.beta = beta;
+    g.act = act;
+    return g;
+  };
+  es.upd.clear(); es.bt1.clear(); es.bt2.clear(); es.bt3.clear(); es.dcg.clear();
+  for (int pi = 0; pi < es.npanel; ++pi) {
+    const int j0 = pi * kEigNb;
+    std::vector<GemmJob> u, b1, b2, b3;
+    for (int k = 0; k < es.np; ++k) {
+      const EigProb& e = probs[k];
+      const int n = e.n, ld = e.ld;
+      if (j0 >= n - 1) continue;
+      const int jb = std::min(kEigNb, n - 1 - j0);
+      const int* act = es.act + k;
+      double* Xp = es.X + e.x_off;
+      double* Yp = Xp + 2LL * kEigNb * ld;
+      const int s0 = j0 + kEigNb;
+      // trailing update A22 -= X^T Y on [s0, n) x [s0, n)
+      if (s0 <= n - 1)
+        u.push_back(job(n - s0, n - s0, 2 * kEigNb, Xp + s0, ld, Yp + s0, ld, es.Aw + e.m_off + (long long)s0 * ld + s0,
+                        ld, -1.0, 1.0, act));
+      // back-transformation of the eigenvector rows Z with this panel
+      double* Zf = (es.zfin[k] ? es.Z1 : es.Z0) + e.m_off;
+      const int k0 = j0 + 1;
+      const double* Vp = es.Vh + e.m_off + (long long)j0 * ld + k0;
+      double* P = es.P + e.p_off;
+      double* P2 = es.P2 + e.p_off;
+      const double* T = es.Tf + ((long long)k * es.npanel + pi) * kEigNb * kEigNb;
+      b1.push_back(job(n, jb, n - k0, Zf + k0, ld, Vp, ld, P, kEigNb, 1.0, 0.0, act));        // P = Z V
+      b2.push_back(job(n, jb, jb, P, kEigNb, T, kEigNb, P2, kEigNb, 1.0, 0.0, act));          // P2 = P T^T
+      b3.push_back(job(n, n - k0, jb, P2, kEigNb, Vp, ld, Zf + k0, ld, -1.0, 1.0, act));      // Z -= P2 V^T
+    }
+    add(es.upd, u);
+    add(es.bt1, b1);
+    add(es.bt2, b2);
+    add(es.bt3, b3);
+  }
+  for (int h = 1; h <= es.maxdepth; ++h) {
+    std::vector<GemmJob> g;
+    for (int x = 0; x < es.lvln[h]; ++x) {
+      const DcNode& nd = nodes[es.lvl0[h] + x];
+      const EigProb& e = probs[nd.prob];
+      const int m = nd.n1 + nd.n2, ld = e.ld;
+      const double* Zin = ((h - 1) & 1 ? es.Z1 : es.Z0) + e.m_off;
+      double* Zout = (h & 1 ? es.Z1 : es.Z0) + e.m_off;
+      int* I = es.ivec + e.i_off;
+      GemmJob j = job(m, m, m, es.U + e.m_off + (long long)nd.start * ld, ld, Zin + nd.start, ld, Zout + nd.start, ld,
+                      1.0, 0.0, es.act + nd.prob);
+      j.bk = I + (long long)kIrow * ld + nd.start;
+      j.crow = I + (long long)kIrot * ld + nd.start;
+      j.kdev = I + (long long)kIcnt * ld + nd.start;
+      g.push_back(j);
+    }
+    add(es.dcg, g);
+  }
+  grow(es.d_jobs, es.cap[16], std::max<long long>(1, (long long)jobs.size()));
+  grow(es.d_tiles, es.cap[17], std::max<long long>(1, (long long)tiles.size()));
+  HIPCHECK(hipMemcpy(es.d_probs, probs.data(), es.np * sizeof(EigProb), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(es.zf, es.zfin.data(), es.np * sizeof(int), hipMemcpyHostToDevice));
+  if (!leaves.empty())
+    HIPCHECK(hipMemcpy(es.d_leaves, leaves.data(), leaves.size() * sizeof(int4), hipMemcpyHostToDevice));
+  if (!nodes.empty())
+    HIPCHECK(hipMemcpy(es.d_nodes, nodes.data(), nodes.size() * sizeof(DcNode), hipMemcpyHostToDevice));
+  if (!jobs.empty()) HIPCHECK(hipMemcpy(es.d_jobs, jobs.data(), jobs.size() * sizeof(GemmJob), hipMemcpyHostToDevice));
+  if (!tiles.empty()) HIPCHECK(hipMemcpy(es.d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice));
+}
+
+void Batch::eig_blocked(const JacSet& js, double* Am, double* Wm, bool check_active, hipStream_t s) {
+  eig_set_run(eig_, js.d_probs, js.state, check_active ? (const MolState*)d_state : (const MolState*)nullptr, Am, Wm,
+              s);
+}
+
+void eig_set_run(EigSet& es, const JacobiProb* jprobs, int* jstate, const MolState* mstate, double* Am, double* Wm,
+                 hipStream_t s) {
+  const int np = es.np;
+  auto gemm_launch = [&](const EigSet::Launch& L, bool ta, bool tb) {
+    if (L.ntile == 0) return;
+    const dim3 g((unsigned)L.ntile), b(256);
+    const GemmJob* jb = es.d_jobs + L.job0;
+    const int2* tl = es.d_tiles + L.tile0;
+    if (ta) hipLaunchKernelGGL((k_bgemm<true, false>), g, b, 0, s, jb, L.njob, tl);
+    else if (tb) hipLaunchKernelGGL((k_bgemm<false, true>), g, b, 0, s, jb, L.njob, tl);
+    else hipLaunchKernelGGL((k_bgemm<false, false>), g, b, 0, s, jb, L.njob, tl);
+  };
+  const long long nn = (long long)es.maxn * es.maxn;
+  hipLaunchKernelGGL(k_eig_in, dim3(gstride(nn), np), dim3(256), 0, s, (const EigProb*)es.d_probs, jprobs, mstate,
+                     es.act, (const double*)Am, es.Aw);
+  // 1. tridiagonalisation
+  const int nmax = es.maxn;
+  for (int pi = 0; pi < es.npanel; ++pi) {
+    const int j0 = pi * kEigNb;
+    const int j1 = std::min(j0 + kEigNb, nmax - 1);   // this panel's columns [j0, j1)
+    hipLaunchKernelGGL(k_fill, dim3(grid1(es.nx, 256)), dim3(256), 0, s, es.X, es.nx, 0.0);
+    hipLaunchKernelGGL(k_trd_fin, dim3(np), dim3(1024), 0, s, (const EigProb*)es.d_probs, (const int*)es.act,
+                       (const double*)es.Aw, es.Vh, es.X, es.vec, -1, j0, j0);
+    for (int j = j0; j < j1; ++j) {
+      hipLaunchKernelGGL(k_trd_symv, dim3(grid1(nmax - 1 - j, 16), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                         (const int*)es.act, (const double*)es.Aw, (const double*)es.Vh, es.vec, j);
+      const int next = j + 1 < j0 + kEigNb ? j + 1 : -1;
+      hipLaunchKernelGGL(k_trd_fin, dim3(np), dim3(1024), 0, s, (const EigProb*)es.d_probs, (const int*)es.act,
+                         (const double*)es.Aw, es.Vh, es.X, es.vec, j, next, j0);
+    }
+    gemm_launch(es.upd[pi], true, false);
+  }
+  hipLaunchKernelGGL(k_trd_save, dim3(grid1(nmax, 256), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                     (const int*)es.act, es.vec);
+  // 2. divide and conquer
+  hipLaunchKernelGGL(k_fill, dim3(grid1(es.nmat, 256)), dim3(256), 0, s, es.Z0, es.nmat, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(grid1(es.nmat, 256)), dim3(256), 0, s, es.Z1, es.nmat, 0.0);
+  if (es.nnodes)
+    hipLaunchKernelGGL(k_dc_split, dim3(grid1(es.nnodes, 64)), dim3(64), 0, s, (const EigProb*)es.d_probs,
+                       (const int*)es.act, (const DcNode*)es.d_nodes, es.nnodes, es.vec);
+  hipLaunchKernelGGL(k_dc_leaf, dim3(es.nleaves), dim3(256), 0, s, (const EigProb*)es.d_probs, (const int*)es.act,
+                     (const int4*)es.d_leaves, es.vec, es.Z0);
+  for (int h = 1; h <= es.maxdepth; ++h) {
+    const int nl = es.lvln[h], mm = es.lvlmax[h];
+    if (nl == 0) continue;
+    const DcNode* nd = es.d_nodes + es.lvl0[h];
+    double* Zin = (h - 1) & 1 ? es.Z1 : es.Z0;
+    double* Zout = h & 1 ? es.Z1 : es.Z0;
+    hipLaunchKernelGGL(k_dc_deflate, dim3(nl), dim3(256), 0, s, (const EigProb*)es.d_probs, (const int*)es.act, nd,
+                       es.vec, es.ivec, Zin);
+    hipLaunchKernelGGL(k_dc_secular, dim3(grid1(mm, 64), nl), dim3(64), 0, s, (const EigProb*)es.d_probs,
+                       (const int*)es.act, nd, es.vec, es.ivec);
+    hipLaunchKernelGGL(k_dc_zhat, dim3(grid1(mm, 64), nl), dim3(64), 0, s, (const EigProb*)es.d_probs,
+                       (const int*)es.act, nd, es.vec, (const int*)es.ivec);
+    hipLaunchKernelGGL(k_dc_place, dim3(nl), dim3(256), 0, s, (const EigProb*)es.d_probs, (const int*)es.act, nd,
+                       es.vec, es.ivec, (const double*)Zin, Zout);
+    hipLaunchKernelGGL(k_dc_vectors, dim3(grid1(mm, 4), nl), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                       (const int*)es.act, nd, es.vec, (const int*)es.ivec, es.U);
+    gemm_launch(es.dcg[h - 1], false, false);
+    hipLaunchKernelGGL(k_dc_lam, dim3(grid1(mm, 256), nl), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                       (const int*)es.act, nd, es.vec);
+  }
+  // 3. back-transformation, last panel first
+  hipLaunchKernelGGL(k_trd_tfactor, dim3(std::max(1, es.npanel), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                     (const int*)es.act, (const double*)es.Vh, (const double*)es.vec, es.Tf, std::max(1, es.npanel));
+  for (int pi = es.npanel - 1; pi >= 0; --pi) {
+    gemm_launch(es.bt1[pi], false, true);
+    gemm_launch(es.bt2[pi], false, true);
+    gemm_launch(es.bt3[pi], false, false);
+  }
+  hipLaunchKernelGGL(k_eig_out, dim3(gstride(nn), np), dim3(256), 0, s, (const EigProb*)es.d_probs, jprobs,
+                     (const int*)es.act, (const double*)es.vec, (const double*)es.Z0, (const double*)es.Z1,
+                     (const int*)es.zf, Am, Wm, jstate);
+}
+
 // block-Jacobi rounds of the two lanes (cyclic sweeps, register pivots and
 // tile updates).  Lane g's kernels go to its stream with its part of the work
 // lists, so the launch gaps of one lane overlap the other lane's work.  A
@@ -1393,11 +1768,15 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       }
       if (!any) break;
     }
+    unsigned long long* tw = (sample && (prof_mask & 8)) ? prof_work_ + 3 : (unsigned long long*)nullptr;
     for (int g = 0; g < kMaxLanes; ++g) {
       if (!live[g]) continue;
+      if (prof_mask & 8) prof_all_[3] += 1;
+      if (sample) prof_begin(3, ls[g]);
       hipLaunchKernelGGL(k_jac_apply_R, dim3(js.gxt[g + 1] - js.gxt[g]), dim3(64), 0, ls[g], js.d_probs,
                          js.d_xt + js.gxt[g], r, Am, Wm, (const double*)js.qbuf, (const int*)js.rot,
-                         (const int*)js.state, js.offmax, nmol);
+                         (const int*)js.state, js.offmax, nmol, tw);
+      if (sample) prof_end(3, nullptr, 0, ls[g]);
     }
     jacobi_rounds_ += 1;
     prof_round_ += 1;
@@ -1507,7 +1886,7 @@ void Batch::gemm(int kind, const double* a, const double* b, double* c, const do
 
 void Batch::prof_enable(int mask) {
   prof_mask = mask;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < kProfKinds; ++k) {
     prof_used_[k] = prof_all_[k] = 0;
     // events created up front: creating them inside a timed loop costs host time
     if ((mask & (1 << k)) && prof_ev_[k].size() < 4096) {
@@ -1517,8 +1896,8 @@ void Batch::prof_enable(int mask) {
     }
   }
   prof_round_ = 0;
-  if (!prof_work_) prof_work_ = alloc<unsigned long long>(3);
-  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, (double*)prof_work_, 3LL, 0.0);
+  if (!prof_work_) prof_work_ = alloc<unsigned long long>(kProfKinds);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, (double*)prof_work_, (long long)kProfKinds, 0.0);
   HIPCHECK(hipStreamSynchronize(stream));
 }
 
@@ -1543,7 +1922,7 @@ void Batch::prof_end(int kind, const int* mask, int mstride, hipStream_t s) {
 }
 
 int Batch::prof_result(int kind, int* launches, double* total_ms, double* work) {
-  if (kind < 1 || kind > 2) throw std::runtime_error("unknown kernel family");
+  if (kind < 1 || kind >= kProfKinds) throw std::runtime_error("unknown kernel family");
   HIPCHECK(hipStreamSynchronize(stream));
   double t = 0.0;
   for (int i = 0; i + 1 < prof_used_[kind]; i += 2) {
@@ -1648,9 +2027,16 @@ void Batch::setup() {
   hipLaunchKernelGGL(k_cn, gat, dim3(64), 0, stream, d_sub, el, at, xyz, cn, cn_cov, cn_d4, cn_eeq);
   hipLaunchKernelGGL(k_shell_setup, dim3(grid1(sub_max_nsh, 64), ns), dim3(64), 0, stream, d_sub, el, at, sh_at,
                      sh_m, cn, selfe, dsedcn, shgam, zsh);
+  // a resident wavefunction: every selected molecule starts from the shell
+  // charges and multipole moments the previous run left on the device, as
+  // the reference starts from the wavefunction of a reused results object
+  // (src/api/interface.f90:83-101), so no initial guess is computed
+  const bool resident = keep_wfn_ && wfn_valid_;
+  const bool eeq = opt.guess == 0 && !resident;
+  wfn_valid_ = false;
   // stream2 (the EEQ solve below) may start once CN and zsh exist; its
   // launches are enqueued after the integrals so those start first.
-  if (opt.guess == 0) {
+  if (eeq) {
     HIPCHECK(hipEventRecord(ev_fork, stream));
     HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
   }
@@ -1685,7 +2071,11 @@ void Batch::setup() {
   // beside the integrals and S^(-1/2) (it reads only CN, zsh and geometry,
   // writes only its own system and the charges); joined below.
   std::vector<int> big;
-  if (opt.guess == 0) {
+  if (resident) {
+    hipLaunchKernelGGL(k_eeq_reset, dim3(grid1(ns, 64)), dim3(64), 0, stream, d_sub, ns, eeq_failed);
+    hipLaunchKernelGGL(k_q_from_qsh, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, sh_at,
+                       (const double*)qsh, q);
+  } else if (eeq) {
     const long long ne = sub_max_n + 1;
     hipLaunchKernelGGL(k_eeq_matrix, dim3(grid1(ne * ne, 256), ns), dim3(256), 0, stream2, d_sub, el, at, xyz,
                        cn_eeq, E, xE);
@@ -1714,17 +2104,18 @@ void Batch::setup() {
     hipLaunchKernelGGL(k_sad_guess, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, at_sh0, q,
                        qsh, (const double*)zsh);
   }
-  hipLaunchKernelGGL(k_zero_atoms, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, dipm, qp);
+  if (!resident)
+    hipLaunchKernelGGL(k_zero_atoms, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, dipm, qp);
   // S-orthonormal initial orbital basis C = S^(-1/2)
   orthogonalizer();
-  if (opt.guess == 0) {
+  if (eeq) {
     HIPCHECK(hipEventRecord(ev_join, stream2));
     HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
   }
   // molecules whose S^(-1/2) or EEQ guess failed do not iterate
   hipLaunchKernelGGL(k_fail_fold, dim3(grid1(nmol, 64)), dim3(64), 0, stream, d_state, nmol, (const int*)ns_failed, 1,
                      kFailOrtho, (const int*)phase_act);
-  if (opt.guess == 0)
+  if (eeq)
     hipLaunchKernelGGL(k_fail_fold, dim3(grid1(nmol, 64)), dim3(64), 0, stream, d_state, nmol, (const int*)eeq_failed,
                        1, kFailEeq, (const int*)phase_act);
   if (use_ql) HIPCHECK(hipMemsetAsync(eig_failed, 0, nmol * sizeof(int), stream));
@@ -1921,6 +2312,19 @@ void Batch::run() {
     if (++guard > opt.maxiter + 2) break;
   }
   gradient();
+  // the final shell charges and moments of every molecule are on the device
+  wfn_valid_ = true;
+}
+
+void Batch::set_positions(int m, const double* xyz_in) {
+  if (m < 0 || m >= nmol || !occupied_[m]) throw std::runtime_error("invalid molecule index");
+  const MolDesc& d = desc[m];
+  HostMol& hm = hmol_[m];
+  hm.xyz.assign(xyz_in, xyz_in + 3 * (size_t)d.n);
+  HIPCHECK(hipStreamSynchronize(stream));
+  HIPCHECK(hipMemcpy(xyz + 3 * (size_t)d.a0, hm.xyz.data(), 3 * (size_t)d.n * sizeof(double),
+                     hipMemcpyHostToDevice));
+  setup_done_ = false;
 }
 
 void Batch::charges(int m, double* out) const {
@@ -2251,6 +2655,20 @@ int xtbamd_solvent_known(const char* solvent, int kernel) {
   return xtbamd::solv_param_set(solvent, kernel) ? 1 : 0;
 }
 
+int xtbamd_batch_set_positions(xtbamd_batch b, int m, const double* positions) {
+  try {
+    if (!b || !positions) throw std::runtime_error("invalid position arguments");
+    b->batch->set_positions(m, positions);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_keep_wavefunction(xtbamd_batch b, int on) {
+  if (!b) return -1;
+  b->batch->keep_wavefunction(on != 0);
+  return 0;
+}
+
 int xtbamd_batch_setup(xtbamd_batch b) {
   try {
     b->batch->select_all();
@@ -2287,7 +2705,7 @@ int xtbamd_batch_counters(xtbamd_batch b, int* scc_steps, long long* jacobi_roun
 
 int xtbamd_batch_profile(xtbamd_batch b, int kernels) {
   try {
-    if (!b || kernels < 0 || kernels > 3) throw std::runtime_error("unknown kernel family");
+    if (!b || kernels < 0 || kernels > 7) throw std::runtime_error("unknown kernel family");
     b->batch->prof_enable(kernels << 1);
     return 0;
   } catch (const std::exception& e) { return fail(e); }
@@ -2438,6 +2856,80 @@ int xtbamd_singlepoint_noalloc(int natoms, const int* numbers, const double* pos
     copy_result(b.result(0, out->coeff || out->density), out);
     return 0;
   } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_eigh(int nprob, const int* n, const double* a, double* evals, double* evecs, double* tri) {
+  std::vector<void*> owned;
+  struct HipMem : xtbamd::EigMem {
+    std::vector<void*>* owned;
+    void* get(size_t bytes) override {
+      void* p = nullptr;
+      HIPCHECK(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+      owned->push_back(p);
+      return p;
+    }
+    void put(void* p) override {
+      auto it = std::find(owned->begin(), owned->end(), p);
+      if (it != owned->end()) owned->erase(it);
+      (void)hipFree(p);
+    }
+  } mem;
+  mem.owned = &owned;
+  auto release = [&]() {
+    (void)hipDeviceSynchronize();
+    for (void* p : owned) (void)hipFree(p);
+    owned.clear();
+  };
+  try {
+    if (nprob <= 0 || !n || !a || !evals || !evecs) throw std::runtime_error("invalid eigenproblem arguments");
+    std::vector<xtbamd::JacobiProb> jp(nprob);
+    std::vector<int> dims(nprob);
+    long long off = 0, voff = 0;
+    for (int k = 0; k < nprob; ++k) {
+      if (n[k] < 2) throw std::runtime_error("eigenproblems of dimension < 2 are not supported");
+      std::memset(&jp[k], 0, sizeof(jp[k]));
+      jp[k].dim = dims[k] = n[k];
+      jp[k].ld = n[k];
+      jp[k].a_off = jp[k].w_off = off;
+      off += (long long)n[k] * n[k];
+    }
+    double* dA = (double*)mem.get(off * sizeof(double));
+    double* dW = (double*)mem.get(off * sizeof(double));
+    int* dst = (int*)mem.get((size_t)nprob * xtbamd::kJacState * sizeof(int));
+    xtbamd::JacobiProb* djp = (xtbamd::JacobiProb*)mem.get(nprob * sizeof(xtbamd::JacobiProb));
+    HIPCHECK(hipMemcpy(dA, a, off * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(djp, jp.data(), nprob * sizeof(xtbamd::JacobiProb), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(dst, 0, (size_t)nprob * xtbamd::kJacState * sizeof(int)));
+    xtbamd::EigSet es;
+    for (int k = 0; k < nprob; ++k) es.mols.push_back(k);
+    xtbamd::eig_set_build(es, dims, mem);
+    xtbamd::eig_set_run(es, djp, dst, nullptr, dA, dW, nullptr);
+    HIPCHECK(hipDeviceSynchronize());
+    std::vector<double> h((size_t)off);
+    HIPCHECK(hipMemcpy(h.data(), dA, off * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(evecs, dW, off * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int> st((size_t)nprob * xtbamd::kJacState);
+    HIPCHECK(hipMemcpy(st.data(), dst, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    long long eo = 0;
+    for (int k = 0; k < nprob; ++k) {
+      for (int i = 0; i < n[k]; ++i) evals[eo + i] = h[jp[k].a_off + (long long)i * n[k] + i];
+      if (tri) {
+        const long long ld = es.ld[k];
+        HIPCHECK(hipMemcpy(tri + 2 * eo, es.vec + es.v_off[k] + xtbamd::kVd0 * ld, n[k] * sizeof(double),
+                           hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(tri + 2 * eo + n[k], es.vec + es.v_off[k] + xtbamd::kVe0 * ld, n[k] * sizeof(double),
+                           hipMemcpyDeviceToHost));
+      }
+      if (st[(size_t)k * xtbamd::kJacState + 1]) throw std::runtime_error("eigensolver produced non-finite values");
+      eo += n[k];
+    }
+    (void)voff;
+    release();
+    return 0;
+  } catch (const std::exception& e) {
+    release();
+    return fail(e);
+  }
 }
 
 int xtbamd_batch_size(xtbamd_batch b) { return b ? b->batch->size() : 0; }

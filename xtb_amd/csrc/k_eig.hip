@@ -1,0 +1,805 @@
+// Blocked FP64 symmetric eigensolver for the large problems of a batch
+// (replaces dsytrd + dstedc + dormtr inside the reference's dsyevd,
+// src/mctc/lapack/eigensolve.F90:218-246, called from src/scc_core.f90:475):
+//
+//  1. Householder tridiagonalisation in panels of kEigNb columns.  Every
+//     column is one launch of k_trd_symv (w = A22 v over the trailing matrix,
+//     one wave per row, coalesced row reads) and one of k_trd_fin (one
+//     workgroup per problem: the panel corrections of w, its scaling, and the
+//     next column's reflector); a panel ends with the rank-2 nb update
+//     A22 -= X^T Y of the trailing matrix on the matrix cores (k_bgemm).
+//  2. Divide and conquer on the tridiagonal matrix: leaves of at most 64
+//     solved whole in LDS, merges level by level with deflation (one
+//     workgroup per merge), the secular equation's roots by safeguarded
+//     Newton steps (one thread per root), eigenvectors from the recomputed
+//     z of Gu and Eisenstat (SIAM J. Matrix Anal. Appl. 16 (1995) 172) and
+//     one matrix-core product per merge.
+//  3. Back-transformation of the eigenvectors with the compact WY form
+//     I - V T V^T of each panel (three matrix-core products per panel).
+//
+// Layouts: every n x n array of a problem is row-major with row stride ld;
+// the Householder vector of column j is row j of Vh (entries j+1..n-1, a 1 at
+// j+1); eigenvectors are rows (row j = eigenvector j) throughout the divide
+// and conquer and the back-transformation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_util.h"
+#include "engine.h"
+
+namespace xtbamd {
+
+constexpr int kEigNb = 32;      // panel width of the tridiagonalisation
+constexpr int kDcLeaf = 64;     // largest leaf of the divide and conquer tree
+constexpr int kEigVecs = 18;    // vectors of length ld per problem in the vector workspace
+// slots of the vector workspace (doubles)
+constexpr int kVd = 0, kVe = 1, kVtau = 2, kVw = 3, kVlam = 4, kVlam2 = 5, kVdn = 6, kVz = 7, kVdd = 8,
+              kVzz = 9, kVtr = 10, kVzh = 11, kVrc = 12, kVrs = 13, kVy = 14, kVev = 15, kVd0 = 16, kVe0 = 17;
+// slots of the integer workspace (ints, length ld each)
+constexpr int kIperm = 0, kIkeep = 1, kIpos = 2, kIorg = 3, kIrot = 4, kIrot2 = 5, kIrow = 6, kIcnt = 7;
+constexpr int kEigInts = 8;
+
+// One batched product job: C = beta C + alpha op(A) op(B), op(A) M x K,
+// op(B) K x N, row-major with the given row strides.  TA: A is stored K x M;
+// TB: B is stored N x K.  bk (optional) gathers the K index of B's rows
+// (B row bk[k] for op(B) row k, !TB only); crow (optional) scatters C's rows;
+// kdev (optional) reads M = K = *kdev on the device (at most the tile grid's M).
+struct GemmJob {
+  int M, N, K;
+  int tm, tn;             // tiles along M and N (64 each)
+  long long tile0;        // first tile of this job in the launch
+  const double* A;
+  const double* B;
+  double* C;
+  long long lda, ldb, ldc;
+  const int* bk;
+  const int* crow;
+  const int* kdev;
+  const int* act;         // (optional) the job is skipped when *act == 0
+  double alpha, beta;
+};
+
+// One problem of the blocked eigensolver.
+struct EigProb {
+  int n, ld;         // dimension, row stride of the solver's n x n arrays
+  long long m_off;   // this problem's n x ld blocks in the solver's A, Vh, Z0, Z1, U
+  long long v_off;   // this problem's kEigVecs x ld vector block
+  long long i_off;   // this problem's kEigInts x ld integer block
+  long long x_off;   // this problem's panel buffers X, Y (2 kEigNb x ld each)
+  long long p_off;   // this problem's n x kEigNb product buffers of the back-transformation
+  int mol;           // batch slot (index of its Jacobi problem)
+  int depth;         // levels of the divide and conquer tree (leaves at this depth)
+};
+
+// A merge of the divide and conquer tree: rows [start, start + n1 + n2) of
+// problem prob, split after n1 rows.
+struct DcNode {
+  int prob, start, n1, n2;
+};
+
+// ---------------------------------------------------------------------------
+// batched FP64 product on the matrix cores: 64 x 64 tiles, four waves of 2 x 2
+// v_mfma_f64_16x16x4f64 blocks, operands staged through LDS in k-blocks of 16
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) k_bgemm(const GemmJob* __restrict__ jobs, int njobs,
+                                               const int2* __restrict__ tiles) {
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  const int2 tl = tiles[blockIdx.x];
+  const GemmJob jb = jobs[tl.x];
+  if (jb.act && !*jb.act) return;
+  int M = jb.M, K = jb.K;
+  if (jb.kdev) {
+    M = *jb.kdev;
+    K = M;
+  }
+  const int N = jb.N;
+  const int row0 = (tl.y / jb.tn) * 64, col0 = (tl.y % jb.tn) * 64;
+  if (row0 >= M || col0 >= N) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1, i16 = lane & 15, k4 = lane >> 4;
+  typedef double v4d_ __attribute__((ext_vector_type(4)));
+  v4d_ acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[r][c] = v4d_{0.0, 0.0, 0.0, 0.0};
+  for (int kb = 0; kb < K; kb += 16) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int ii, kk;
+      if (TA) { kk = e >> 6; ii = e & 63; } else { ii = e >> 4; kk = e & 15; }
+      const int gi = row0 + ii, gk = kb + kk;
+      double x = 0.0;
+      if (gi < M && gk < K) x = TA ? jb.A[(long long)gk * jb.lda + gi] : jb.A[(long long)gi * jb.lda + gk];
+      As[kk][ii] = x;
+      int jj, k2;
+      if (TB) { jj = e >> 4; k2 = e & 15; } else { k2 = e >> 6; jj = e & 63; }
+      const int gj = col0 + jj, gk2 = kb + k2;
+      double y = 0.0;
+      if (gj < N && gk2 < K) {
+        if (TB) y = jb.B[(long long)gj * jb.ldb + gk2];
+        else y = jb.B[(long long)(jb.bk ? jb.bk[gk2] : gk2) * jb.ldb + gj];
+      }
+      Bs[k2][jj] = y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      double a[2], b[2];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) a[r] = As[kk + k4][32 * wr + 16 * r + i16];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) b[c] = Bs[kk + k4][32 * wc + 16 * c + i16];
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int gj = col0 + 32 * wc + 16 * c + i16;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int gi = row0 + 32 * wr + 16 * r + k4 + 4 * i;
+        if (gi >= M || gj >= N) continue;
+        const long long row = jb.crow ? jb.crow[gi] : gi;
+        double* cp = jb.C + row * jb.ldc + gj;
+        *cp = jb.beta == 0.0 ? jb.alpha * acc[r][c][i] : jb.beta * *cp + jb.alpha * acc[r][c][i];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1. tridiagonalisation
+
+// w = A22 v for column j: rows j+1..n-1, four rows per wave, 16 per workgroup
+__global__ void __launch_bounds__(256) k_trd_symv(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                  const double* __restrict__ A, const double* __restrict__ Vh,
+                                                  double* vec, int j) {
+  if (!act[blockIdx.y]) return;
+  const EigProb p = probs[blockIdx.y];
+  const int n = p.n, ld = p.ld;
+  if (j >= n - 1) return;
+  const int lane = threadIdx.x & 63;
+  const int r0 = j + 1 + blockIdx.x * 16 + (threadIdx.x >> 6) * 4;
+  if (r0 >= n) return;
+  const double* Ap = A + p.m_off;
+  const double* v = Vh + p.m_off + (long long)j * ld;
+  double* w = vec + p.v_off + (long long)kVw * ld;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  const int nr = min(4, n - r0);
+  for (int c = j + 1 + lane; c < n; c += 64) {
+    const double vc = v[c];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < nr) s[q] = fma(Ap[(long long)(r0 + q) * ld + c], vc, s[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double t = wave_sum(s[q]);
+    if (lane == 0 && q < nr) w[r0 + q] = t;
+  }
+}
+
+// Deterministic sum over a 1024-thread block (16 waves); sh holds 16 slots.
+__device__ __forceinline__ double sum1024(double v, double* sh) {
+  v = wave_sum(v);
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[wv] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) r += sh[i];
+  __syncthreads();
+  return r;
+}
+
+// One workgroup (1024 threads) per problem.  finish >= 0: complete
+// w = tau (A22 v - W V^T v - V W^T v) + alpha v for column finish (its panel
+// index i = finish - j0) and store it in the panel buffers.  next >= 0: form
+// the reflector of column next from row next of A corrected by the panel's
+// first (next - j0) vector pairs: d[next], e[next], tau[next], v (row next of
+// Vh and of the panel buffers); at next = n - 1 only d[next].
+// X rows [0, nb): v of the panel's columns, rows [nb, 2nb): their w; Y the
+// other way round, so that the trailing update is A22 -= X^T Y.
+__global__ void __launch_bounds__(1024) k_trd_fin(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                  const double* __restrict__ A, double* Vh, double* X, double* vec,
+                                                  int finish, int next, int j0) {
+  if (!act[blockIdx.x]) return;
+  __shared__ double sh[16];
+  __shared__ double y[2 * kEigNb];
+  __shared__ double cw[2 * kEigNb];
+  __shared__ double bc[4];
+  const EigProb p = probs[blockIdx.x];
+  const int n = p.n, ld = p.ld, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  double* Xp = X + p.x_off;                              // X: 2 nb rows
+  double* Yp = X + p.x_off + (long long)2 * kEigNb * ld;   // Y: 2 nb rows
+  double* d = vec + p.v_off + (long long)kVd * ld;
+  double* e = vec + p.v_off + (long long)kVe * ld;
+  double* tau = vec + p.v_off + (long long)kVtau * ld;
+  double* w = vec + p.v_off + (long long)kVw * ld;
+  if (finish >= 0 && finish < n - 1) {
+    const int j = finish, i = j - j0;
+    const double* v = Vh + p.m_off + (long long)j * ld;
+    // y1 = V^T v (t < i), y2 = W^T v over rows j+1..n-1: one wave per dot product
+    for (int t = wv; t < 2 * i; t += 16) {
+      const double* row = t < i ? Xp + (long long)t * ld : Xp + (long long)(kEigNb + t - i) * ld;
+      double s = 0.0;
+      for (int r = j + 1 + lane; r < n; r += 64) s = fma(row[r], v[r], s);
+      s = wave_sum(s);
+      if (lane == 0) y[t] = s;
+    }
+    __syncthreads();
+    const double tj = tau[j];
+    double dot = 0.0;
+    for (int r = j + 1 + tid; r < n; r += 1024) {
+      double x = w[r];
+      for (int t = 0; t < i; ++t)
+        x -= Xp[(long long)(kEigNb + t) * ld + r] * y[t] + Xp[(long long)t * ld + r] * y[i + t];
+      x *= tj;
+      w[r] = x;
+      dot = fma(x, v[r], dot);
+    }
+    dot = sum1024(dot, sh);
+    const double al = -0.5 * tj * dot;
+    for (int r = j + 1 + tid; r < n; r += 1024) {
+      const double x = fma(al, v[r], w[r]);
+      Xp[(long long)(kEigNb + i) * ld + r] = x;
+      Yp[(long long)i * ld + r] = x;
+    }
+    __syncthreads();
+  }
+  if (next >= 0 && next < n) {
+    const int jn = next, i = jn - j0;
+    // coefficients of the corrections: v_t[jn] and w_t[jn] for t < i
+    if (tid < i) {
+      cw[tid] = Xp[(long long)tid * ld + jn];
+      cw[kEigNb + tid] = Xp[(long long)(kEigNb + tid) * ld + jn];
+    }
+    __syncthreads();
+    const double* arow = A + p.m_off + (long long)jn * ld;
+    double* v = Vh + p.m_off + (long long)jn * ld;
+    double* a = vec + p.v_off + (long long)kVy * ld;   // the corrected row
+    double ss = 0.0;
+    for (int r = jn + tid; r < n; r += 1024) {
+      double x = arow[r];
+      for (int t = 0; t < i; ++t)
+        x -= cw[t] * Xp[(long long)(kEigNb + t) * ld + r] + cw[kEigNb + t] * Xp[(long long)t * ld + r];
+      a[r] = x;
+      if (r >= jn + 2) ss = fma(x, x, ss);
+    }
+    ss = sum1024(ss, sh);   // includes the barrier that publishes a[]
+    if (tid == 0) {
+      d[jn] = a[jn];
+      if (jn < n - 1) {
+        const double alpha = a[jn + 1];
+        double t = 0.0, beta = alpha, scale = 0.0;
+        if (ss > 0.0) {
+          beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+          t = (beta - alpha) / beta;
+          scale = 1.0 / (alpha - beta);
+        }
+        e[jn] = beta;
+        tau[jn] = t;
+        bc[0] = scale;
+      }
+    }
+    __syncthreads();
+    if (jn < n - 1) {
+      // v: zero up to jn, 1 at jn + 1, the scaled row beyond
+      const double scale = bc[0];
+      for (int r = tid; r < n; r += 1024) {
+        const double x = r <= jn ? 0.0 : (r == jn + 1 ? 1.0 : a[r] * scale);
+        v[r] = x;
+        if (i < kEigNb) {
+          Xp[(long long)i * ld + r] = x;
+          Yp[(long long)(kEigNb + i) * ld + r] = x;
+        }
+      }
+    }
+  }
+}
+
+// the tridiagonal form as it leaves the tridiagonalisation (kept beside the
+// working copy that the divide and conquer splits)
+__global__ void k_trd_save(const EigProb* __restrict__ probs, const int* __restrict__ act, double* vec) {
+  if (!act[blockIdx.y]) return;
+  const EigProb p = probs[blockIdx.y];
+  double* V = vec + p.v_off;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < p.n; r += gridDim.x * blockDim.x) {
+    V[kVd0 * p.ld + r] = V[kVd * p.ld + r];
+    V[kVe0 * p.ld + r] = r + 1 < p.n ? V[kVe * p.ld + r] : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. divide and conquer
+
+// Cuppen's splitting: at every merge the coupling beta = |e| is taken off the
+// two diagonal entries next to the split (T = diag(T1, T2) + beta u u^T)
+__global__ void k_dc_split(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                           const DcNode* __restrict__ nodes, int nn, double* vec) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  const DcNode nd = nodes[k];
+  if (!act[nd.prob]) return;
+  const EigProb p = probs[nd.prob];
+  double* d = vec + p.v_off + (long long)kVd * p.ld;
+  const double* e = vec + p.v_off + (long long)kVe * p.ld;
+  const int m = nd.start + nd.n1;
+  const double b = fabs(e[m - 1]);
+  d[m - 1] -= b;
+  d[m] -= b;
+}
+
+// leaves: the dense tridiagonal block in LDS, cyclic Jacobi to convergence,
+// eigenvalues ascending into lam, eigenvectors as rows of Z (zero outside
+// the leaf's columns)
+__global__ void __launch_bounds__(256) k_dc_leaf(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                 const int4* __restrict__ leaves, double* vec, double* Z) {
+  __shared__ double a[64 * kLd];
+  __shared__ double q[64 * kLd];
+  __shared__ double cs[96];
+  __shared__ double red[4];
+  __shared__ int pq[64];
+  __shared__ int flags[2];
+  __shared__ int rank[64];
+  const int4 lf = leaves[blockIdx.x];   // (prob, start, size, -)
+  if (!act[lf.x]) return;
+  const EigProb p = probs[lf.x];
+  const int s0 = lf.y, m = lf.z, ld = p.ld;
+  const double* d = vec + p.v_off + (long long)kVd * ld;
+  const double* e = vec + p.v_off + (long long)kVe * ld;
+  double* lam = vec + p.v_off + (long long)kVlam * ld;
+  double scl = 0.0;
+  for (int u = threadIdx.x; u < m; u += blockDim.x)
+    scl = fmax(scl, fabs(d[s0 + u]) + (u + 1 < m ? fabs(e[s0 + u]) : 0.0));
+  scl = block_max<256>(scl, red);
+  for (int x = threadIdx.x; x < 64 * 64; x += blockDim.x) {
+    const int u = x >> 6, v = x & 63;
+    double val = 0.0;
+    if (u < m && v < m) {
+      if (u == v) val = d[s0 + u];
+      else if (v == u + 1) val = e[s0 + u];
+      else if (u == v + 1) val = e[s0 + v];
+    }
+    a[u * kLd + v] = val;
+    q[u * kLd + v] = (u == v) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  jacobi64(a, q, 1e-15 * fmax(scl, 1e-300), 40, cs, pq, flags, red);
+  __syncthreads();
+  // ascending order of the leaf's eigenvalues (ties by index)
+  for (int u = threadIdx.x; u < m; u += blockDim.x) {
+    const double x = a[u * kLd + u];
+    int r = 0;
+    for (int v = 0; v < m; ++v) {
+      const double y = a[v * kLd + v];
+      r += (y < x || (y == x && v < u)) ? 1 : 0;
+    }
+    rank[u] = r;
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < m * m; x += blockDim.x) {
+    const int u = x / m, r = x % m;   // eigenvector u (column u of q), component r
+    Z[p.m_off + (long long)(s0 + rank[u]) * ld + s0 + r] = q[r * kLd + u];
+  }
+  for (int u = threadIdx.x; u < m; u += blockDim.x) lam[s0 + rank[u]] = a[u * kLd + u];
+}
+
+// merge, part 1 (one workgroup of 256 per node): z from the children's
+// boundary components, the merged ascending order, normalisation, deflation
+// (small z components; close pairs rotated so that one z component vanishes,
+// the rotations applied to the eigenvector rows), and the list of the k
+// non-deflated entries for the secular equation.
+__global__ void __launch_bounds__(256) k_dc_deflate(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                    const DcNode* __restrict__ nodes, double* vec, int* ivec,
+                                                    double* Z) {
+  __shared__ double red[4];
+  __shared__ double bz_s;
+  const DcNode nd = nodes[blockIdx.x];
+  if (!act[nd.prob]) return;
+  const EigProb p = probs[nd.prob];
+  const int ld = p.ld, s0 = nd.start, n1 = nd.n1, m = nd.n1 + nd.n2;
+  double* V = vec + p.v_off;
+  int* I = ivec + p.i_off;
+  const double* lam = V + kVlam * ld;
+  const double* e = V + kVe * ld;
+  double* dn = V + kVdn * ld;     // sorted (then deflation-modified) poles, node-local index
+  double* z = V + kVz * ld;       // sorted normalised z
+  double* rc = V + kVrc * ld;     // rotation cosines / sines by step
+  double* rs = V + kVrs * ld;
+  int* perm = I + kIperm * ld;    // sorted index -> child row (node-local)
+  int* keep = I + kIkeep * ld;    // non-deflated sorted indices, ascending pole
+  int* rot = I + kIrot * ld;      // rotation pairs (prev, cur) by step
+  int* rot2 = I + kIrot2 * ld;
+  int* cnt = I + kIcnt * ld;      // [s0]: k, [s0 + 1]: rotations
+  double* zbuf = V + kVzz * ld;
+  Z += p.m_off;
+  const double ecoup = e[s0 + n1 - 1];
+  const double beta = fabs(ecoup), sg = ecoup >= 0.0 ? 1.0 : -1.0;
+  // z in child order: the last component of the left child's vectors, the
+  // first component (times the sign of the coupling) of the right child's
+  for (int u = threadIdx.x; u < m; u += blockDim.x)
+    zbuf[s0 + u] = u < n1 ? Z[(long long)(s0 + u) * ld + s0 + n1 - 1] : sg * Z[(long long)(s0 + u) * ld + s0 + n1];
+  __syncthreads();
+  // merged ascending order of the two sorted children (ties: left first)
+  for (int u = threadIdx.x; u < m; u += blockDim.x) {
+    const double x = lam[s0 + u];
+    int lo, hi, base;
+    if (u < n1) { lo = n1; hi = m; base = u; } else { lo = 0; hi = n1; base = u - n1; }
+    // number of entries of the other child before x
+    int a0 = lo, a1 = hi;
+    while (a0 < a1) {
+      const int mid = (a0 + a1) >> 1;
+      const double y = lam[s0 + mid];
+      const bool before = u < n1 ? (y < x) : (y <= x);
+      if (before) a0 = mid + 1; else a1 = mid;
+    }
+    const int r = base + (a0 - lo);
+    perm[s0 + r] = u;
+  }
+  __syncthreads();
+  double zn = 0.0, dmax = 0.0;
+  for (int r = threadIdx.x; r < m; r += blockDim.x) {
+    const int u = perm[s0 + r];
+    dn[s0 + r] = lam[s0 + u];
+    const double zz = zbuf[s0 + u];
+    z[s0 + r] = zz;
+    zn = fma(zz, zz, zn);
+    dmax = fmax(dmax, fabs(lam[s0 + u]));
+  }
+  zn = block_sum<256>(zn, red);
+  dmax = block_max<256>(dmax, red);
+  const double rn = 1.0 / sqrt(zn);
+  for (int r = threadIdx.x; r < m; r += blockDim.x) z[s0 + r] *= rn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double bz = beta * zn;
+    bz_s = bz;
+    const double tol = 8.0 * 2.220446049250313e-16 * fmax(dmax, bz);
+    int k = 0, nr = 0, prev = -1;
+    for (int r = 0; r < m; ++r) {
+      if (bz * fabs(z[s0 + r]) <= tol) {   // negligible coupling: d is an eigenvalue
+        z[s0 + r] = 0.0;
+        continue;
+      }
+      if (prev < 0) {
+        prev = r;
+        continue;
+      }
+      double c = z[s0 + r], s = z[s0 + prev];
+      const double h = sqrt(c * c + s * s);
+      c /= h;
+      s = -s / h;
+      const double t = dn[s0 + r] - dn[s0 + prev];
+      if (fabs(t * c * s) <= tol) {
+        // close pair: rotate so that z[prev] vanishes; prev is deflated
+        const double dp = dn[s0 + prev], dr = dn[s0 + r];
+        dn[s0 + prev] = dp * c * c + dr * s * s;
+        dn[s0 + r] = dp * s * s + dr * c * c;
+        z[s0 + r] = h;
+        z[s0 + prev] = 0.0;
+        rot[s0 + nr] = prev;
+        rot2[s0 + nr] = r;
+        rc[s0 + nr] = c;
+        rs[s0 + nr] = s;
+        ++nr;
+      } else {
+        keep[s0 + k++] = prev;
+      }
+      prev = r;
+    }
+    if (prev >= 0) keep[s0 + k++] = prev;
+    cnt[s0] = k;
+    cnt[s0 + 1] = nr;
+  }
+  __syncthreads();
+  // the rotations, in order, on the eigenvector rows (node columns only)
+  const int nr = cnt[s0 + 1];
+  for (int t = 0; t < nr; ++t) {
+    const long long ra = s0 + perm[s0 + rot[s0 + t]], rb = s0 + perm[s0 + rot2[s0 + t]];
+    const double c = rc[s0 + t], s = rs[s0 + t];
+    for (int col = threadIdx.x; col < m; col += blockDim.x) {
+      const double xa = Z[ra * ld + s0 + col], xb = Z[rb * ld + s0 + col];
+      Z[ra * ld + s0 + col] = c * xa + s * xb;
+      Z[rb * ld + s0 + col] = -s * xa + c * xb;
+    }
+    __syncthreads();
+  }
+  // poles and z of the secular equation (ascending poles), beta |z|^2 in vec
+  const int k = cnt[s0];
+  double* dd = V + kVdd * ld;
+  double* zz = V + kVzz * ld;
+  for (int t = threadIdx.x; t < k; t += blockDim.x) {
+    dd[s0 + t] = dn[s0 + keep[s0 + t]];
+    zz[s0 + t] = z[s0 + keep[s0 + t]];
+  }
+  if (threadIdx.x == 0) V[kVy * ld + s0] = bz_s;
+}
+
+// secular equation 1 + bz sum_t zz_t^2 / (dd_t - lambda) = 0: root t lies in
+// (dd_t, dd_{t+1}) (the last in (dd_{k-1}, dd_{k-1} + bz)); it is kept as
+// its pole of origin (the nearer end of the interval) and the offset tau from
+// it, so that every difference dd_i - lambda is formed without cancellation.
+// Safeguarded Newton steps inside a shrinking bracket.
+__global__ void k_dc_secular(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                             const DcNode* __restrict__ nodes, double* vec, int* ivec) {
+  const DcNode nd = nodes[blockIdx.y];
+  if (!act[nd.prob]) return;
+  const EigProb p = probs[nd.prob];
+  const int ld = p.ld, s0 = nd.start;
+  double* V = vec + p.v_off;
+  int* I = ivec + p.i_off;
+  const int k = I[kIcnt * ld + s0];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k) return;
+  const double* dd = V + kVdd * ld + s0;
+  const double* zz = V + kVzz * ld + s0;
+  const double bz = V[kVy * ld + s0];
+  int org;
+  double a, b;
+  if (t < k - 1) {
+    const double gap = dd[t + 1] - dd[t];
+    const double mid = 0.5 * gap;
+    double f = 1.0;
+    for (int i = 0; i < k; ++i) f += bz * zz[i] * zz[i] / ((dd[i] - dd[t]) - mid);
+    if (f >= 0.0) { org = t; a = 0.0; b = mid; }
+    else { org = t + 1; a = -mid; b = 0.0; }
+  } else {
+    org = k - 1;
+    a = 0.0;
+    double s = 0.0;
+    for (int i = 0; i < k; ++i) s += zz[i] * zz[i];
+    b = bz * s;
+  }
+  const double dorg = dd[org];
+  double x = 0.5 * (a + b);
+  for (int it = 0; it < 200; ++it) {
+    double f = 1.0, fp = 0.0;
+    for (int i = 0; i < k; ++i) {
+      const double del = (dd[i] - dorg) - x;
+      const double q = zz[i] / del;
+      f = fma(bz * zz[i], q, f);
+      fp = fma(bz * q, q, fp);
+    }
+    if (f == 0.0) { a = b = x; break; }
+    if (f > 0.0) b = x; else a = x;
+    // Newton step (f is increasing in lambda), kept inside the bracket
+    double xn = x - f / fp;
+    if (!(xn > a && xn < b)) xn = 0.5 * (a + b);
+    if (xn == x || b - a <= 2.220446049250313e-16 * fmax(fabs(a), fabs(b))) {
+      x = xn;
+      break;
+    }
+    x = xn;
+  }
+  I[kIorg * ld + s0 + t] = org;
+  V[kVtr * ld + s0 + t] = x;
+}
+
+// recomputed z (Gu and Eisenstat): zh_i^2 = prod_j (lambda_j - d_i) /
+// (bz prod_{j != i} (d_j - d_i)), with the sign of z_i; one thread per i
+__global__ void k_dc_zhat(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                          const DcNode* __restrict__ nodes, double* vec, const int* __restrict__ ivec) {
+  const DcNode nd = nodes[blockIdx.y];
+  if (!act[nd.prob]) return;
+  const EigProb p = probs[nd.prob];
+  const int ld = p.ld, s0 = nd.start;
+  double* V = vec + p.v_off;
+  const int* I = ivec + p.i_off;
+  const int k = I[kIcnt * ld + s0];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const double* dd = V + kVdd * ld + s0;
+  const double* zz = V + kVzz * ld + s0;
+  const double* tr = V + kVtr * ld + s0;
+  const int* org = I + kIorg * ld + s0;
+  const double bz = V[kVy * ld + s0];
+  const double di = dd[i];
+  // lambda_j - d_i = (d_org(j) - d_i) + tau_j
+  double val = ((dd[org[k - 1]] - di) + tr[k - 1]) / bz;
+  for (int j = 0; j < k - 1; ++j) {
+    const double lj = (dd[org[j]] - di) + tr[j];
+    val *= j < i ? lj / (dd[j] - di) : lj / (dd[j + 1] - di);
+  }
+  V[kVzh * ld + s0 + i] = copysign(sqrt(fmax(val, 0.0)), zz[i]);
+}
+
+// merge, part 2 (one workgroup per node): every eigenvalue of the node
+// (roots and deflated poles) gets its ascending position; the product's row
+// gather (child rows of the non-deflated entries) and row scatter (output
+// rows of the roots), the copies of the deflated rows into the other buffer
+__global__ void __launch_bounds__(256) k_dc_place(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                  const DcNode* __restrict__ nodes, double* vec, int* ivec,
+                                                  const double* __restrict__ Zin, double* Zout) {
+  const DcNode nd = nodes[blockIdx.x];
+  if (!act[nd.prob]) return;
+  const EigProb p = probs[nd.prob];
+  const int ld = p.ld, s0 = nd.start, m = nd.n1 + nd.n2;
+  double* V = vec + p.v_off;
+  int* I = ivec + p.i_off;
+  const int k = I[kIcnt * ld + s0];
+  const double* dn = V + kVdn * ld + s0;
+  const double* z = V + kVz * ld + s0;
+  const double* dd = V + kVdd * ld + s0;
+  const double* tr = V + kVtr * ld + s0;
+  const int* org = I + kIorg * ld + s0;
+  const int* perm = I + kIperm * ld + s0;
+  const int* keep = I + kIkeep * ld + s0;
+  double* lam2 = V + kVlam2 * ld;
+  int* pos = I + kIpos * ld + s0;    // by sorted index r: output position
+  int* row = I + kIrow * ld + s0;    // by root t: child row of keep[t] (product gather)
+  // eigenvalue of sorted index r: a root for kept entries, the pole otherwise;
+  // kept entries are marked by z != 0
+  double* ev = V + kVev * ld + s0;
+  for (int t = threadIdx.x; t < k; t += blockDim.x) ev[keep[t]] = dd[org[t]] + tr[t];
+  __syncthreads();
+  for (int r = threadIdx.x; r < m; r += blockDim.x)
+    if (z[r] == 0.0) ev[r] = dn[r];
+  __syncthreads();
+  for (int r = threadIdx.x; r < m; r += blockDim.x) {
+    const double x = ev[r];
+    int c = 0;
+    for (int u = 0; u < m; ++u) {
+      const double y = ev[u];
+      c += (y < x || (y == x && u < r)) ? 1 : 0;
+    }
+    pos[r] = c;
+    lam2[s0 + c] = x;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < k; t += blockDim.x) {
+    row[t] = s0 + perm[keep[t]];
+    I[kIrot * ld + s0 + t] = s0 + pos[keep[t]];   // output row of root t (rotation list no longer needed)
+  }
+  // deflated rows: copied whole (node columns) to their positions
+  const double* Zi = Zin + p.m_off;
+  double* Zo = Zout + p.m_off;
+  for (int r = 0; r < m; ++r) {
+    if (z[r] != 0.0) continue;
+    const long long src = s0 + perm[r], dst = s0 + pos[r];
+    for (int col = threadIdx.x; col < m; col += blockDim.x) Zo[dst * ld + s0 + col] = Zi[src * ld + s0 + col];
+  }
+}
+
+// the eigenvectors of D + bz z z^T in the recomputed z: row t of U (t < k)
+// holds u_t(i) = zh_i / (d_i - lambda_t), normalised; one wave per row
+__global__ void __launch_bounds__(256) k_dc_vectors(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                    const DcNode* __restrict__ nodes, double* vec,
+                                                    const int* __restrict__ ivec, double* U) {
+  const DcNode nd = nodes[blockIdx.y];
+  if (!act[nd.prob]) return;
+  const EigProb p = probs[nd.prob];
+  const int ld = p.ld, s0 = nd.start;
+  const double* V = vec + p.v_off;
+  const int* I = ivec + p.i_off;
+  const int k = I[kIcnt * ld + s0];
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (t >= k) return;
+  const double* dd = V + kVdd * ld + s0;
+  const double* zh = V + kVzh * ld + s0;
+  const int og = I[kIorg * ld + s0 + t];
+  const double tt = V[kVtr * ld + s0 + t];
+  const double dorg = dd[og];
+  double* Ur = U + p.m_off + (long long)(s0 + t) * ld;
+  double s = 0.0;
+  for (int i = lane; i < k; i += 64) {
+    const double x = zh[i] / ((dd[i] - dorg) - tt);
+    Ur[i] = x;
+    s = fma(x, x, s);
+  }
+  s = wave_sum(s);
+  const double rn = 1.0 / sqrt(s);
+  for (int i = lane; i < k; i += 64) Ur[i] *= rn;
+}
+
+// the node's eigenvalues in ascending order back into lam
+__global__ void k_dc_lam(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                         const DcNode* __restrict__ nodes, double* vec) {
+  const DcNode nd = nodes[blockIdx.y];
+  if (!act[nd.prob]) return;
+  const EigProb p = probs[nd.prob];
+  const int m = nd.n1 + nd.n2;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  double* V = vec + p.v_off;
+  V[kVlam * p.ld + nd.start + r] = V[kVlam2 * p.ld + nd.start + r];
+}
+
+// ---------------------------------------------------------------------------
+// 3. back-transformation
+
+// T factor of each panel's compact WY form (I - V T V^T = H_j0 ... H_j0+nb-1,
+// forward, columnwise): T_ii = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) (V^T v_i)
+__global__ void __launch_bounds__(256) k_trd_tfactor(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                     const double* __restrict__ Vh, const double* __restrict__ vec,
+                                                     double* Tf, int npanel) {
+  __shared__ double g[kEigNb][kEigNb + 1];
+  __shared__ double T[kEigNb][kEigNb + 1];
+  if (!act[blockIdx.y]) return;
+  const int pan = blockIdx.x;
+  const EigProb p = probs[blockIdx.y];
+  const int n = p.n, ld = p.ld;
+  const int j0 = pan * kEigNb;
+  double* To = Tf + ((long long)blockIdx.y * npanel + pan) * kEigNb * kEigNb;
+  if (j0 >= n - 1) return;
+  const int jb = min(kEigNb, n - 1 - j0);
+  const double* tau = vec + p.v_off + (long long)kVtau * ld;
+  const double* Vp = Vh + p.m_off + (long long)j0 * ld;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // Gram matrix of the panel's vectors (upper part), one wave per entry
+  for (int x = wv; x < jb * jb; x += 4) {
+    const int a = x / jb, b = x % jb;
+    if (a >= b) continue;
+    // v_a is zero before row j0 + a + 1 (and 1 there), v_b before j0 + b + 1
+    double s = 0.0;
+    for (int r = j0 + b + 1 + lane; r < n; r += 64) s = fma(Vp[(long long)a * ld + r], Vp[(long long)b * ld + r], s);
+    s = wave_sum(s);
+    if (lane == 0) g[a][b] = s;
+  }
+  for (int x = threadIdx.x; x < kEigNb * kEigNb; x += blockDim.x) T[x / kEigNb][x % kEigNb] = 0.0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < jb; ++i) {
+      const double ti = tau[j0 + i];
+      T[i][i] = ti;
+      for (int a = 0; a < i; ++a) {
+        double s = 0.0;
+        for (int c = a; c < i; ++c) s += T[a][c] * g[c][i];
+        T[a][i] = -ti * s;
+      }
+    }
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < kEigNb * kEigNb; x += blockDim.x) To[x] = T[x / kEigNb][x % kEigNb];
+}
+
+// the problems' matrices into the solver's layout (problems of molecules
+// that no longer iterate are switched off)
+__global__ void __launch_bounds__(256) k_eig_in(const EigProb* __restrict__ probs, const JacobiProb* __restrict__ jp,
+                                                const MolState* __restrict__ mstate, int* act,
+                                                const double* __restrict__ Asrc, double* Aw) {
+  const EigProb p = probs[blockIdx.y];
+  const JacobiProb q = jp[p.mol];
+  const bool on = !mstate || mstate[p.mol].active;
+  if (blockIdx.x == 0 && threadIdx.x == 0) act[blockIdx.y] = on ? 1 : 0;
+  if (!on) return;
+  const int n = p.n;
+  for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < (long long)n * n;
+       x += (long long)gridDim.x * blockDim.x) {
+    const long long u = x / n, v = x % n;
+    Aw[p.m_off + u * p.ld + v] = Asrc[q.a_off + u * q.ld + v];
+  }
+}
+
+// eigenvalues onto the diagonal of the caller's A, eigenvector rows into its
+// W (the Jacobi problem's layout); a non-finite eigenvalue fails the problem
+__global__ void __launch_bounds__(256) k_eig_out(const EigProb* __restrict__ probs, const JacobiProb* __restrict__ jp,
+                                                 const int* __restrict__ act, const double* __restrict__ vec,
+                                                 const double* __restrict__ Z0, const double* __restrict__ Z1,
+                                                 const int* __restrict__ zfin, double* A, double* W, int* jstate) {
+  if (!act[blockIdx.y]) return;
+  const EigProb p = probs[blockIdx.y];
+  const JacobiProb q = jp[p.mol];
+  const int n = p.n;
+  const double* lam = vec + p.v_off + (long long)kVlam * p.ld;
+  const double* Zf = (zfin[blockIdx.y] ? Z1 : Z0) + p.m_off;
+  for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < (long long)n * n;
+       x += (long long)gridDim.x * blockDim.x) {
+    const long long u = x / n, v = x % n;
+    W[q.w_off + u * q.ld + v] = Zf[u * p.ld + v];
+    A[q.a_off + u * q.ld + v] = u == v ? lam[u] : 0.0;
+    if (u == v && !isfinite(lam[u])) jstate[p.mol * kJacState + 1] = 1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) jstate[p.mol * kJacState] = 1;
+}
+
+}  // namespace xtbamd

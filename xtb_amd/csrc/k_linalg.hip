@@ -1599,11 +1599,41 @@ __global__ void __launch_bounds__(64) k_jac_apply_wR(const JacobiProb* __restric
 // per workgroup: kind 0 updates A tile (y, z) = (k, l) (and measures the
 // off-diagonal maximum into the buffer of the sweep's parity, offmax[(sweep &
 // 1) * stride + problem]), kind 1 W tile (rb, l) = (y, z), kind 2 is a skip
+// algorithmic HBM bytes of one tile update: every element of A and W that a
+// rotation touches is read once and written once per round, A through one of
+// its two mirrored tiles (an off-diagonal A tile: 8 B read, 16 B written per
+// element; a diagonal one 8 + 8; a W tile 8 + 8); untouched tiles count zero
+__device__ __forceinline__ unsigned long long jac_tile_bytes(const JacobiProb* __restrict__ probs, const int4 it,
+                                                             int r, const int* __restrict__ rotflag,
+                                                             const int* __restrict__ state) {
+  if (it.w > 1) return 0ull;
+  const JacobiProb pr = probs[it.x];
+  if (state[it.x * kJacState]) return 0ull;
+  const int qb = (int)(pr.q_off / (32 * 32)), n = pr.dim;
+  auto valid = [&](int b, int w) { return (long long)max(0, min(w, n - w * b)); };
+  int lp, lq;
+  jac_pair(pr.nbp, pr.rounds, r, it.z, lp, lq);
+  const long long cl = valid(lp, 16) + valid(lq, 16);
+  if (it.w == 1) return rotflag[qb + it.z] ? 16ull * cl * valid(it.y, 32) : 0ull;
+  const bool measure = r < kBanded && (r + 1) % pr.rounds == 0;
+  if (!rotflag[qb + it.y] && !rotflag[qb + it.z] && !measure) return 0ull;
+  int kp, kq;
+  jac_pair(pr.nbp, pr.rounds, r, it.y, kp, kq);
+  const long long ck = valid(kp, 16) + valid(kq, 16);
+  return (it.y == it.z ? 16ull : 24ull) * ck * cl;
+}
+
 __global__ void __launch_bounds__(64) k_jac_apply_R(const JacobiProb* __restrict__ probs, const int4* __restrict__ list,
                                                     int r, double* A, double* W, const double* __restrict__ Qbuf,
                                                     const int* __restrict__ rotflag, const int* __restrict__ state,
-                                                    unsigned long long* offmax, int stride) {
+                                                    unsigned long long* offmax, int stride,
+                                                    unsigned long long* work) {
   const int4 it = list[blockIdx.x];
+  // sampled launches count their algorithmic bytes (one atomic per tile)
+  if (work && threadIdx.x == 0) {
+    const unsigned long long b = jac_tile_bytes(probs, it, r, rotflag, state);
+    if (b) atomicAdd(work, b);
+  }
   if (it.w == 0)
     jac_tile_a(probs, it, r, A, Qbuf, rotflag, state, offmax, stride);
   else if (it.w == 1)

@@ -83,6 +83,8 @@ def library():
                                                ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                                ctypes.POINTER(Result)]
     lib.xtbamd_singlepoint_noalloc.restype = ctypes.c_int
+    lib.xtbamd_eigh.argtypes = [ctypes.c_int, ip, _dp, _dp, _dp, _dp]
+    lib.xtbamd_eigh.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -163,17 +165,17 @@ class Batch:
         self._check(self._lib.xtbamd_batch_counters(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
-    KERNELS = {"gemm": 1, "jacobi": 2}
+    KERNELS = {"gemm": 1, "jacobi": 2, "apply": 3}
 
     def profile(self, *kernels):
-        """Start HIP-event timing of kernel families ("gemm", "jacobi")."""
+        """Start HIP-event timing of kernel families ("gemm", "jacobi" pivots, "apply" tile updates)."""
         bits = 0
         for k in kernels:
             bits |= 1 << (self.KERNELS[k] - 1)
         self._check(self._lib.xtbamd_batch_profile(self._h, bits))
 
     def profile_result(self, kernel="gemm"):
-        """(launches, summed ms, summed algorithmic flops) since profile()."""
+        """(launches, summed ms, summed algorithmic work: flops, or bytes for "apply") since profile()."""
         n, t, w = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         self._check(self._lib.xtbamd_batch_profile_result(self._h, self.KERNELS[kernel], ctypes.byref(n),
                                                           ctypes.byref(t), ctypes.byref(w)))
@@ -363,3 +365,27 @@ def singlepoint_noalloc(numbers, positions, charge=0.0, uhf=0, accuracy=1.0, ete
         raise SolverFailure(out)
     return out
 
+
+def eigh(matrices, tridiagonal=False):
+    """Eigen-decomposition of symmetric matrices by the engine's blocked
+    eigensolver (xtbamd_eigh): [(eigenvalues ascending, eigenvectors as rows)]
+    (and the tridiagonal forms [(d, e)] when tridiagonal=True)."""
+    lib = library()
+    mats = [np.ascontiguousarray(m, dtype=np.float64) for m in matrices]
+    n = np.array([m.shape[0] for m in mats], dtype=np.int32)
+    a = np.ascontiguousarray(np.concatenate([m.ravel() for m in mats]))
+    ev = np.zeros(int(n.sum()))
+    vec = np.zeros(a.size)
+    tri = np.zeros(2 * int(n.sum()))
+    rc = lib.xtbamd_eigh(len(mats), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _ptr(a), _ptr(ev), _ptr(vec),
+                         _ptr(tri))
+    if rc < 0:
+        raise RuntimeError(_err(lib))
+    out, tr, o, o2 = [], [], 0, 0
+    for k in n:
+        k = int(k)
+        out.append((ev[o:o + k].copy(), vec[o2:o2 + k * k].reshape(k, k).copy()))
+        tr.append((tri[2 * o:2 * o + k].copy(), tri[2 * o + k:2 * o + 2 * k - 1].copy()))
+        o += k
+        o2 += k * k
+    return (out, tr) if tridiagonal else out

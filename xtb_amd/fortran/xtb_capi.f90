@@ -84,6 +84,19 @@ module xtbamd_capi
          integer(c_int), value :: kernel
          integer(c_int) :: ok
       end function engine_solvent_known
+      function engine_set_positions(handle, m, positions) result(rc) bind(C, name="xtbamd_batch_set_positions")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: m
+         real(c_double), intent(in) :: positions(*)
+         integer(c_int) :: rc
+      end function engine_set_positions
+      function engine_keep_wfn(handle, on) result(rc) bind(C, name="xtbamd_batch_keep_wavefunction")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: on
+         integer(c_int) :: rc
+      end function engine_keep_wfn
       subroutine engine_free(handle) bind(C, name="xtbamd_batch_free")
          import :: c_ptr
          type(c_ptr), value :: handle
@@ -143,6 +156,15 @@ module xtbamd_capi
       real(wp), allocatable :: gradient(:, :)
       real(wp) :: sigma(3, 3) = 0.0_wp
       real(wp) :: dipole(3) = 0.0_wp
+      !> device-resident engine of the last single point and what it was built
+      !> for: while the atoms, charge, spin, calculator settings and solvent
+      !> stay the same, the next single point reuses its device arrays and
+      !> starts from the wavefunction it left on the device
+      type(c_ptr) :: engine = c_null_ptr
+      integer(c_int), allocatable :: eng_at(:)
+      real(wp) :: eng_chrg = 0.0_wp, eng_acc = 0.0_wp, eng_etemp = 0.0_wp, eng_stemp = 0.0_wp
+      integer :: eng_uhf = 0, eng_maxiter = 0, eng_kernel = 0, eng_alpb = 0, eng_state = 0
+      character(len=:), allocatable :: eng_solvent
    end type results_t
 
 contains
@@ -733,6 +755,7 @@ contains
       integer(c_int) :: natoms(1), uhf(1), rc, guess
       real(c_double) :: chrg(1)
       integer :: nsh, nao, n
+      logical :: reuse
       real(wp), allocatable, target :: qsh(:), dipm(:, :), qp(:, :), grad(:, :), q(:), wbo(:, :), emo(:), &
          & focc(:), coeff(:, :)
 
@@ -765,13 +788,26 @@ contains
       ! the cached wavefunction seeds the SCC unless the dimensions changed
       guess = 0
       if (res%has_wfn .and. res%n /= n) res%has_wfn = .false.
-      handle = engine_new(1_c_int, natoms, mol%at, mol%xyz, chrg, uhf, calc%accuracy, calc%etemp, &
-         & int(calc%maxiter, c_int), guess)
-      if (.not. c_associated(handle)) then
-         call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
-         return
+      reuse = engine_matches(res, mol, calc)
+      if (reuse) then
+         ! the results object's engine: new positions, device arrays kept
+         handle = res%engine
+         rc = engine_set_positions(handle, 0_c_int, mol%xyz)
+         if (rc /= 0) then
+            call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
+            call drop_engine(res)
+            return
+         end if
+      else
+         call drop_engine(res)
+         handle = engine_new(1_c_int, natoms, mol%at, mol%xyz, chrg, uhf, calc%accuracy, calc%etemp, &
+            & int(calc%maxiter, c_int), guess)
+         if (.not. c_associated(handle)) then
+            call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
+            return
+         end if
+         if (.not. apply_solvent(env, calc, handle)) return
       end if
-      if (.not. apply_solvent(env, calc, handle)) return
       ! sizes
       er%gradient = c_null_ptr; er%charges = c_null_ptr; er%qsh = c_null_ptr; er%dipm = c_null_ptr
       er%qp = c_null_ptr; er%wbo = c_null_ptr; er%emo = c_null_ptr; er%focc = c_null_ptr
@@ -780,13 +816,20 @@ contains
       nsh = er%nshells
       nao = er%nao
       if (res%has_wfn .and. (res%nsh /= nsh .or. res%nao /= nao)) res%has_wfn = .false.
-      if (res%has_wfn) then
+      if (reuse) then
+         ! the wavefunction of the last single point is still on the device
+         rc = engine_keep_wfn(handle, merge(1_c_int, 0_c_int, res%has_wfn))
+      else if (res%has_wfn) then
          rc = engine_set_guess(handle, 0_c_int, res%qsh, res%dipm, res%qp)
       end if
       rc = engine_run(handle)
       if (rc /= 0) then
          call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
-         call engine_free(handle)
+         if (reuse) then
+            call drop_engine(res)
+         else
+            call engine_free(handle)
+         end if
          return
       end if
       allocate(qsh(nsh), dipm(3, n), qp(6, n), grad(3, n), q(n), wbo(n, n), emo(nao), focc(nao), coeff(nao, nao))
@@ -794,11 +837,17 @@ contains
       er%qp = c_loc(qp); er%wbo = c_loc(wbo); er%emo = c_loc(emo); er%focc = c_loc(focc)
       er%coeff = c_loc(coeff); er%density = c_null_ptr
       rc = engine_get(handle, 0_c_int, er)
-      call engine_free(handle)
       if (rc /= 0) then
          call env_error(env, engine_message(), 'xtb_calculator_singlepoint')
+         if (reuse) then
+            call drop_engine(res)
+         else
+            call engine_free(handle)
+         end if
          return
       end if
+      ! the engine stays with the results object (freed by xtb_delResults)
+      if (.not. reuse) call keep_engine(res, handle, mol, calc)
 
       ! cache the wavefunction; coefficients are stored AO x MO (column = orbital)
       res%has_wfn = .true.
@@ -838,6 +887,60 @@ contains
          call env_error(env, "Self consistent charge iterator did not converge", 'scf')
       end if
    end subroutine singlepoint_api
+
+   !> true when the results object holds an engine built for the same atoms,
+   !> charge, spin, calculator settings and solvent
+   logical function engine_matches(res, mol, calc) result(ok)
+      type(results_t), intent(in) :: res
+      type(molecule_t), intent(in) :: mol
+      type(calculator_t), intent(in) :: calc
+      ok = c_associated(res%engine) .and. allocated(res%eng_at)
+      if (.not. ok) return
+      ok = size(res%eng_at) == mol%n
+      if (.not. ok) return
+      ok = all(res%eng_at == mol%at(1:mol%n)) .and. res%eng_chrg == mol%chrg .and. res%eng_uhf == mol%uhf &
+         & .and. res%eng_acc == calc%accuracy .and. res%eng_etemp == calc%etemp &
+         & .and. res%eng_maxiter == calc%maxiter
+      if (.not. ok) return
+      if (allocated(calc%solvent) .neqv. allocated(res%eng_solvent)) then
+         ok = .false.
+      else if (allocated(calc%solvent)) then
+         ok = calc%solvent == res%eng_solvent .and. calc%solv_kernel == res%eng_kernel &
+            & .and. calc%solv_alpb == res%eng_alpb .and. calc%solv_state == res%eng_state &
+            & .and. calc%solv_temp == res%eng_stemp
+      end if
+   end function engine_matches
+
+   !> hand an engine to the results object, with what it was built for
+   subroutine keep_engine(res, handle, mol, calc)
+      type(results_t), intent(inout) :: res
+      type(c_ptr), intent(in) :: handle
+      type(molecule_t), intent(in) :: mol
+      type(calculator_t), intent(in) :: calc
+      res%engine = handle
+      res%eng_at = mol%at(1:mol%n)
+      res%eng_chrg = mol%chrg
+      res%eng_uhf = mol%uhf
+      res%eng_acc = calc%accuracy
+      res%eng_etemp = calc%etemp
+      res%eng_maxiter = calc%maxiter
+      if (allocated(res%eng_solvent)) deallocate(res%eng_solvent)
+      if (allocated(calc%solvent)) then
+         res%eng_solvent = calc%solvent
+         res%eng_kernel = calc%solv_kernel
+         res%eng_alpb = calc%solv_alpb
+         res%eng_state = calc%solv_state
+         res%eng_stemp = calc%solv_temp
+      end if
+   end subroutine keep_engine
+
+   !> free the results object's engine, if any
+   subroutine drop_engine(res)
+      type(results_t), intent(inout) :: res
+      if (c_associated(res%engine)) call engine_free(res%engine)
+      res%engine = c_null_ptr
+      if (allocated(res%eng_at)) deallocate(res%eng_at)
+   end subroutine drop_engine
 
    !> message of an engine failure code (xtbamd_result%failed): the solver
    !> that gave up, in the reference's wording where it has one
@@ -1104,6 +1207,7 @@ contains
       type(results_t), pointer :: res
       if (c_associated(vres)) then
          call c_f_pointer(vres, res)
+         call drop_engine(res)
          deallocate(res)
          vres = c_null_ptr
       end if
@@ -1118,6 +1222,9 @@ contains
       call c_f_pointer(vold, old)
       allocate(res)
       res = old
+      ! the copy keeps the wavefunction on the host; the engine stays with the original
+      res%engine = c_null_ptr
+      if (allocated(res%eng_at)) deallocate(res%eng_at)
       vres = c_loc(res)
    end function copyResults_api
 

@@ -98,6 +98,24 @@ def traffic_table(path):
     return table
 
 
+def gpu_time_shares(path):
+    """kernel short name -> % of GPU time, read from a table written by
+    ``stats_table`` (instantiations of one kernel template summed)."""
+    shares = {}
+    if not os.path.exists(path):
+        return shares
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 5 and cells[0].startswith("`"):
+                k = kernel_short_name(cells[0].strip("`"))
+                try:
+                    shares[k] = shares.get(k, 0.0) + float(cells[4])
+                except ValueError:
+                    pass
+    return shares
+
+
 def stats_table(run_dir, top=24):
     """Markdown table of a ``rocprofv3 --kernel-trace --stats`` run (kernel_stats.csv)."""
     paths = glob.glob(os.path.join(run_dir, "**", "*kernel_stats.csv"), recursive=True)
