@@ -107,23 +107,46 @@ def serve(args, reqs, qps):
             "p99_ms": 1e3 * float(np.percentile(lat, 99)), "hbm_gb": args.serve_hbm_gb}
 
 
-def single_instances(xtb_amd):
+def _one(xtb_amd, z, x, mixed=False):
+    """Second of two runs of a one-molecule batch (the first allocates and
+    loads the kernels); mixed: the FP32-MFMA products with FP64 residual."""
+    if mixed:
+        os.environ["XTBAMD_MIXED"] = "1"
+    try:
+        b = xtb_amd.Batch([(z, x, 0.0, 0)])
+    finally:
+        os.environ.pop("XTBAMD_MIXED", None)
+    b.run()
+    t = time.perf_counter()
+    b.run()
+    dt = time.perf_counter() - t
+    r = b.result(0)
+    b.close()
+    return dt, r
+
+
+def single_instances(xtb_amd, c1_oracle=None):
     """One molecule per batch: the 32-water cluster (192 AOs), C64H130 (386
-    AOs) and the 700-water cluster (4200 AOs); the second of two runs is timed
-    (the first allocates and loads the kernels)."""
+    AOs; FP64 and mixed precision, errors against the FP64 oracle) and the
+    700-water cluster (4200 AOs)."""
     cases = [("water32", workloads.water_cluster(32, 0)), ("alkane_c64h130", workloads.alkane(64, 1)),
              ("water700", workloads.large_water_cluster(2024))]
     out = {}
     for name, (z, x) in cases:
-        b = xtb_amd.Batch([(z, x, 0.0, 0)])
-        b.run()
-        t = time.perf_counter()
-        b.run()
-        dt = time.perf_counter() - t
-        r = b.result(0)
+        dt, r = _one(xtb_amd, z, x)
         out[name] = {"atoms": int(len(z)), "aos": int(r["nao"]), "ms": 1e3 * dt, "single_points_per_s": 1.0 / dt,
                      "iterations": int(r["iterations"]), "energy": r["energy"], "converged": bool(r["converged"])}
-        b.close()
+        if name == "alkane_c64h130":
+            dtm, rm = _one(xtb_amd, z, x, mixed=True)
+            mix = {"precision": "Fock transforms and density products on FP32 matrix cores with FP64 residual; "
+                                "eigensolver, mixing and energies FP64",
+                   "ms": 1e3 * dtm, "energy": rm["energy"], "iterations": int(rm["iterations"]),
+                   "dE_vs_fp64_path": abs(rm["energy"] - r["energy"]),
+                   "max_dgrad_vs_fp64_path": float(np.max(np.abs(rm["gradient"] - r["gradient"])))}
+            if c1_oracle is not None:
+                mix["dE_vs_fp64_oracle"] = abs(rm["energy"] - c1_oracle)
+                out[name]["dE_vs_fp64_oracle"] = abs(r["energy"] - c1_oracle)
+            out[name]["mixed_precision"] = mix
     return out
 
 
@@ -158,7 +181,12 @@ def main():
     naos = np.array([workloads.ao_count(z) for z, _ in mols])
 
     cpu = None
+    c1_oracle = None
     if not args.no_cpu:
+        # the FP64 oracle energy of C64H130 (the precision comparison's reference)
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(1) as pool:
+            c1_oracle = pool.map(_cpu_worker, [workloads.alkane(64, 1)])[0][0]
         done, wall, ecpu, busy = cpu_baseline(mols, args.cpu_budget, args.cpu_procs)
         cpu = {"value": done / wall, "unit": "single-points/s", "cores": args.cpu_procs, "kind": "port",
                "single_thread_value": done / busy, "cpu_model": cpu_model(),
@@ -203,6 +231,7 @@ def main():
         b.run()
         times.append(time.perf_counter() - t)
     prof = {k: b.profile_result(k) for k in ("gemm", "jacobi", "apply")}
+    apply_span_ms = b.profile_span("apply")
     res = b.results()
     # every timed step is a complete single point of every molecule: the
     # results of the last step equal the plain path's and iterate from scratch
@@ -269,11 +298,18 @@ def main():
             continue
         sec = tms * 1e-3
         if k == "apply":
-            ach = work / sec / 1e9
+            # duration: the sampled launches' execution spans on the device
+            # wall clock (first wave's start to last wave's end, as a kernel
+            # trace measures it); the HIP-event time beside it also holds the
+            # stream's dispatch gap before each launch
+            span = apply_span_ms * 1e-3 if apply_span_ms > 0 else sec
+            ach = work / span / 1e9
             line = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                    "bytes_per_launch": work / n,
+                    "bytes_per_launch": work / n, "avg_us_events": 1e3 * tms / n,
+                    "duration": "device wall clock per sampled launch (kernel execution span)",
                     "work": "compulsory HBM bytes: each element of A and W a rotation touches read once and "
                             "written once (A through one of its mirrored tiles), counted on the device"}
+            tms = span * 1e3
         else:
             ach = work / sec / 1e12
             line = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -329,7 +365,7 @@ def main():
         for f in args.serve_loads:
             out["serving"][f"poisson_{f:g}x"] = serve(args, reqs, f * sat["value"])
     if not args.no_singles:
-        out["single_instances"] = single_instances(xtb_amd)
+        out["single_instances"] = single_instances(xtb_amd, c1_oracle)
     print(json.dumps(out), flush=True)
 
 

@@ -98,6 +98,12 @@ void xtbamd_batch_free(xtbamd_batch b);
  * rounds and scaled to all of them. */
 int xtbamd_batch_profile(xtbamd_batch b, int kernels);
 int xtbamd_batch_profile_result(xtbamd_batch b, int kernel, int* launches, double* total_ms, double* work);
+/* kernels 2 (pivots) and 3 (tile updates): the summed execution spans of the sampled
+ * launches (device wall clock from the start of the launch's first wave to the
+ * end of its last, what a kernel trace records as its duration), scaled to all
+ * launches; the HIP-event times of profile_result also hold the dispatch gaps
+ * in the stream */
+int xtbamd_batch_profile_span(xtbamd_batch b, int kernel, double* span_ms);
 /* ---- continuous batching (serving) ----
  * Requests are queued by xtbamd_server_submit and admitted into the running
  * batch whenever its packed device arrays (sized from hbm_gb for molecules of

@@ -21,14 +21,19 @@ def _mols():
             w.water_cluster(40, 10)]
 
 
-def _run(monkeypatch, blk_min):
+def _run(monkeypatch, blk_min, iters=None):
     import xtb_amd
     if xtb_amd.device_count() < 1:
         pytest.fail("no GPU visible to the HIP runtime")
+    monkeypatch.setenv("XTBAMD_TUNING", "off")
     if blk_min is None:
         monkeypatch.delenv("XTBAMD_EIG_BLK_MIN", raising=False)
     else:
         monkeypatch.setenv("XTBAMD_EIG_BLK_MIN", str(blk_min))
+    if iters is None:
+        monkeypatch.delenv("XTBAMD_EIG_BLK_ITERS", raising=False)
+    else:
+        monkeypatch.setenv("XTBAMD_EIG_BLK_ITERS", str(iters))
     mols = _mols()
     b = xtb_amd.Batch([(z, x, 0.0, 0) for z, x in mols]).run()
     res = b.results()
@@ -74,12 +79,14 @@ def test_eigh_blocked_against_lapack():
         assert dt < 1e-12 and dl < 1e-12 and res < 1e-12 and orth < 1e-12, k
 
 
-@pytest.mark.parametrize("blk_min", [65, 200])
-def test_blocked_eigensolver_matches_jacobi_and_oracle(monkeypatch, blk_min):
+@pytest.mark.parametrize("blk_min,iters", [(65, None), (200, None), (65, 2)])
+def test_blocked_eigensolver_matches_jacobi_and_oracle(monkeypatch, blk_min, iters):
+    """Every iteration on the blocked path (iters None), or only the first two
+    SCC iterations (the block-Jacobi lanes from the third on)."""
     mols, ref = _run(monkeypatch, None)
-    _, got = _run(monkeypatch, blk_min)
+    _, got = _run(monkeypatch, blk_min, iters)
     for k, (r, g) in enumerate(zip(ref, got)):
-        tag = f"blk_min={blk_min} mol{k} nao={g['nao']}"
+        tag = f"blk_min={blk_min} iters={iters} mol{k} nao={g['nao']}"
         dE = abs(r["energy"] - g["energy"])
         dg = float(np.max(np.abs(r["gradient"] - g["gradient"])))
         dq = float(np.max(np.abs(r["charges"] - g["charges"])))

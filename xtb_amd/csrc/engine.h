@@ -119,6 +119,7 @@ struct JacSet {
   std::vector<JacobiProb> probs;
   JacobiProb* d_probs = nullptr;
   int nsmall = 0, npairs = 0, nta = 0, ntw = 0, max_rounds = 0;
+  int max_rounds_lanes = 0;   // the same without the blocked eigensolver's problems
   int* d_small = nullptr;
   int2* d_pairs = nullptr;
   int4 *d_ta = nullptr, *d_tw = nullptr, *d_to = nullptr;
@@ -165,7 +166,8 @@ struct EigSet {
   double *Aw = nullptr, *Vh = nullptr, *Z0 = nullptr, *Z1 = nullptr, *U = nullptr;
   double *X = nullptr, *P = nullptr, *P2 = nullptr, *Tf = nullptr, *vec = nullptr;
   int* ivec = nullptr;
-  long long cap[18] = {};     // capacities (elements) of the arrays above and below
+  double* part2 = nullptr;    // partial sums of the tridiagonalisation's row kernel
+  long long cap[20] = {};     // capacities (elements) of the arrays above and below
   long long nmat = 0, nx = 0;   // elements in use of the n x ld arrays and of X
   // product launches: jobs [job0, job0 + njob), tiles [tile0, tile0 + ntile)
   struct Launch {
@@ -368,6 +370,11 @@ class Batch {
   // the block-Jacobi rounds; register path only.  Off unless the tuning file
   // or XTBAMD_EIG_BLK_MIN sets it.
   int eig_blk_min = 1 << 30;
+  // SCC iterations (from the first) whose eigenproblems above eig_blk_min go
+  // to the blocked eigensolver; later ones, with the previous orbitals as the
+  // start, to the block-Jacobi lanes
+  int eig_blk_iters = 1 << 30;
+  int jac_period_ = 1;   // rounds between the lanes' progress tests
   EigSet eig_;
   void eig_build();
   // the blocked eigensolver on eig_'s problems: A from Am (Jacobi problem
@@ -395,6 +402,11 @@ class Batch {
   bool apply_lds = false;  // 32x32 pivots: LDS tile updates with W as columns instead of register ones with W as rows
   int jac_p = 32;          // block-Jacobi pivot size (64: 32-wide blocks, 32: 16-wide blocks)
   bool use_mfma = true;   // FP64 matrix-core GEMM (else FP64 VALU FMA)
+  // mixed precision: the Fock transforms (H C, C^T H C) and the density
+  // products (C f C^T, P S) on the FP32 matrix cores with an FP64 residual
+  // correction (k_gemm_mix); the eigenvector update C W stays FP64
+  bool mixed = false;
+  bool mixed_here_ = false;   // set around the products of an SCC iteration that may run mixed
   double* Sm = nullptr;
   double* G = nullptr;
   int* oa_act = nullptr;
@@ -416,6 +428,10 @@ class Batch {
   // one of its mirrored tiles).  The Jacobi families are timed on a sample of
   // the rounds.
   static constexpr int kProfKinds = 4;   // 1 GEMM, 2 Jacobi pivots, 3 Jacobi tile updates
+  static constexpr int kProfTsSlots = 8192;   // sampled tile-update launches with device wall-clock spans
+  unsigned long long* prof_ts_ = nullptr;
+  int prof_ts_used_[2] = {0, 0};   // pivots, tile updates
+  double prof_span(int kind);
   int prof_mask = 0;
   std::vector<hipEvent_t> prof_ev_[kProfKinds];
   int prof_used_[kProfKinds] = {};

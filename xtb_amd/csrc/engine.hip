@@ -46,6 +46,13 @@ __global__ void k_fill_int(int* p, long long n, int v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_ts_init(unsigned long long* ts, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    ts[2 * i] = ~0ull;
+    ts[2 * i + 1] = 0ull;
+  }
+}
 __global__ void k_copy(const double* a, double* b, long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
@@ -649,7 +656,7 @@ static long long* cap_field(Capacity& c, int k) {
 // "off" none): "key value" lines, '#' comments.  Read once per process.
 namespace {
 struct Tuning {
-  int eig_blk_min = -1, jac_lanes = -1, jac_cross = -1, gemm_valu = -1;
+  int eig_blk_min = -1, eig_blk_iters = -1, jac_lanes = -1, jac_cross = -1, gemm_valu = -1;
 };
 const Tuning& tuning() {
   static const Tuning t = [] {
@@ -675,6 +682,7 @@ const Tuning& tuning() {
       if (line[0] == '#' || std::sscanf(line, "%63s %63s", key, val) != 2) continue;
       const std::string k(key), v(val);
       if (k == "eig_blk_min") r.eig_blk_min = std::atoi(val);
+      else if (k == "eig_blk_iters") r.eig_blk_iters = std::atoi(val);
       else if (k == "jac_lanes") r.jac_lanes = std::atoi(val);
       else if (k == "jac_cross") r.jac_cross = std::atoi(val);
       else if (k == "gemm") r.gemm_valu = v == "valu" ? 1 : 0;
@@ -690,10 +698,12 @@ void Batch::init_common() {
   // the autotuner's choices first; environment variables override them
   const Tuning& tu = tuning();
   if (tu.eig_blk_min > 0) eig_blk_min = std::max(65, tu.eig_blk_min);
+  if (tu.eig_blk_iters >= 0) eig_blk_iters = tu.eig_blk_iters;
   if (tu.jac_lanes > 0) jac_lanes = std::min(kMaxLanes, tu.jac_lanes);
   if (tu.jac_cross >= 0) jac_cross = tu.jac_cross != 0;
   if (tu.gemm_valu >= 0) use_mfma = tu.gemm_valu == 0;
   if (const char* g = std::getenv("XTBAMD_GEMM")) use_mfma = std::string(g) != "valu";
+  if (const char* g = std::getenv("XTBAMD_MIXED")) mixed = std::atoi(g) != 0;
   if (const char* r = std::getenv("XTBAMD_REFINE_STEPS")) oa_max_steps = std::atoi(r);
   if (const char* r = std::getenv("XTBAMD_INNER_SWEEPS")) inner_sweeps = std::max(1, std::atoi(r));
   if (const char* r = std::getenv("XTBAMD_PIVOT")) jac_p = std::atoi(r) == 64 ? 64 : 32;
@@ -710,6 +720,7 @@ void Batch::init_common() {
   trace_rounds = std::getenv("XTBAMD_TRACE_ROUNDS") != nullptr;
   if (const char* r = std::getenv("XTBAMD_JAC_LANES")) jac_lanes = std::min(kMaxLanes, std::max(1, std::atoi(r)));
   if (const char* r = std::getenv("XTBAMD_EIG_BLK_MIN")) eig_blk_min = std::max(65, std::atoi(r));
+  if (const char* r = std::getenv("XTBAMD_EIG_BLK_ITERS")) eig_blk_iters = std::max(0, std::atoi(r));
   if (on_loan()) {
     // the lender's streams and events (created and kept by the lender)
     stream = loan_.streams[0];
@@ -1008,6 +1019,7 @@ void Batch::rebuild() {
   // the blocked eigensolver takes the largest problems on the register path
   const bool blk_path = jac_padded && jac_p == 32 && !use_ql;
   js.max_rounds = 0;
+  js.max_rounds_lanes = 0;
   js.big_max_dim = 0;
   js.big_max_ld = 0;
   long long poff = 0;
@@ -1032,16 +1044,15 @@ void Batch::rebuild() {
       small.push_back(m);
       continue;
     }
-    if (blk_path && p.dim >= eig_blk_min) {
-      // the blocked eigensolver (no lane: group -1 is counted by none)
-      p.group = -1;
-      blk.push_back(m);
-      continue;
-    }
+    // the largest problems also go to the blocked eigensolver, which takes
+    // them for the first eig_blk_iters SCC iterations (the lanes after)
+    const bool to_blk = blk_path && p.dim >= eig_blk_min;
+    if (to_blk) blk.push_back(m);
     big.push_back(m);
     js.big_max_dim = std::max(js.big_max_dim, p.dim);
     qoff += (long long)(p.nbp / 2) * jac_p * jac_p;
     js.max_rounds = std::max(js.max_rounds, p.rounds);
+    if (!to_blk) js.max_rounds_lanes = std::max(js.max_rounds_lanes, p.rounds);
   }
   // lanes of equal tile-update bytes per round (dim^2): largest problems
   // first, each to the lane with least so far
@@ -1314,7 +1325,12 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   const bool lanes = rows && !pivot_lds && js.npairs > 0;
   // the largest problems go to the blocked eigensolver on stream3 (after the
   // small ones), beside the lanes
-  const bool blk = rows && !pivot_lds && eig_.np > 0;
+  const bool blk = rows && !pivot_lds && eig_.np > 0 && iterations_ < eig_blk_iters;
+  // the blocked path's problems are marked done for the lanes
+  if (blk)
+    hipLaunchKernelGGL(k_eig_claim, dim3(1), dim3(256), 0, stream, (const EigProb*)eig_.d_probs, eig_.np, js.state);
+  // the lanes test their progress once per period of their largest problem
+  jac_period_ = blk ? std::max(1, js.max_rounds_lanes) : js.max_rounds;
   const bool forked = lanes || blk;
   // the problems of dimension <= 64 (one workgroup each, k_jac_small) run
   // on stream3 beside the lanes
@@ -1366,7 +1382,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
         else
           hipLaunchKernelGGL(k_jac_pairsR<false>, dim3(js.npairs), dim3(64), 0, stream, js.d_probs, js.d_pairs, r,
                              (const double*)Am, js.qbuf, js.rot, (const double*)js.scale, js.state, inner_sweeps, jw,
-                             jac_tol, (unsigned long long*)nullptr, 0);
+                             jac_tol, (unsigned long long*)nullptr, 0, (unsigned long long*)nullptr);
         prof_end(2, nullptr, 0);
         if (rows) {
           hipLaunchKernelGGL(k_jac_apply_aR, dim3(js.nta), dim3(64), 0, stream, js.d_probs, js.d_ta, r, Am,
@@ -1445,7 +1461,7 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
     ptr = (T_*)mem.get((size_t)std::max(1LL, capv) * sizeof(T_));
   };
   std::vector<EigProb> probs(es.np);
-  long long mat = 0, vv = 0, iv = 0, xx = 0, pp = 0;
+  long long mat = 0, vv = 0, iv = 0, xx = 0, pp = 0, pp2 = 0, tt = 0;
   es.v_off.clear();
   es.ld.clear();
   es.maxn = 0;
@@ -1462,12 +1478,18 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
     es.ld.push_back(e.ld);
     e.i_off = iv;
     e.x_off = xx;
+    e.nchunk = std::max(1, (e.n - 1 + kBtChunk - 1) / kBtChunk);
     e.p_off = pp;
+    e.p_off2 = pp2;
+    e.t_off = tt;
+    pp2 += (long long)((e.ld + 255) / 256) * kTrdPart;
+    const int npan_e = (e.n - 1 + kEigNb - 1) / kEigNb;
+    tt += (long long)std::max(1, npan_e) * kEigNb * e.nchunk * kEigNb;
     mat += (long long)e.n * e.ld;
     vv += (long long)kEigVecs * e.ld;
     iv += (long long)kEigInts * e.ld;
     xx += 4LL * kEigNb * e.ld;
-    pp += (long long)e.n * kEigNb;
+    pp += (long long)e.n * kEigNb * e.nchunk;
     int depth = 0;
     while (((e.n + (1 << depth) - 1) >> depth) > kDcLeaf) ++depth;
     e.depth = depth;
@@ -1485,10 +1507,11 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
   grow(es.X, es.cap[7], xx);
   grow(es.P, es.cap[8], pp);
   grow(es.P2, es.cap[9], pp);
-  grow(es.Tf, es.cap[10], (long long)es.np * std::max(1, es.npanel) * kEigNb * kEigNb);
+  grow(es.Tf, es.cap[10], tt);
   grow(es.d_probs, es.cap[11], es.np);
   grow(es.act, es.cap[12], es.np);
   grow(es.zf, es.cap[13], es.np);
+  grow(es.part2, es.cap[18], pp2);
   es.nmat = mat;
   es.nx = xx;
   // divide and conquer trees: every leaf at the problem's depth
@@ -1587,10 +1610,16 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
       const double* Vp = es.Vh + e.m_off + (long long)j0 * ld + k0;
       double* P = es.P + e.p_off;
       double* P2 = es.P2 + e.p_off;
-      const double* T = es.Tf + ((long long)k * es.npanel + pi) * kEigNb * kEigNb;
-      b1.push_back(job(n, jb, n - k0, Zf + k0, ld, Vp, ld, P, kEigNb, 1.0, 0.0, act));        // P = Z V
-      b2.push_back(job(n, jb, jb, P, kEigNb, T, kEigNb, P2, kEigNb, 1.0, 0.0, act));          // P2 = P T^T
-      b3.push_back(job(n, n - k0, jb, P2, kEigNb, Vp, ld, Zf + k0, ld, -1.0, 1.0, act));      // Z -= P2 V^T
+      const int tw = e.nchunk * kEigNb;
+      const double* T = es.Tf + e.t_off + (long long)pi * kEigNb * tw;
+      // P = Z V in K chunks (column block c of P), P2 = [P_1 .. P_C] [T .. T]^T, Z -= P2 V^T
+      for (int c = 0; c < e.nchunk; ++c) {
+        const int ks = k0 + c * kBtChunk, kl = std::max(0, std::min(kBtChunk, n - ks));
+        b1.push_back(job(n, jb, kl, Zf + std::min(ks, n - 1), ld, Vp + std::min(ks - k0, n - 1 - k0), ld,
+                         P + c * kEigNb, tw, 1.0, 0.0, act));
+      }
+      b2.push_back(job(n, jb, tw, P, tw, T, tw, P2, kEigNb, 1.0, 0.0, act));
+      b3.push_back(job(n, n - k0, jb, P2, kEigNb, Vp, ld, Zf + k0, ld, -1.0, 1.0, act));
     }
     add(es.upd, u);
     add(es.bt1, b1);
@@ -1625,6 +1654,9 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
     HIPCHECK(hipMemcpy(es.d_nodes, nodes.data(), nodes.size() * sizeof(DcNode), hipMemcpyHostToDevice));
   if (!jobs.empty()) HIPCHECK(hipMemcpy(es.d_jobs, jobs.data(), jobs.size() * sizeof(GemmJob), hipMemcpyHostToDevice));
   if (!tiles.empty()) HIPCHECK(hipMemcpy(es.d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice));
+  // P's columns beyond a partial panel's width meet zero columns of T: keep
+  // them finite
+  HIPCHECK(hipMemset(es.P, 0, (size_t)std::max(1LL, pp) * sizeof(double)));
 }
 
 void Batch::eig_blocked(const JacSet& js, double* Am, double* Wm, bool check_active, hipStream_t s) {
@@ -1653,14 +1685,15 @@ void eig_set_run(EigSet& es, const JacobiProb* jprobs, int* jstate, const MolSta
     const int j0 = pi * kEigNb;
     const int j1 = std::min(j0 + kEigNb, nmax - 1);   // this panel's columns [j0, j1)
     hipLaunchKernelGGL(k_fill, dim3(grid1(es.nx, 256)), dim3(256), 0, s, es.X, es.nx, 0.0);
-    hipLaunchKernelGGL(k_trd_fin, dim3(np), dim3(1024), 0, s, (const EigProb*)es.d_probs, (const int*)es.act,
-                       (const double*)es.Aw, es.Vh, es.X, es.vec, -1, j0, j0);
+    hipLaunchKernelGGL(k_trd_row, dim3(grid1(nmax - j0, 256), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                       (const int*)es.act, (const double*)es.Aw, es.X, es.vec, es.part2, -1, j0, j0);
     for (int j = j0; j < j1; ++j) {
-      hipLaunchKernelGGL(k_trd_symv, dim3(grid1(nmax - 1 - j, 16), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
-                         (const int*)es.act, (const double*)es.Aw, (const double*)es.Vh, es.vec, j);
+      hipLaunchKernelGGL(k_trd_col, dim3(grid1(nmax - 1 - j, 4), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                         (const int*)es.act, (const double*)es.Aw, es.Vh, es.X, es.vec, (const double*)es.part2, j,
+                         j0);
       const int next = j + 1 < j0 + kEigNb ? j + 1 : -1;
-      hipLaunchKernelGGL(k_trd_fin, dim3(np), dim3(1024), 0, s, (const EigProb*)es.d_probs, (const int*)es.act,
-                         (const double*)es.Aw, es.Vh, es.X, es.vec, j, next, j0);
+      hipLaunchKernelGGL(k_trd_row, dim3(grid1(nmax - 1 - j, 256), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                         (const int*)es.act, (const double*)es.Aw, es.X, es.vec, es.part2, j, next, j0);
     }
     gemm_launch(es.upd[pi], true, false);
   }
@@ -1682,12 +1715,14 @@ void eig_set_run(EigSet& es, const JacobiProb* jprobs, int* jstate, const MolSta
     double* Zout = h & 1 ? es.Z1 : es.Z0;
     hipLaunchKernelGGL(k_dc_deflate, dim3(nl), dim3(256), 0, s, (const EigProb*)es.d_probs, (const int*)es.act, nd,
                        es.vec, es.ivec, Zin);
-    hipLaunchKernelGGL(k_dc_secular, dim3(grid1(mm, 64), nl), dim3(64), 0, s, (const EigProb*)es.d_probs,
+    hipLaunchKernelGGL(k_dc_secular, dim3(grid1(mm, 4), nl), dim3(256), 0, s, (const EigProb*)es.d_probs,
                        (const int*)es.act, nd, es.vec, es.ivec);
     hipLaunchKernelGGL(k_dc_zhat, dim3(grid1(mm, 64), nl), dim3(64), 0, s, (const EigProb*)es.d_probs,
                        (const int*)es.act, nd, es.vec, (const int*)es.ivec);
-    hipLaunchKernelGGL(k_dc_place, dim3(nl), dim3(256), 0, s, (const EigProb*)es.d_probs, (const int*)es.act, nd,
-                       es.vec, es.ivec, (const double*)Zin, Zout);
+    hipLaunchKernelGGL(k_dc_place, dim3(grid1(mm, 256), nl), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                       (const int*)es.act, nd, es.vec, es.ivec);
+    hipLaunchKernelGGL(k_dc_rows, dim3(mm, nl), dim3(256), 0, s, (const EigProb*)es.d_probs, (const int*)es.act, nd,
+                       (const double*)es.vec, es.ivec, (const double*)Zin, Zout);
     hipLaunchKernelGGL(k_dc_vectors, dim3(grid1(mm, 4), nl), dim3(256), 0, s, (const EigProb*)es.d_probs,
                        (const int*)es.act, nd, es.vec, (const int*)es.ivec, es.U);
     gemm_launch(es.dcg[h - 1], false, false);
@@ -1696,7 +1731,7 @@ void eig_set_run(EigSet& es, const JacobiProb* jprobs, int* jstate, const MolSta
   }
   // 3. back-transformation, last panel first
   hipLaunchKernelGGL(k_trd_tfactor, dim3(std::max(1, es.npanel), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
-                     (const int*)es.act, (const double*)es.Vh, (const double*)es.vec, es.Tf, std::max(1, es.npanel));
+                     (const int*)es.act, (const double*)es.Vh, (const double*)es.vec, es.Tf);
   for (int pi = es.npanel - 1; pi >= 0; --pi) {
     gemm_launch(es.bt1[pi], false, true);
     gemm_launch(es.bt2[pi], false, true);
@@ -1729,6 +1764,7 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       hipLaunchKernelGGL(k_jac_offmax, dim3(nto), dim3(256), 0, ls[g], js.d_probs, js.d_to + js.gto[g],
                          (const double*)Am, (const int*)js.state, js.offmax + nmol, 0);
   }
+  const int period = std::max(1, jac_period_);
   const int rmax = (jac_max_sweeps + 1) * js.max_rounds;
   for (int r = 0; r <= rmax; ++r) {
     // timing events (and the work counts) on every kProfEvery-th round only,
@@ -1740,17 +1776,20 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       if (!live[g]) continue;
       if (prof_mask & 4) prof_all_[2] += 1;
       if (sample) prof_begin(2, ls[g]);
+      unsigned long long* pts = nullptr;
+      if (sample && (prof_mask & 4) && prof_ts_used_[0] < kProfTsSlots)
+        pts = prof_ts_ + 2 * (prof_ts_used_[0]++);
       if (jac_cross)
         hipLaunchKernelGGL(k_jac_pairsR<true>, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
                            js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
-                           js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol);
+                           js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol, pts);
       else
         hipLaunchKernelGGL(k_jac_pairsR<false>, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
                            js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
-                           js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol);
+                           js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol, pts);
       if (sample) prof_end(2, nullptr, 0, ls[g]);
     }
-    if (r % js.max_rounds == 0) {
+    if (r % period == 0) {
       for (int g = 0; g < kMaxLanes; ++g)
         if (live[g])
           hipLaunchKernelGGL(k_jac_count, dim3(1), dim3(256), 0, ls[g], js.d_probs, nmol, (const int*)js.state, g,
@@ -1773,9 +1812,12 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       if (!live[g]) continue;
       if (prof_mask & 8) prof_all_[3] += 1;
       if (sample) prof_begin(3, ls[g]);
+      unsigned long long* ts = nullptr;
+      if (sample && (prof_mask & 8) && prof_ts_used_[1] < kProfTsSlots)
+        ts = prof_ts_ + 2 * (kProfTsSlots + prof_ts_used_[1]++);
       hipLaunchKernelGGL(k_jac_apply_R, dim3(js.gxt[g + 1] - js.gxt[g]), dim3(64), 0, ls[g], js.d_probs,
                          js.d_xt + js.gxt[g], r, Am, Wm, (const double*)js.qbuf, (const int*)js.rot,
-                         (const int*)js.state, js.offmax, nmol, tw);
+                         (const int*)js.state, js.offmax, nmol, tw, ts);
       if (sample) prof_end(3, nullptr, 0, ls[g]);
     }
     jacobi_rounds_ += 1;
@@ -1863,6 +1905,21 @@ void Batch::gemm(int kind, const double* a, const double* b, double* c, const do
     ms = (int)(sizeof(MolState) / sizeof(int));
   }
   prof_begin(1);
+  // the mixed-precision option: the Fock transforms and the density product
+  // (kinds 0-2) on the FP32 matrix cores with the FP64 residual correction
+  if (mixed && mixed_here_ && kind <= 2) {
+    if (kind == 0)
+      hipLaunchKernelGGL((k_gemm_mix<false, false, false>), dim3(nt), dim3(256), 0, stream, tiles, d_desc, a, b, c,
+                         scale_, alpha, m, ms);
+    else if (kind == 1)
+      hipLaunchKernelGGL((k_gemm_mix<true, false, false>), dim3(nt), dim3(256), 0, stream, tiles, d_desc, a, b, c,
+                         scale_, alpha, m, ms);
+    else
+      hipLaunchKernelGGL((k_gemm_mix<false, true, true>), dim3(nt), dim3(256), 0, stream, tiles, d_desc, a, b, c,
+                         scale_, alpha, m, ms);
+    prof_end(1, m, ms);
+    return;
+  }
 #define XTBAMD_GEMM(K, TA, TB, SC, PM)                                                                              \
   case K:                                                                                                          \
     if (use_mfma)                                                                                                  \
@@ -1896,6 +1953,9 @@ void Batch::prof_enable(int mask) {
     }
   }
   prof_round_ = 0;
+  if (!prof_ts_) prof_ts_ = alloc<unsigned long long>(4 * kProfTsSlots);
+  prof_ts_used_[0] = prof_ts_used_[1] = 0;
+  hipLaunchKernelGGL(k_ts_init, dim3(grid1(2 * kProfTsSlots, 256)), dim3(256), 0, stream, prof_ts_, 2 * kProfTsSlots);
   if (!prof_work_) prof_work_ = alloc<unsigned long long>(kProfKinds);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, (double*)prof_work_, (long long)kProfKinds, 0.0);
   HIPCHECK(hipStreamSynchronize(stream));
@@ -1919,6 +1979,30 @@ void Batch::prof_end(int kind, const int* mask, int mstride, hipStream_t s) {
   if (kind == 1)
     hipLaunchKernelGGL(k_count_gemm, dim3(grid1(nmol, 256)), dim3(256), 0, stream, d_desc, nmol, mask, mstride,
                        prof_work_ + 1);
+}
+
+double Batch::prof_span(int kind) {
+  // the tile updates' sampled execution spans (device wall clock), scaled
+  // like the event times to all launches of the family
+  if ((kind != 2 && kind != 3) || !prof_ts_) return 0.0;
+  const int fam = kind - 2, nused = prof_ts_used_[fam];
+  if (nused == 0) return 0.0;
+  HIPCHECK(hipDeviceSynchronize());
+  std::vector<unsigned long long> h(2 * (size_t)nused);
+  HIPCHECK(hipMemcpy(h.data(), prof_ts_ + 2 * (size_t)fam * kProfTsSlots, h.size() * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+  int khz = 0;
+  HIPCHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
+  double ticks = 0.0;
+  int used = 0;
+  for (int i = 0; i < nused; ++i)
+    if (h[2 * i + 1] >= h[2 * i] && h[2 * i] != ~0ull) {
+      ticks += (double)(h[2 * i + 1] - h[2 * i]);
+      ++used;
+    }
+  if (used == 0 || khz <= 0) return 0.0;
+  const double ms = ticks / (double)khz;   // ticks / (kHz) = ms
+  return ms * (double)prof_all_[kind] / used;
 }
 
 int Batch::prof_result(int kind, int* launches, double* total_ms, double* work) {
@@ -2164,8 +2248,10 @@ bool Batch::scc_iteration() {
   hipLaunchKernelGGL(k_fock, dim3(gstride((long long)max_nao * max_nao), nmol), dim3(256), 0, stream, d_desc, S, H0,
                      D, Q, shift, vs, vd, vq, ao_sh, ao_at, thr2, H, (const MolState*)d_state);
   // generalized eigenproblem in the current orbital basis C (S-orthonormal)
+  mixed_here_ = true;   // the Fock transforms and the density product may run mixed
   gemm(0, H, C, T, nullptr, nullptr, true);   // T = H C
   gemm(1, C, T, A, nullptr, nullptr, true, 1.0, nullptr, true);   // A = C^T H C (symmetric)
+  mixed_here_ = false;
   // C holds the previous eigenvectors from the second iteration on: A is
   // nearly diagonal, refine W = I towards its eigenvectors with GEMMs first
   const bool warm = iterations_ > 0 && oa_max_steps > 0;
@@ -2197,7 +2283,9 @@ bool Batch::scc_iteration() {
                      d_desc, (const MolState*)d_state, (const double*)T, C);
   hipLaunchKernelGGL(k_occ_fermi, dim3(nmol), dim3(256), 0, stream, d_desc, d_state, (const double*)emo, focca, foccb,
                      focc, opt.etemp, (const double*)qsh, (const double*)dipm, (const double*)qp, q_in);
+  mixed_here_ = true;
   gemm(2, C, C, P, focc, nullptr, true, 1.0, nullptr, true);      // P = C diag(f) C^T (symmetric)
+  mixed_here_ = false;
   hipLaunchKernelGGL(k_population, dim3(grid1(max_nao, 4), nmol), dim3(256), 0, stream, d_desc, (const double*)P,
                      (const double*)S, (const double*)H0, (const double*)D, (const double*)Q, row_q, row_e, row_d,
                      row_qp, (const MolState*)d_state, 0);
@@ -2707,6 +2795,14 @@ int xtbamd_batch_profile(xtbamd_batch b, int kernels) {
   try {
     if (!b || kernels < 0 || kernels > 7) throw std::runtime_error("unknown kernel family");
     b->batch->prof_enable(kernels << 1);
+    return 0;
+  } catch (const std::exception& e) { return fail(e); }
+}
+
+int xtbamd_batch_profile_span(xtbamd_batch b, int kernel, double* span_ms) {
+  try {
+    if (!b || !span_ms) throw std::runtime_error("invalid batch");
+    *span_ms = b->batch->prof_span(kernel);
     return 0;
   } catch (const std::exception& e) { return fail(e); }
 }

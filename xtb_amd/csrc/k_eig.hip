@@ -38,6 +38,7 @@ constexpr int kVd = 0, kVe = 1, kVtau = 2, kVw = 3, kVlam = 4, kVlam2 = 5, kVdn 
 // slots of the integer workspace (ints, length ld each)
 constexpr int kIperm = 0, kIkeep = 1, kIpos = 2, kIorg = 3, kIrot = 4, kIrot2 = 5, kIrow = 6, kIcnt = 7;
 constexpr int kEigInts = 8;
+constexpr int kBtChunk = 512;   // K chunk of the back-transformation's product Z V
 
 // One batched product job: C = beta C + alpha op(A) op(B), op(A) M x K,
 // op(B) K x N, row-major with the given row strides.  TA: A is stored K x M;
@@ -66,7 +67,11 @@ struct EigProb {
   long long v_off;   // this problem's kEigVecs x ld vector block
   long long i_off;   // this problem's kEigInts x ld integer block
   long long x_off;   // this problem's panel buffers X, Y (2 kEigNb x ld each)
-  long long p_off;   // this problem's n x kEigNb product buffers of the back-transformation
+  long long p_off;   // this problem's product buffers of the back-transformation
+  long long p_off2;  // this problem's k_trd_row partial sums (ceil(n / 256) x kTrdPart)
+  long long t_off;   // this problem's T factors (npanel x kEigNb x nchunk kEigNb: T repeated nchunk times)
+  int nchunk;        // K chunks of the back-transformation's first product
+  int pad2_;
   int mol;           // batch slot (index of its Jacobi problem)
   int depth;         // levels of the divide and conquer tree (leaves at this depth)
 };
@@ -158,152 +163,203 @@ __global__ void __launch_bounds__(256) k_bgemm(const GemmJob* __restrict__ jobs,
 // ---------------------------------------------------------------------------
 // 1. tridiagonalisation
 
-// w = A22 v for column j: rows j+1..n-1, four rows per wave, 16 per workgroup
-__global__ void __launch_bounds__(256) k_trd_symv(const EigProb* __restrict__ probs, const int* __restrict__ act,
-                                                  const double* __restrict__ A, const double* __restrict__ Vh,
-                                                  double* vec, int j) {
-  if (!act[blockIdx.y]) return;
-  const EigProb p = probs[blockIdx.y];
-  const int n = p.n, ld = p.ld;
-  if (j >= n - 1) return;
-  const int lane = threadIdx.x & 63;
-  const int r0 = j + 1 + blockIdx.x * 16 + (threadIdx.x >> 6) * 4;
-  if (r0 >= n) return;
-  const double* Ap = A + p.m_off;
-  const double* v = Vh + p.m_off + (long long)j * ld;
-  double* w = vec + p.v_off + (long long)kVw * ld;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  const int nr = min(4, n - r0);
-  for (int c = j + 1 + lane; c < n; c += 64) {
-    const double vc = v[c];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (q < nr) s[q] = fma(Ap[(long long)(r0 + q) * ld + c], vc, s[q]);
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const double t = wave_sum(s[q]);
-    if (lane == 0 && q < nr) w[r0 + q] = t;
-  }
-}
-
-// Deterministic sum over a 1024-thread block (16 waves); sh holds 16 slots.
-__device__ __forceinline__ double sum1024(double v, double* sh) {
-  v = wave_sum(v);
-  const int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) sh[wv] = v;
-  __syncthreads();
-  double r = 0.0;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) r += sh[i];
-  __syncthreads();
-  return r;
-}
-
-// One workgroup (1024 threads) per problem.  finish >= 0: complete
-// w = tau (A22 v - W V^T v - V W^T v) + alpha v for column finish (its panel
-// index i = finish - j0) and store it in the panel buffers.  next >= 0: form
-// the reflector of column next from row next of A corrected by the panel's
-// first (next - j0) vector pairs: d[next], e[next], tau[next], v (row next of
-// Vh and of the panel buffers); at next = n - 1 only d[next].
+// Two wide launches per column j (panel start j0, i = j - j0 earlier
+// columns in the panel).  k_trd_row has left the corrected row a = A(j, j:n)
+// of column j (with the panel's earlier vector pairs applied) and, per
+// workgroup, the partial sums that column j's reflector needs: the squared
+// norm of a(j+2:n) and the products of a(j+2:n) with the panel's v and w.
+// k_trd_col forms the reflector from them (every workgroup for itself),
+// w = A22 v over the trailing rows (one wave per row, coalesced row reads,
+// v formed on the fly from a), the panel corrections and the scaling by tau,
+// and the partial sums of w . v.  The next k_trd_row completes
+// w += alpha v, stores the pair in the panel buffers and forms the next
+// column's corrected row and partial sums.
 // X rows [0, nb): v of the panel's columns, rows [nb, 2nb): their w; Y the
 // other way round, so that the trailing update is A22 -= X^T Y.
-__global__ void __launch_bounds__(1024) k_trd_fin(const EigProb* __restrict__ probs, const int* __restrict__ act,
-                                                  const double* __restrict__ A, double* Vh, double* X, double* vec,
-                                                  int finish, int next, int j0) {
-  if (!act[blockIdx.x]) return;
-  __shared__ double sh[16];
-  __shared__ double y[2 * kEigNb];
-  __shared__ double cw[2 * kEigNb];
-  __shared__ double bc[4];
-  const EigProb p = probs[blockIdx.x];
-  const int n = p.n, ld = p.ld, tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
-  double* Xp = X + p.x_off;                              // X: 2 nb rows
-  double* Yp = X + p.x_off + (long long)2 * kEigNb * ld;   // Y: 2 nb rows
-  double* d = vec + p.v_off + (long long)kVd * ld;
-  double* e = vec + p.v_off + (long long)kVe * ld;
-  double* tau = vec + p.v_off + (long long)kVtau * ld;
-  double* w = vec + p.v_off + (long long)kVw * ld;
-  if (finish >= 0 && finish < n - 1) {
-    const int j = finish, i = j - j0;
-    const double* v = Vh + p.m_off + (long long)j * ld;
-    // y1 = V^T v (t < i), y2 = W^T v over rows j+1..n-1: one wave per dot product
-    for (int t = wv; t < 2 * i; t += 16) {
-      const double* row = t < i ? Xp + (long long)t * ld : Xp + (long long)(kEigNb + t - i) * ld;
-      double s = 0.0;
-      for (int r = j + 1 + lane; r < n; r += 64) s = fma(row[r], v[r], s);
-      s = wave_sum(s);
-      if (lane == 0) y[t] = s;
-    }
-    __syncthreads();
-    const double tj = tau[j];
-    double dot = 0.0;
-    for (int r = j + 1 + tid; r < n; r += 1024) {
-      double x = w[r];
-      for (int t = 0; t < i; ++t)
-        x -= Xp[(long long)(kEigNb + t) * ld + r] * y[t] + Xp[(long long)t * ld + r] * y[i + t];
-      x *= tj;
-      w[r] = x;
-      dot = fma(x, v[r], dot);
-    }
-    dot = sum1024(dot, sh);
-    const double al = -0.5 * tj * dot;
-    for (int r = j + 1 + tid; r < n; r += 1024) {
-      const double x = fma(al, v[r], w[r]);
-      Xp[(long long)(kEigNb + i) * ld + r] = x;
-      Yp[(long long)i * ld + r] = x;
-    }
-    __syncthreads();
+constexpr int kTrdPart = 1 + 2 * kEigNb;   // partial sums per k_trd_row workgroup
+
+__device__ __forceinline__ double sum256(double v, double* sh) { return block_sum<256>(v, sh); }
+
+// column j's reflector from the corrected row a and k_trd_row's partials:
+// beta, tau, scale (v = scale a beyond j + 1, v[j+1] = 1) and the panel
+// products y1 = V^T v, y2 = W^T v (t < i) into LDS
+__device__ __forceinline__ void trd_reflector(const double* __restrict__ a, const double* __restrict__ part,
+                                              const double* __restrict__ Xp, int ld, int n, int j, int i,
+                                              double& beta, double& tau, double& scale, double* y) {
+  const int nw = (n - j + 255) / 256;   // k_trd_row workgroups over rows j..n-1
+  double ss = 0.0;
+  for (int w = 0; w < nw; ++w) ss += part[w * kTrdPart];
+  const double alpha = a[j + 1];
+  beta = alpha;
+  tau = 0.0;
+  scale = 0.0;
+  if (ss > 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+    tau = (beta - alpha) / beta;
+    scale = 1.0 / (alpha - beta);
   }
-  if (next >= 0 && next < n) {
-    const int jn = next, i = jn - j0;
-    // coefficients of the corrections: v_t[jn] and w_t[jn] for t < i
-    if (tid < i) {
-      cw[tid] = Xp[(long long)tid * ld + jn];
-      cw[kEigNb + tid] = Xp[(long long)(kEigNb + tid) * ld + jn];
-    }
-    __syncthreads();
-    const double* arow = A + p.m_off + (long long)jn * ld;
-    double* v = Vh + p.m_off + (long long)jn * ld;
-    double* a = vec + p.v_off + (long long)kVy * ld;   // the corrected row
-    double ss = 0.0;
-    for (int r = jn + tid; r < n; r += 1024) {
-      double x = arow[r];
-      for (int t = 0; t < i; ++t)
-        x -= cw[t] * Xp[(long long)(kEigNb + t) * ld + r] + cw[kEigNb + t] * Xp[(long long)t * ld + r];
-      a[r] = x;
-      if (r >= jn + 2) ss = fma(x, x, ss);
-    }
-    ss = sum1024(ss, sh);   // includes the barrier that publishes a[]
+  for (int t = threadIdx.x; t < 2 * i; t += blockDim.x) {
+    const int tt = t < i ? t : t - i;
+    const int row = t < i ? tt : kEigNb + tt;   // v rows, then w rows
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += part[w * kTrdPart + 1 + (t < i ? 0 : kEigNb) + tt];
+    y[t] = Xp[(long long)row * ld + j + 1] + scale * s;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_trd_col(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                 const double* __restrict__ A, double* Vh, double* X, double* vec,
+                                                 const double* __restrict__ part2, int j, int j0) {
+  __shared__ double y[2 * kEigNb];
+  __shared__ double sh[4];
+  if (!act[blockIdx.y]) return;
+  const EigProb p = probs[blockIdx.y];
+  const int n = p.n, ld = p.ld, i = j - j0;
+  if (j >= n - 1) return;
+  const int nrows = n - 1 - j;
+  if ((int)blockIdx.x * 4 >= nrows) return;
+  double* V = vec + p.v_off;
+  const double* a = V + (long long)kVy * ld;
+  double* Xp = X + p.x_off;
+  double* Yp = Xp + (long long)2 * kEigNb * ld;
+  double beta, tau, scale;
+  trd_reflector(a, part2 + p.p_off2, Xp, ld, n, j, i, beta, tau, scale, y);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (blockIdx.x == 0) {
+    // publish column j: d, e, tau and v (Vh row j, panel buffers)
     if (tid == 0) {
-      d[jn] = a[jn];
-      if (jn < n - 1) {
-        const double alpha = a[jn + 1];
-        double t = 0.0, beta = alpha, scale = 0.0;
-        if (ss > 0.0) {
-          beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-          t = (beta - alpha) / beta;
-          scale = 1.0 / (alpha - beta);
-        }
-        e[jn] = beta;
-        tau[jn] = t;
-        bc[0] = scale;
-      }
+      V[(long long)kVd * ld + j] = a[j];
+      V[(long long)kVe * ld + j] = beta;
+      V[(long long)kVtau * ld + j] = tau;
     }
-    __syncthreads();
-    if (jn < n - 1) {
-      // v: zero up to jn, 1 at jn + 1, the scaled row beyond
-      const double scale = bc[0];
-      for (int r = tid; r < n; r += 1024) {
-        const double x = r <= jn ? 0.0 : (r == jn + 1 ? 1.0 : a[r] * scale);
-        v[r] = x;
-        if (i < kEigNb) {
-          Xp[(long long)i * ld + r] = x;
-          Yp[(long long)(kEigNb + i) * ld + r] = x;
-        }
-      }
+    double* vr = Vh + p.m_off + (long long)j * ld;
+    for (int r = tid; r < n; r += 256) {
+      const double x = r <= j ? 0.0 : (r == j + 1 ? 1.0 : a[r] * scale);
+      vr[r] = x;
+      Xp[(long long)i * ld + r] = x;
+      Yp[(long long)(kEigNb + i) * ld + r] = x;
     }
+  }
+  // w = tau (A22 v - W y1 - V y2) for this wave's row: the row product with
+  // four independent partial sums per lane, the 2i panel corrections one per
+  // lane, both reduced across the wave
+  const int r = j + 1 + blockIdx.x * 4 + wv;
+  double wvdot = 0.0;
+  if (r < n) {
+    const double* Ar = A + p.m_off + (long long)r * ld;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = j + 2 + lane;
+    for (; c + 192 < n; c += 256) {
+      s0 = fma(Ar[c], a[c], s0);
+      s1 = fma(Ar[c + 64], a[c + 64], s1);
+      s2 = fma(Ar[c + 128], a[c + 128], s2);
+      s3 = fma(Ar[c + 192], a[c + 192], s3);
+    }
+    for (; c < n; c += 64) s0 = fma(Ar[c], a[c], s0);
+    double s = scale * ((s0 + s1) + (s2 + s3));
+    if (lane == 0) s += Ar[j + 1];   // v[j+1] = 1
+    // corrections: lane t < i takes W(r, t) y1[t], lane i + t takes V(r, t) y2[t]
+    if (lane < 2 * i) {
+      const int t = lane < i ? lane : lane - i;
+      const double xv = lane < i ? Xp[(long long)(kEigNb + t) * ld + r] : Xp[(long long)t * ld + r];
+      s -= xv * y[lane];
+    }
+    s = tau * wave_sum(s);
+    if (lane == 0) {
+      V[(long long)kVw * ld + r] = s;
+      const double vr = r == j + 1 ? 1.0 : a[r] * scale;
+      wvdot = s * vr;
+    }
+  }
+  wvdot = sum256(wvdot, sh);
+  if (tid == 0) V[(long long)kVrc * ld + blockIdx.x] = wvdot;
+}
+
+// finish >= 0: w of column finish gets alpha v (alpha = -tau (w . v) / 2 from
+// k_trd_col's partials) and goes to the panel buffers.  next >= 0: the
+// corrected row a = A(next, next:n) minus the panel's first next - j0 vector
+// pairs, and this workgroup's partial sums for next's reflector; at
+// next = n - 1 only the last diagonal entry.  One thread per row.
+__global__ void __launch_bounds__(256) k_trd_row(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                 const double* __restrict__ A, double* X, double* vec, double* part2,
+                                                 int finish, int next, int j0) {
+  __shared__ double sh[4];
+  __shared__ double cw[2 * kEigNb + 2];
+  __shared__ double ra[256];   // this workgroup's rows of the corrected row a (0 outside the tail)
+  __shared__ double rw[256];   // and of the just finished w (t = i)
+  if (!act[blockIdx.y]) return;
+  const EigProb p = probs[blockIdx.y];
+  const int n = p.n, ld = p.ld, tid = threadIdx.x;
+  const int first = next >= 0 ? next : finish + 1;   // rows [first, n)
+  if (first >= n || (int)blockIdx.x * 256 >= n - first) return;
+  double* V = vec + p.v_off;
+  double* Xp = X + p.x_off;
+  double* Yp = Xp + (long long)2 * kEigNb * ld;
+  const int r0 = first + blockIdx.x * 256;
+  const int r = r0 + tid;
+  double alpha = 0.0, wi = 0.0;
+  int i = -1;
+  if (finish >= 0 && finish < n - 1) {
+    const int j = finish;
+    i = j - j0;
+    const int nw = (n - 1 - j + 3) / 4;   // k_trd_col workgroups
+    double s = 0.0;
+    for (int w = tid; w < nw; w += 256) s += V[(long long)kVrc * ld + w];
+    s = sum256(s, sh);
+    alpha = -0.5 * V[(long long)kVtau * ld + j] * s;
+    if (r < n && r > j) {
+      wi = fma(alpha, Xp[(long long)i * ld + r], V[(long long)kVw * ld + r]);
+      Xp[(long long)(kEigNb + i) * ld + r] = wi;
+      Yp[(long long)i * ld + r] = wi;
+    }
+  }
+  if (next < 0 || next >= n) return;
+  const int jn = next, ni = jn - j0;   // pairs 0..ni-1 correct the row (ni = i + 1 after a finish)
+  if (tid < ni) {
+    cw[tid] = Xp[(long long)tid * ld + jn];   // v_t[jn]
+    // w_t[jn]; the pair just finished is completed here, not yet visible in X
+    cw[kEigNb + tid] = tid == i ? fma(alpha, Xp[(long long)tid * ld + jn], V[(long long)kVw * ld + jn])
+                                : Xp[(long long)(kEigNb + tid) * ld + jn];
+  }
+  __syncthreads();
+  double ar = 0.0;
+  if (r < n) {
+    ar = A[p.m_off + (long long)jn * ld + r];
+    for (int t = 0; t < ni; ++t) {
+      const double vt = Xp[(long long)t * ld + r];
+      const double wt = t == i ? wi : Xp[(long long)(kEigNb + t) * ld + r];
+      ar -= cw[t] * wt + cw[kEigNb + t] * vt;
+    }
+    V[(long long)kVy * ld + r] = ar;
+    if (jn == n - 1) V[(long long)kVd * ld + jn] = ar;
+  }
+  // partial sums over rows jn + 2..n-1: |a|^2, v_t . a, w_t . a (t < ni);
+  // one wave per product, lanes over this workgroup's 256 rows
+  const bool tail = r < n && r >= jn + 2;
+  ra[tid] = tail ? ar : 0.0;
+  rw[tid] = wi;
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6;
+  const int rows = min(256, n - r0);
+  double* pr = part2 + p.p_off2 + (long long)blockIdx.x * kTrdPart;
+  for (int k = wv; k < kTrdPart; k += 4) {
+    const bool used = k == 0 || (k >= 1 && k < 1 + ni) || (k >= 1 + kEigNb && k < 1 + kEigNb + ni);
+    if (!used) {
+      if (lane == 0) pr[k] = 0.0;
+      continue;
+    }
+    double x = 0.0;
+    for (int u = lane; u < rows; u += 64) {
+      const double au = ra[u];
+      double o;
+      if (k == 0) o = au;
+      else if (k < 1 + kEigNb) o = Xp[(long long)(k - 1) * ld + r0 + u];
+      else o = (k - 1 - kEigNb) == i ? rw[u] : Xp[(long long)(k - 1) * ld + r0 + u];
+      x = fma(o, au, x);
+    }
+    x = wave_sum(x);
+    if (lane == 0) pr[k] = x;
   }
 }
 
@@ -530,8 +586,8 @@ __global__ void __launch_bounds__(256) k_dc_deflate(const EigProb* __restrict__ 
 // its pole of origin (the nearer end of the interval) and the offset tau from
 // it, so that every difference dd_i - lambda is formed without cancellation.
 // Safeguarded Newton steps inside a shrinking bracket.
-__global__ void k_dc_secular(const EigProb* __restrict__ probs, const int* __restrict__ act,
-                             const DcNode* __restrict__ nodes, double* vec, int* ivec) {
+__global__ void __launch_bounds__(256) k_dc_secular(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                    const DcNode* __restrict__ nodes, double* vec, int* ivec) {
   const DcNode nd = nodes[blockIdx.y];
   if (!act[nd.prob]) return;
   const EigProb p = probs[nd.prob];
@@ -539,7 +595,8 @@ __global__ void k_dc_secular(const EigProb* __restrict__ probs, const int* __res
   double* V = vec + p.v_off;
   int* I = ivec + p.i_off;
   const int k = I[kIcnt * ld + s0];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  // one wave per root, the lanes share the sums over the poles
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (t >= k) return;
   const double* dd = V + kVdd * ld + s0;
   const double* zz = V + kVzz * ld + s0;
@@ -549,27 +606,30 @@ __global__ void k_dc_secular(const EigProb* __restrict__ probs, const int* __res
   if (t < k - 1) {
     const double gap = dd[t + 1] - dd[t];
     const double mid = 0.5 * gap;
-    double f = 1.0;
-    for (int i = 0; i < k; ++i) f += bz * zz[i] * zz[i] / ((dd[i] - dd[t]) - mid);
+    double f = 0.0;
+    for (int i = lane; i < k; i += 64) f += bz * zz[i] * zz[i] / ((dd[i] - dd[t]) - mid);
+    f = 1.0 + wave_sum(f);
     if (f >= 0.0) { org = t; a = 0.0; b = mid; }
     else { org = t + 1; a = -mid; b = 0.0; }
   } else {
     org = k - 1;
     a = 0.0;
     double s = 0.0;
-    for (int i = 0; i < k; ++i) s += zz[i] * zz[i];
-    b = bz * s;
+    for (int i = lane; i < k; i += 64) s += zz[i] * zz[i];
+    b = bz * wave_sum(s);
   }
   const double dorg = dd[org];
   double x = 0.5 * (a + b);
   for (int it = 0; it < 200; ++it) {
-    double f = 1.0, fp = 0.0;
-    for (int i = 0; i < k; ++i) {
+    double f = 0.0, fp = 0.0;
+    for (int i = lane; i < k; i += 64) {
       const double del = (dd[i] - dorg) - x;
       const double q = zz[i] / del;
       f = fma(bz * zz[i], q, f);
       fp = fma(bz * q, q, fp);
     }
+    f = 1.0 + wave_sum(f);
+    fp = wave_sum(fp);
     if (f == 0.0) { a = b = x; break; }
     if (f > 0.0) b = x; else a = x;
     // Newton step (f is increasing in lambda), kept inside the bracket
@@ -581,8 +641,10 @@ __global__ void k_dc_secular(const EigProb* __restrict__ probs, const int* __res
     }
     x = xn;
   }
-  I[kIorg * ld + s0 + t] = org;
-  V[kVtr * ld + s0 + t] = x;
+  if (lane == 0) {
+    I[kIorg * ld + s0 + t] = org;
+    V[kVtr * ld + s0 + t] = x;
+  }
 }
 
 // recomputed z (Gu and Eisenstat): zh_i^2 = prod_j (lambda_j - d_i) /
@@ -613,14 +675,13 @@ __global__ void k_dc_zhat(const EigProb* __restrict__ probs, const int* __restri
   V[kVzh * ld + s0 + i] = copysign(sqrt(fmax(val, 0.0)), zz[i]);
 }
 
-// merge, part 2 (one workgroup per node): every eigenvalue of the node
-// (roots and deflated poles) gets its ascending position; the product's row
-// gather (child rows of the non-deflated entries) and row scatter (output
-// rows of the roots), the copies of the deflated rows into the other buffer
+// merge, part 2: every eigenvalue of the node (roots and deflated poles) gets
+// its ascending position (one thread per entry, grid over the node); the
+// product's row gather (child rows of the non-deflated entries) and row
+// scatter (output rows of the roots)
 __global__ void __launch_bounds__(256) k_dc_place(const EigProb* __restrict__ probs, const int* __restrict__ act,
-                                                  const DcNode* __restrict__ nodes, double* vec, int* ivec,
-                                                  const double* __restrict__ Zin, double* Zout) {
-  const DcNode nd = nodes[blockIdx.x];
+                                                  const DcNode* __restrict__ nodes, double* vec, int* ivec) {
+  const DcNode nd = nodes[blockIdx.y];
   if (!act[nd.prob]) return;
   const EigProb p = probs[nd.prob];
   const int ld = p.ld, s0 = nd.start, m = nd.n1 + nd.n2;
@@ -634,40 +695,60 @@ __global__ void __launch_bounds__(256) k_dc_place(const EigProb* __restrict__ pr
   const int* org = I + kIorg * ld + s0;
   const int* perm = I + kIperm * ld + s0;
   const int* keep = I + kIkeep * ld + s0;
+  int* pos = I + kIpos * ld + s0;
   double* lam2 = V + kVlam2 * ld;
-  int* pos = I + kIpos * ld + s0;    // by sorted index r: output position
-  int* row = I + kIrow * ld + s0;    // by root t: child row of keep[t] (product gather)
-  // eigenvalue of sorted index r: a root for kept entries, the pole otherwise;
-  // kept entries are marked by z != 0
-  double* ev = V + kVev * ld + s0;
-  for (int t = threadIdx.x; t < k; t += blockDim.x) ev[keep[t]] = dd[org[t]] + tr[t];
-  __syncthreads();
-  for (int r = threadIdx.x; r < m; r += blockDim.x)
-    if (z[r] == 0.0) ev[r] = dn[r];
-  __syncthreads();
-  for (int r = threadIdx.x; r < m; r += blockDim.x) {
-    const double x = ev[r];
-    int c = 0;
-    for (int u = 0; u < m; ++u) {
-      const double y = ev[u];
-      c += (y < x || (y == x && u < r)) ? 1 : 0;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  // eigenvalue of sorted index u: a root for kept entries (z != 0), the pole otherwise;
+  // the root of a kept entry: keep[t] == u, found by its position in keep
+  auto ev = [&](int u) -> double {
+    if (z[u] == 0.0) return dn[u];
+    int lo = 0, hi = k - 1;   // keep is ascending
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (keep[mid] < u) lo = mid + 1; else hi = mid;
     }
-    pos[r] = c;
-    lam2[s0 + c] = x;
+    return dd[org[lo]] + tr[lo];
+  };
+  const double x = ev(r);
+  int c = 0;
+  for (int u = 0; u < m; ++u) {
+    const double y = ev(u);
+    c += (y < x || (y == x && u < r)) ? 1 : 0;
   }
-  __syncthreads();
-  for (int t = threadIdx.x; t < k; t += blockDim.x) {
-    row[t] = s0 + perm[keep[t]];
-    I[kIrot * ld + s0 + t] = s0 + pos[keep[t]];   // output row of root t (rotation list no longer needed)
+  pos[r] = c;
+  lam2[s0 + c] = x;
+  (void)perm;
+}
+
+// the product's row lists (gather: child row of kept entry t; scatter: its
+// output row) and the deflated rows copied whole to their positions (one
+// workgroup per node row)
+__global__ void __launch_bounds__(256) k_dc_rows(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                 const DcNode* __restrict__ nodes, const double* __restrict__ vec,
+                                                 int* ivec, const double* __restrict__ Zin, double* Zout) {
+  const DcNode nd = nodes[blockIdx.y];
+  if (!act[nd.prob]) return;
+  const EigProb p = probs[nd.prob];
+  const int ld = p.ld, s0 = nd.start, m = nd.n1 + nd.n2;
+  const int r = blockIdx.x;
+  if (r >= m) return;
+  const double* V = vec + p.v_off;
+  int* I = ivec + p.i_off;
+  const int k = I[kIcnt * ld + s0];
+  const double* z = V + kVz * ld + s0;
+  const int* perm = I + kIperm * ld + s0;
+  const int* keep = I + kIkeep * ld + s0;
+  const int* pos = I + kIpos * ld + s0;
+  if (r < k && threadIdx.x == 0) {
+    I[kIrow * ld + s0 + r] = s0 + perm[keep[r]];
+    I[kIrot * ld + s0 + r] = s0 + pos[keep[r]];   // output row of root r (rotation list no longer needed)
   }
-  // deflated rows: copied whole (node columns) to their positions
+  if (z[r] != 0.0) return;
+  const long long src = s0 + perm[r], dst = s0 + pos[r];
   const double* Zi = Zin + p.m_off;
   double* Zo = Zout + p.m_off;
-  for (int r = 0; r < m; ++r) {
-    if (z[r] != 0.0) continue;
-    const long long src = s0 + perm[r], dst = s0 + pos[r];
-    for (int col = threadIdx.x; col < m; col += blockDim.x) Zo[dst * ld + s0 + col] = Zi[src * ld + s0 + col];
-  }
+  for (int col = threadIdx.x; col < m; col += blockDim.x) Zo[dst * ld + s0 + col] = Zi[src * ld + s0 + col];
 }
 
 // the eigenvectors of D + bz z z^T in the recomputed z: row t of U (t < k)
@@ -721,7 +802,7 @@ __global__ void k_dc_lam(const EigProb* __restrict__ probs, const int* __restric
 // forward, columnwise): T_ii = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) (V^T v_i)
 __global__ void __launch_bounds__(256) k_trd_tfactor(const EigProb* __restrict__ probs, const int* __restrict__ act,
                                                      const double* __restrict__ Vh, const double* __restrict__ vec,
-                                                     double* Tf, int npanel) {
+                                                     double* Tf) {
   __shared__ double g[kEigNb][kEigNb + 1];
   __shared__ double T[kEigNb][kEigNb + 1];
   if (!act[blockIdx.y]) return;
@@ -729,7 +810,8 @@ __global__ void __launch_bounds__(256) k_trd_tfactor(const EigProb* __restrict__
   const EigProb p = probs[blockIdx.y];
   const int n = p.n, ld = p.ld;
   const int j0 = pan * kEigNb;
-  double* To = Tf + ((long long)blockIdx.y * npanel + pan) * kEigNb * kEigNb;
+  const int tw = p.nchunk * kEigNb;   // row stride of the repeated T
+  double* To = Tf + p.t_off + (long long)pan * kEigNb * tw;
   if (j0 >= n - 1) return;
   const int jb = min(kEigNb, n - 1 - j0);
   const double* tau = vec + p.v_off + (long long)kVtau * ld;
@@ -759,7 +841,14 @@ __global__ void __launch_bounds__(256) k_trd_tfactor(const EigProb* __restrict__
     }
   }
   __syncthreads();
-  for (int x = threadIdx.x; x < kEigNb * kEigNb; x += blockDim.x) To[x] = T[x / kEigNb][x % kEigNb];
+  // T repeated once per K chunk: the products of the chunks are summed by
+  // the product with this wide T
+  for (int x = threadIdx.x; x < kEigNb * tw; x += blockDim.x) To[x] = T[x / tw][(x % tw) % kEigNb];
+}
+
+// the blocked path's problems are done for the block-Jacobi lanes
+__global__ void k_eig_claim(const EigProb* __restrict__ probs, int np, int* jstate) {
+  for (int k = threadIdx.x; k < np; k += blockDim.x) jstate[probs[k].mol * kJacState] = 1;
 }
 
 // the problems' matrices into the solver's layout (problems of molecules

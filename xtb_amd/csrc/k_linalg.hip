@@ -209,6 +209,107 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(const int4* __restrict__ tile
     }
 }
 
+// The same product on the FP32 matrix cores with an FP64 residual correction
+// (the mixed-precision option): each FP64 operand is split into hi = fl32(x)
+// and lo = fl32(x - hi), every 4-deep step forms hi*hi + hi*lo + lo*hi with
+// v_mfma_f32_16x16x4f32 (FP32 accumulation of 12 products) and adds that
+// partial to FP64 accumulators, so the rounding of FP32 accumulation is
+// confined to one step.  The f32 result layout: register i of a lane holds
+// row 4 (lane / 16) + i, column lane % 16.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <bool TA, bool TB, bool SCALE>
+__global__ void __launch_bounds__(256) k_gemm_mix(const int4* __restrict__ tiles, const MolDesc* __restrict__ desc,
+                                                  const double* __restrict__ A, const double* __restrict__ B,
+                                                  double* C, const double* __restrict__ scale, double alpha,
+                                                  const int* __restrict__ mask, int mstride) {
+  __shared__ double As[kBK][kTile + 1];
+  __shared__ double Bs[kBK][kTile + 1];
+  const int4 tl = tiles[blockIdx.x];
+  if (mask && !mask[(long long)tl.x * mstride]) return;
+  const MolDesc md = desc[tl.x];
+  const int N = md.nao;
+  const double* Am = A + md.m0;
+  const double* Bm = B + md.m0;
+  double* Cm = C + md.m0;
+  const double* sc = SCALE ? scale + md.o0 : nullptr;
+  const int row0 = tl.y * kTile, col0 = tl.z * kTile;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1, i16 = lane & 15, k4 = lane >> 4;
+  double acc[2][2][4];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[r][c][i] = 0.0;
+  for (int kb = 0; kb < N; kb += kBK) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int ii, kk, jj, k2;
+      if (TA) { kk = e >> 6; ii = e & 63; } else { ii = e >> 4; kk = e & 15; }
+      if (TB) { jj = e >> 4; k2 = e & 15; } else { k2 = e >> 6; jj = e & 63; }
+      const int gi = row0 + ii, gk = kb + kk, gj = col0 + jj, gk2 = kb + k2;
+      As[kk][ii] = (gi < N && gk < N) ? (TA ? Am[(long long)gk * N + gi] : Am[(long long)gi * N + gk]) : 0.0;
+      double x = 0.0;
+      if (gj < N && gk2 < N) {
+        x = TB ? Bm[(long long)gj * N + gk2] : Bm[(long long)gk2 * N + gj];
+        if (SCALE) x *= sc[gk2];
+      }
+      Bs[k2][jj] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 4) {
+      float ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const double a = As[kk + k4][32 * wr + 16 * r + i16];
+        ah[r] = (float)a;
+        al[r] = (float)(a - (double)ah[r]);
+      }
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const double b = Bs[kk + k4][32 * wc + 16 * c + i16];
+        bh[c] = (float)b;
+        bl[c] = (float)(b - (double)bh[c]);
+      }
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          v4f t = v4f{0.f, 0.f, 0.f, 0.f};
+          t = __builtin_amdgcn_mfma_f32_16x16x4f32(al[r], bh[c], t, 0, 0, 0);
+          t = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[r], bl[c], t, 0, 0, 0);
+          t = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[r], bh[c], t, 0, 0, 0);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) acc[r][c][i] += (double)t[i];
+        }
+    }
+    __syncthreads();
+  }
+  const bool sym = tl.w != 0, dtile = tl.y == tl.z;
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int gj = col0 + 32 * wc + 16 * c + i16;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int gi = row0 + 32 * wr + 16 * r + 4 * k4 + i;
+        if (gi >= N || gj >= N) continue;
+        const double v = alpha * acc[r][c][i];
+        if (!sym) {
+          Cm[(long long)gi * N + gj] = v;
+        } else if (!dtile || gi <= gj) {
+          Cm[(long long)gi * N + gj] = v;
+          Cm[(long long)gj * N + gi] = v;
+        }
+      }
+    }
+}
+
 // instantiations used by the engine
 template __global__ void k_gemm<false, false, false, false>(const int4*, const MolDesc*, const double*, const double*,
                                                             double*, const double*, const int*, double, const int*, int);
@@ -1268,11 +1369,11 @@ __device__ __forceinline__ void jac_roundR(double (&T)[4][2], double (&B)[4][2],
 constexpr int kCrossPeriod = 8;
 
 template <bool CROSSK>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
-                                                   int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
-                                                   const double* __restrict__ scale_in, int* state,
-                                                   int inner_sweeps, unsigned long long* work, double tolrel,
-                                                   unsigned long long* offmax, int stride) {
+__device__ __forceinline__ void jac_pivot_body(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
+                                               int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
+                                               const double* __restrict__ scale_in, int* state, int inner_sweeps,
+                                               unsigned long long* work, double tolrel, unsigned long long* offmax,
+                                               int stride) {
   __shared__ double2 rcs[16];
   __shared__ double red[2];
   const int2 it = list[blockIdx.x];
@@ -1396,12 +1497,24 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
   }
 }
 
+// sampled launches (ts != nullptr) record the device wall clock at the start
+// of their earliest wave and the end of their latest: the execution span
+template <bool CROSSK>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list, int r, const double* __restrict__ A,
+             double* Qbuf, int* rotflag, const double* __restrict__ scale_in, int* state, int inner_sweeps,
+             unsigned long long* work, double tolrel, unsigned long long* offmax, int stride, unsigned long long* ts) {
+  if (ts && threadIdx.x == 0) atomicMin(&ts[0], (unsigned long long)__builtin_amdgcn_s_memrealtime());
+  jac_pivot_body<CROSSK>(probs, list, r, A, Qbuf, rotflag, scale_in, state, inner_sweeps, work, tolrel, offmax, stride);
+  if (ts && threadIdx.x == 0) atomicMax(&ts[1], (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+
 template __global__ void k_jac_pairsR<false>(const JacobiProb*, const int2*, int, const double*, double*, int*,
                                              const double*, int*, int, unsigned long long*, double,
-                                             unsigned long long*, int);
+                                             unsigned long long*, int, unsigned long long*);
 template __global__ void k_jac_pairsR<true>(const JacobiProb*, const int2*, int, const double*, double*, int*,
                                             const double*, int*, int, unsigned long long*, double,
-                                            unsigned long long*, int);
+                                            unsigned long long*, int, unsigned long long*);
 
 // ---------------------------------------------------------------------------
 // Tile updates of the block-Jacobi rounds with 32 x 32 pivots, one wave per
@@ -1627,7 +1740,11 @@ __global__ void __launch_bounds__(64) k_jac_apply_R(const JacobiProb* __restrict
                                                     int r, double* A, double* W, const double* __restrict__ Qbuf,
                                                     const int* __restrict__ rotflag, const int* __restrict__ state,
                                                     unsigned long long* offmax, int stride,
-                                                    unsigned long long* work) {
+                                                    unsigned long long* work, unsigned long long* ts) {
+  // sampled launches record the device wall clock at the start of their
+  // earliest wave (ts[0], minimum) and the end of their latest (ts[1],
+  // maximum): the launch's execution span
+  if (ts && threadIdx.x == 0) atomicMin(&ts[0], (unsigned long long)__builtin_amdgcn_s_memrealtime());
   const int4 it = list[blockIdx.x];
   // sampled launches count their algorithmic bytes (one atomic per tile)
   if (work && threadIdx.x == 0) {
@@ -1638,6 +1755,7 @@ __global__ void __launch_bounds__(64) k_jac_apply_R(const JacobiProb* __restrict
     jac_tile_a(probs, it, r, A, Qbuf, rotflag, state, offmax, stride);
   else if (it.w == 1)
     jac_tile_w(probs, it, r, W, Qbuf, rotflag, state);
+  if (ts && threadIdx.x == 0) atomicMax(&ts[1], (unsigned long long)__builtin_amdgcn_s_memrealtime());
 }
 
 // initialise W = I and the per-problem scale max|A_ii| and state, clear

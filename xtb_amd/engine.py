@@ -79,6 +79,8 @@ def library():
     lib.xtbamd_batch_profile_result.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.xtbamd_batch_profile_result.restype = ctypes.c_int
+    lib.xtbamd_batch_profile_span.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.xtbamd_batch_profile_span.restype = ctypes.c_int
     lib.xtbamd_singlepoint_noalloc.argtypes = [ctypes.c_int, ip, _dp, ctypes.c_double, ctypes.c_int,
                                                ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                                ctypes.POINTER(Result)]
@@ -180,6 +182,13 @@ class Batch:
         self._check(self._lib.xtbamd_batch_profile_result(self._h, self.KERNELS[kernel], ctypes.byref(n),
                                                           ctypes.byref(t), ctypes.byref(w)))
         return n.value, t.value, w.value
+
+    def profile_span(self, kernel="apply"):
+        """Summed execution spans (device wall clock) of the sampled launches,
+        scaled to all launches, in ms (tile updates only)."""
+        t = ctypes.c_double()
+        self._check(self._lib.xtbamd_batch_profile_span(self._h, self.KERNELS[kernel], ctypes.byref(t)))
+        return t.value
 
     def charges(self, m):
         """Current atomic charges of molecule m (after setup(): the initial guess)."""

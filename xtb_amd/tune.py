@@ -4,13 +4,17 @@ Times, on the device, the choices the engine leaves open and writes them to a
 tuning file that every batch reads at set-up (``tuning_gfx950.txt`` next to
 ``libxtb_amd.so``; environment variables ``XTBAMD_*`` override it):
 
-* ``eig_blk_min``: the smallest eigenproblem dimension that goes to the
-  blocked eigensolver (Householder tridiagonalisation, divide and conquer,
-  back-transformation) instead of block-Jacobi rounds.  One-molecule single
-  points per size bin (water clusters and alkanes) are timed with both paths;
-  the cutoff is the smallest bin from which the blocked path wins in every
-  larger bin, and it is kept only if the 512-molecule batch is not slower
-  with it.
+* ``eig_blk_min`` and ``eig_blk_iters``: the smallest eigenproblem
+  dimension that goes to the blocked eigensolver (Householder
+  tridiagonalisation, divide and conquer, back-transformation) instead of
+  block-Jacobi rounds, and for how many SCC iterations (from the first; the
+  block-Jacobi rounds start from the previous orbitals after that).
+  One-molecule single points per size bin (water clusters and alkanes, up to
+  the 700-water cluster) are timed with block Jacobi, the blocked solver in
+  every iteration and in the first 2 or 4; the cutoff is the smallest bin
+  from which a blocked variant wins in every larger bin, the iteration count
+  that of the largest bin, and both are kept only if the 512-molecule batch
+  is not slower with them.
 * ``jac_lanes`` (1..3) and ``jac_cross`` (0/1): the stream lanes and the
   cross-round schedule of the block-Jacobi rounds, timed on the 512-molecule
   workload.
@@ -35,7 +39,7 @@ if ROOT not in sys.path:
 
 from xtb_amd import workloads  # noqa: E402
 
-KEYS = ("XTBAMD_EIG_BLK_MIN", "XTBAMD_JAC_LANES", "XTBAMD_JAC_CROSS", "XTBAMD_GEMM")
+KEYS = ("XTBAMD_EIG_BLK_MIN", "XTBAMD_EIG_BLK_ITERS", "XTBAMD_JAC_LANES", "XTBAMD_JAC_CROSS", "XTBAMD_GEMM")
 
 
 def _set(env):
@@ -64,24 +68,38 @@ def _time_batch(mols, env, steps, warmup=1):
 
 def tune(steps=3, out=HERE, log=print):
     rows_bins, rows_batch = [], []
-    # 1. eigensolver path per size bin (one molecule per batch)
+    # 1. eigensolver path per size bin (one molecule per batch): block Jacobi,
+    # the blocked solver in every SCC iteration, or in the first few only
     bins = [("water", 12), ("alkane", 24), ("water", 22), ("alkane", 44), ("water", 43), ("alkane", 86),
-            ("water", 64), ("water", 100), ("water", 170), ("water", 256), ("water", 400)]
-    faster = []
+            ("water", 64), ("water", 100), ("water", 170), ("water", 256), ("water", 400), ("water", 700)]
+    hybrid = (2, 4)
+    best_per_bin = []
     for kind, size in bins:
-        z, x = workloads.water_cluster(size, 7) if kind == "water" else workloads.alkane(size, 7)
+        if kind == "water" and size == 700:
+            z, x = workloads.large_water_cluster(2024)
+        else:
+            z, x = workloads.water_cluster(size, 7) if kind == "water" else workloads.alkane(size, 7)
         nao = workloads.ao_count(z)
         tj, ej = _time_batch([(z, x)], {"XTBAMD_EIG_BLK_MIN": 1 << 30}, steps)
         tb, eb = _time_batch([(z, x)], {"XTBAMD_EIG_BLK_MIN": 65}, steps)
+        th = []
+        for it in hybrid:
+            t, _ = _time_batch([(z, x)], {"XTBAMD_EIG_BLK_MIN": 65, "XTBAMD_EIG_BLK_ITERS": it}, steps)
+            th.append(t)
         dE = abs(ej[0] - eb[0])
-        rows_bins.append((kind, size, nao, tj, tb, dE))
-        faster.append((nao, tb < tj))
-        log(f"tune eig: {kind} {size} nao={nao} jacobi {1e3 * tj:.1f} ms blocked {1e3 * tb:.1f} ms dE={dE:.1e}")
-    faster.sort()
-    cut = 1 << 30
-    for i, (nao, win) in enumerate(faster):
-        if all(w for _, w in faster[i:]):
+        rows_bins.append((kind, size, nao, tj, tb, th, dE))
+        opts = {1 << 30: tb}
+        opts.update({it: t for it, t in zip(hybrid, th)})
+        it_best = min(opts, key=lambda k: opts[k])
+        best_per_bin.append((nao, opts[it_best] < tj, it_best))
+        log(f"tune eig: {kind} {size} nao={nao} jacobi {1e3 * tj:.1f} ms blocked {1e3 * tb:.1f} ms "
+            + " ".join(f"first-{it} {1e3 * t:.1f} ms" for it, t in zip(hybrid, th)) + f" dE={dE:.1e}")
+    best_per_bin.sort()
+    cut, iters = 1 << 30, 1 << 30
+    for i, (nao, win, _) in enumerate(best_per_bin):
+        if all(w for _, w, _ in best_per_bin[i:]):
             cut = nao
+            iters = best_per_bin[-1][2]   # the choice of the largest molecule
             break
     # 2. batch schedule on the 512-molecule workload
     mols = workloads.mixed_batch(512, seed=2024)
@@ -103,7 +121,8 @@ def tune(steps=3, out=HERE, log=print):
     # the cutoff must not slow the batch down (its largest problems are below
     # it unless the blocked path wins there too)
     if cut < 1 << 30:
-        tc, _ = _time_batch(mols, dict(base, XTBAMD_GEMM=gemm, XTBAMD_EIG_BLK_MIN=cut), steps)
+        tc, _ = _time_batch(mols, dict(base, XTBAMD_GEMM=gemm, XTBAMD_EIG_BLK_MIN=cut, XTBAMD_EIG_BLK_ITERS=iters),
+                            steps)
         rows_batch.append((f"eig_blk_min={cut}", tc))
         if tc > min(tv, tm):
             rows_batch.append((f"eig_blk_min={cut} rejected for the batch", tc))
@@ -113,21 +132,25 @@ def tune(steps=3, out=HERE, log=print):
     stamp = time.strftime("%Y-%m-%d %H:%M:%S")
     with open(os.path.join(out, "tuning_gfx950.txt"), "w") as f:
         f.write(f"# written by python -m xtb_amd.tune ({stamp}); environment variables XTBAMD_* override\n")
-        f.write(f"eig_blk_min {cut}\njac_lanes {lanes}\njac_cross {cross}\ngemm {gemm}\n")
+        f.write(f"eig_blk_min {cut}\neig_blk_iters {iters}\njac_lanes {lanes}\njac_cross {cross}\ngemm {gemm}\n")
     with open(os.path.join(out, "autotune_gfx950.md"), "w") as f:
         f.write("# Autotuner measurements (one MI355X)\n\n")
         f.write("Command: `python -m xtb_amd.tune --out <dir>` (median of "
                 f"{steps} timed runs after one warm-up per configuration).\n\n")
         f.write("## Eigensolver path per size (one molecule per batch, whole single point)\n\n")
-        f.write("| molecule | AOs | block Jacobi ms | blocked ms | dE (Eh) |\n|---|---:|---:|---:|---:|\n")
-        for kind, size, nao, tj, tb, dE in rows_bins:
+        f.write("| molecule | AOs | block Jacobi ms | blocked, every iteration ms | "
+                + " | ".join(f"blocked, first {it} iterations ms" for it in hybrid)
+                + " | dE Jacobi vs blocked (Eh) |\n|---|---:|---:|---:|" + "---:|" * len(hybrid) + "---:|\n")
+        for kind, size, nao, tj, tb, th, dE in rows_bins:
             name = f"(H2O){size}" if kind == "water" else f"C{size}H{2 * size + 2}"
-            f.write(f"| {name} | {nao} | {1e3 * tj:.1f} | {1e3 * tb:.1f} | {dE:.1e} |\n")
+            f.write(f"| {name} | {nao} | {1e3 * tj:.1f} | {1e3 * tb:.1f} | "
+                    + " | ".join(f"{1e3 * t:.1f}" for t in th) + f" | {dE:.1e} |\n")
         f.write("\n## 512-molecule batch (ms per batched step)\n\n| configuration | ms |\n|---|---:|\n")
         for name, t in rows_batch:
             f.write(f"| {name} | {1e3 * t:.1f} |\n")
-        f.write(f"\nChosen: eig_blk_min {cut}, jac_lanes {lanes}, jac_cross {cross}, gemm {gemm}.\n")
-    return {"eig_blk_min": cut, "jac_lanes": lanes, "jac_cross": cross, "gemm": gemm}
+        f.write(f"\nChosen: eig_blk_min {cut}, eig_blk_iters {iters}, jac_lanes {lanes}, jac_cross {cross}, "
+                f"gemm {gemm}.\n")
+    return {"eig_blk_min": cut, "eig_blk_iters": iters, "jac_lanes": lanes, "jac_cross": cross, "gemm": gemm}
 
 
 def main(argv=None):
