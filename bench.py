@@ -231,7 +231,7 @@ def main():
         b.run()
         times.append(time.perf_counter() - t)
     prof = {k: b.profile_result(k) for k in ("gemm", "jacobi", "apply")}
-    apply_span_ms = b.profile_span("apply")
+    spans = {"jacobi": b.profile_span("jacobi"), "apply": b.profile_span("apply")}
     res = b.results()
     # every timed step is a complete single point of every molecule: the
     # results of the last step equal the plain path's and iterate from scratch
@@ -297,19 +297,21 @@ def main():
         if not n:
             continue
         sec = tms * 1e-3
+        events_us = 1e3 * tms / n
+        if spans.get(k, 0.0) > 0.0:
+            # the Jacobi launches: duration from the sampled launches'
+            # execution spans on the device wall clock (first wave's start to
+            # last wave's end, as a kernel trace measures it); the HIP-event
+            # time beside it also holds the stream's dispatch gap before each
+            # launch in the three-lane schedule
+            sec = spans[k] * 1e-3
+            tms = spans[k]
         if k == "apply":
-            # duration: the sampled launches' execution spans on the device
-            # wall clock (first wave's start to last wave's end, as a kernel
-            # trace measures it); the HIP-event time beside it also holds the
-            # stream's dispatch gap before each launch
-            span = apply_span_ms * 1e-3 if apply_span_ms > 0 else sec
-            ach = work / span / 1e9
+            ach = work / sec / 1e9
             line = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                    "bytes_per_launch": work / n, "avg_us_events": 1e3 * tms / n,
-                    "duration": "device wall clock per sampled launch (kernel execution span)",
+                    "bytes_per_launch": work / n,
                     "work": "compulsory HBM bytes: each element of A and W a rotation touches read once and "
                             "written once (A through one of its mirrored tiles), counted on the device"}
-            tms = span * 1e3
         else:
             ach = work / sec / 1e12
             line = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -317,6 +319,9 @@ def main():
             if k == "jacobi":
                 line["note"] = ("FP64 VALU work (the FP64 vector and matrix peaks are both 78.6 TFLOP/s); bound by "
                                 "the latency of its dependent inner rounds")
+        if k in spans:
+            line["avg_us_events"] = events_us
+            line["duration"] = "device wall clock per sampled launch (kernel execution span)"
         line.update({"kernel": names[k], "traffic": table.get(short[k]), "launches": n, "avg_us": 1e3 * tms / n,
                      "share_of_step_events": tms / (args.steps * ms),
                      "share_of_gpu_time_rocprof": shares.get(short[k])})

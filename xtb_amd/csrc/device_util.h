@@ -7,6 +7,21 @@
 
 namespace xtbamd {
 
+// sampled-launch counters and clocks, spread over kProfLines lines (engine.h)
+__device__ __forceinline__ unsigned long long* prof_line(unsigned long long* p) {
+  return p + (blockIdx.x % kProfLines) * kProfLineU64;
+}
+__device__ __forceinline__ void prof_add(unsigned long long* work, unsigned long long v) {
+  atomicAdd(prof_line(work), v);
+}
+// ts[0] (per line): earliest start, ts[1]: latest end on the device wall clock
+__device__ __forceinline__ void prof_clock_start(unsigned long long* ts) {
+  if (ts && threadIdx.x == 0) atomicMin(prof_line(ts), (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+__device__ __forceinline__ void prof_clock_end(unsigned long long* ts) {
+  if (ts && threadIdx.x == 0) atomicMax(prof_line(ts) + 1, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -113,6 +113,11 @@ struct JacobiProb {
 
 // stream lanes of the block-Jacobi rounds (at most)
 constexpr int kMaxLanes = 3;
+// Counters and clocks of sampled launches are spread over kProfLines 128-byte
+// lines, workgroup b using line b mod kProfLines: same-address device atomics
+// from thousands of workgroups serialise (about 20 ns each), which stretched a
+// sampled launch to ten times its length.  The host reduces over the lines.
+constexpr int kProfLines = 64, kProfLineU64 = 16;
 
 // A set of Jacobi problems (one per molecule) with its device work lists.
 struct JacSet {
@@ -428,7 +433,9 @@ class Batch {
   // one of its mirrored tiles).  The Jacobi families are timed on a sample of
   // the rounds.
   static constexpr int kProfKinds = 4;   // 1 GEMM, 2 Jacobi pivots, 3 Jacobi tile updates
-  static constexpr int kProfTsSlots = 8192;   // sampled tile-update launches with device wall-clock spans
+  // sampled launches per family (pivots, tile updates) with device wall-clock
+  // spans; each holds a clock pair in each of its kProfLines lines
+  static constexpr int kProfTsSlots = 1024, kProfSlotU64 = kProfLines * kProfLineU64;
   unsigned long long* prof_ts_ = nullptr;
   int prof_ts_used_[2] = {0, 0};   // pivots, tile updates
   double prof_span(int kind);

@@ -46,11 +46,12 @@ __global__ void k_fill_int(int* p, long long n, int v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+// n (start, end) clock pairs, one per 128-byte line
 __global__ void k_ts_init(unsigned long long* ts, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    ts[2 * i] = ~0ull;
-    ts[2 * i + 1] = 0ull;
+    ts[(long long)kProfLineU64 * i] = ~0ull;
+    ts[(long long)kProfLineU64 * i + 1] = 0ull;
   }
 }
 __global__ void k_copy(const double* a, double* b, long long n) {
@@ -397,7 +398,7 @@ __global__ void k_count_gemm(const MolDesc* __restrict__ desc, int nmol, const i
   if (m >= nmol) return;
   if (mask && !mask[(long long)m * mstride]) return;
   const unsigned long long N = (unsigned long long)desc[m].nao;
-  atomicAdd(work, 2ull * N * N * N);
+  prof_add(work, 2ull * N * N * N);
 }
 
 __global__ void k_copy_state_active(const MolState* __restrict__ st, int nmol, int* act) {
@@ -1748,9 +1749,10 @@ void eig_set_run(EigSet& es, const JacobiProb* jprobs, int* jstate, const MolSta
 // round is two launches per lane: the pivots (which first test the problems
 // at a sweep boundary against the off-diagonal maximum the tile updates
 // measured at the end of the sweep before) and the A and W tile updates.
-// Once per max_rounds rounds, after the pivots, the host reads both lanes'
-// counts of problems still running and stops a lane that has none left.
-constexpr int kProfEvery = 17;
+// Once per period (the rounds of a sweep of the lanes' largest problem),
+// after the pivots, the host reads the lanes' counts of problems still
+// running and stops a lane that has none left.
+constexpr int kProfEvery = 51;
 
 void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   const hipStream_t ls[kMaxLanes] = {stream, stream2, stream4};
@@ -1767,10 +1769,10 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   const int period = std::max(1, jac_period_);
   const int rmax = (jac_max_sweeps + 1) * js.max_rounds;
   for (int r = 0; r <= rmax; ++r) {
-    // timing events (and the work counts) on every kProfEvery-th round only,
-    // counted over all eigensolves (17 is prime to the cross-round period 8,
-    // so the sample meets every kind of round alike): an event pair per
-    // launch would cost more host time than the launch
+    // timing events, device spans and work counts on every kProfEvery-th
+    // round only, counted over all eigensolves (51 is prime to the
+    // cross-round period 8, so the sample meets every kind of round alike):
+    // an event pair per launch would cost more host time than the launch
     const bool sample = (prof_round_ % kProfEvery) == 0;
     for (int g = 0; g < kMaxLanes; ++g) {
       if (!live[g]) continue;
@@ -1778,7 +1780,7 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       if (sample) prof_begin(2, ls[g]);
       unsigned long long* pts = nullptr;
       if (sample && (prof_mask & 4) && prof_ts_used_[0] < kProfTsSlots)
-        pts = prof_ts_ + 2 * (prof_ts_used_[0]++);
+        pts = prof_ts_ + (size_t)kProfSlotU64 * (prof_ts_used_[0]++);
       if (jac_cross)
         hipLaunchKernelGGL(k_jac_pairsR<true>, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
                            js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
@@ -1814,7 +1816,7 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       if (sample) prof_begin(3, ls[g]);
       unsigned long long* ts = nullptr;
       if (sample && (prof_mask & 8) && prof_ts_used_[1] < kProfTsSlots)
-        ts = prof_ts_ + 2 * (kProfTsSlots + prof_ts_used_[1]++);
+        ts = prof_ts_ + (size_t)kProfSlotU64 * (kProfTsSlots + prof_ts_used_[1]++);
       hipLaunchKernelGGL(k_jac_apply_R, dim3(js.gxt[g + 1] - js.gxt[g]), dim3(64), 0, ls[g], js.d_probs,
                          js.d_xt + js.gxt[g], r, Am, Wm, (const double*)js.qbuf, (const int*)js.rot,
                          (const int*)js.state, js.offmax, nmol, tw, ts);
@@ -1953,11 +1955,15 @@ void Batch::prof_enable(int mask) {
     }
   }
   prof_round_ = 0;
-  if (!prof_ts_) prof_ts_ = alloc<unsigned long long>(4 * kProfTsSlots);
+  // two families of kProfTsSlots sampled launches, kProfLines clock pairs each
+  if (!prof_ts_) prof_ts_ = alloc<unsigned long long>((size_t)2 * kProfTsSlots * kProfSlotU64);
   prof_ts_used_[0] = prof_ts_used_[1] = 0;
-  hipLaunchKernelGGL(k_ts_init, dim3(grid1(2 * kProfTsSlots, 256)), dim3(256), 0, stream, prof_ts_, 2 * kProfTsSlots);
-  if (!prof_work_) prof_work_ = alloc<unsigned long long>(kProfKinds);
-  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, stream, (double*)prof_work_, (long long)kProfKinds, 0.0);
+  hipLaunchKernelGGL(k_ts_init, dim3(grid1(2 * kProfTsSlots * kProfLines, 256)), dim3(256), 0, stream, prof_ts_,
+                     2 * kProfTsSlots * kProfLines);
+  // the work counters: kind k at offset k of each of the kProfLines lines
+  if (!prof_work_) prof_work_ = alloc<unsigned long long>((size_t)kProfLines * kProfLineU64);
+  hipLaunchKernelGGL(k_fill, dim3(grid1(kProfLines * kProfLineU64, 256)), dim3(256), 0, stream, (double*)prof_work_,
+                     (long long)kProfLines * kProfLineU64, 0.0);
   HIPCHECK(hipStreamSynchronize(stream));
 }
 
@@ -1988,18 +1994,26 @@ double Batch::prof_span(int kind) {
   const int fam = kind - 2, nused = prof_ts_used_[fam];
   if (nused == 0) return 0.0;
   HIPCHECK(hipDeviceSynchronize());
-  std::vector<unsigned long long> h(2 * (size_t)nused);
-  HIPCHECK(hipMemcpy(h.data(), prof_ts_ + 2 * (size_t)fam * kProfTsSlots, h.size() * sizeof(unsigned long long),
-                     hipMemcpyDeviceToHost));
+  std::vector<unsigned long long> h((size_t)kProfSlotU64 * nused);
+  HIPCHECK(hipMemcpy(h.data(), prof_ts_ + (size_t)kProfSlotU64 * fam * kProfTsSlots,
+                     h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   int khz = 0;
   HIPCHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
   double ticks = 0.0;
   int used = 0;
-  for (int i = 0; i < nused; ++i)
-    if (h[2 * i + 1] >= h[2 * i] && h[2 * i] != ~0ull) {
-      ticks += (double)(h[2 * i + 1] - h[2 * i]);
+  for (int i = 0; i < nused; ++i) {
+    // earliest start and latest end over the slot's lines
+    unsigned long long t0 = ~0ull, t1 = 0ull;
+    for (int l = 0; l < kProfLines; ++l) {
+      const unsigned long long* c = h.data() + (size_t)kProfSlotU64 * i + (size_t)kProfLineU64 * l;
+      t0 = std::min(t0, c[0]);
+      t1 = std::max(t1, c[1]);
+    }
+    if (t0 != ~0ull && t1 >= t0) {
+      ticks += (double)(t1 - t0);
       ++used;
     }
+  }
   if (used == 0 || khz <= 0) return 0.0;
   const double ms = ticks / (double)khz;   // ticks / (kHz) = ms
   return ms * (double)prof_all_[kind] / used;
@@ -2015,7 +2029,11 @@ int Batch::prof_result(int kind, int* launches, double* total_ms, double* work) 
     t += ms;
   }
   unsigned long long w = 0;
-  if (prof_work_) HIPCHECK(hipMemcpy(&w, prof_work_ + kind, sizeof(w), hipMemcpyDeviceToHost));
+  if (prof_work_) {
+    std::vector<unsigned long long> h((size_t)kProfLines * kProfLineU64);
+    HIPCHECK(hipMemcpy(h.data(), prof_work_, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int l = 0; l < kProfLines; ++l) w += h[(size_t)kProfLineU64 * l + kind];
+  }
   // a sampled family (timed on a subset of its launches) is reported for all
   // of its launches: the sample's time and work scaled by all / sampled
   const int timed = prof_used_[kind] / 2;

@@ -351,11 +351,14 @@ __global__ void __launch_bounds__(64) k_d4_grad2(const MolDesc* __restrict__ des
 
 // ATM three-body term with zero-charge C6 (atm_gradient_latp, molecular branch:
 // src/disp/dftd4.F90:2121-2174, deriv_atm_triple :2354-2433): the triples of atom a
+constexpr int kAtmLds = 4096;   // molecules up to this size keep per-atom factors in LDS
+
 __global__ void __launch_bounds__(256) k_d4_atm(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
                                                 const int* __restrict__ at, const double* __restrict__ xyz,
                                                 const double* __restrict__ c6, const double* __restrict__ dc6,
                                                 double* grad, double* dedcn, double* eatm) {
   __shared__ double sh[4];
+  __shared__ double sq[kAtmLds];   // sqrt of each atom's sqrt(Z) <r4>/<r2> factor
   const MolDesc md = desc[blockIdx.y];
   const int a = blockIdx.x;
   if (a >= md.n) return;
@@ -364,6 +367,14 @@ __global__ void __launch_bounds__(256) k_d4_atm(const MolDesc* __restrict__ desc
   const double* C = c6 + md.p0;
   const double* dC = dc6 + md.p0;
   const double cut2 = 1600.0;
+  const bool cached = n <= kAtmLds;
+  if (cached)
+    for (int b = threadIdx.x; b < n; b += blockDim.x) sq[b] = sqrt(el[at[md.a0 + b]].sqrtzr4r2);
+  __syncthreads();
+  // damping radius of a pair: a1 sqrt(3 s_i s_j) + a2 = a1 sqrt(3) sqrt(s_i) sqrt(s_j) + a2
+  auto rad = [&](int i, int j, double si, double sj) {
+    return cached ? kD4a1 * 1.7320508075688772 * sq[i] * sq[j] + kD4a2 : kD4a1 * sqrt(3.0 * si * sj) + kD4a2;
+  };
   const double ra[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
   const double sa = el[at[md.a0 + a]].sqrtzr4r2;
   double g[3] = {0, 0, 0}, e = 0.0, dcn = 0.0;
@@ -382,9 +393,9 @@ __global__ void __launch_bounds__(256) k_d4_atm(const MolDesc* __restrict__ desc
     const double r2jk = rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2];
     if (r2jk > cut2) continue;
     const double sb = el[at[md.a0 + b]].sqrtzr4r2, sc = el[at[md.a0 + c]].sqrtzr4r2;
-    const double cij = kD4a1 * sqrt(3.0 * sa * sb) + kD4a2;
-    const double cik = kD4a1 * sqrt(3.0 * sa * sc) + kD4a2;
-    const double cjk = kD4a1 * sqrt(3.0 * sb * sc) + kD4a2;
+    const double cij = rad(a, b, sa, sb);
+    const double cik = rad(a, c, sa, sc);
+    const double cjk = rad(b, c, sb, sc);
     const double c6ij = C[(long long)a * n + b], c6ik = C[(long long)a * n + c], c6jk = C[(long long)b * n + c];
     const double c9 = -sqrt(c6ij * c6ik * c6jk);
     const double ccc1 = cij * cjk * cik;
@@ -393,7 +404,10 @@ __global__ void __launch_bounds__(256) k_d4_atm(const MolDesc* __restrict__ desc
     const double rrr3 = rrr1 * rrr2;
     const double ang_fact = 0.375 / (rrr2 * rrr3);
     const double ang = ang_fact * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) * (-r2ij + r2jk + r2ik) + 1.0 / rrr3;
-    const double cralp = pow(ccc1 / rrr1, (double)kD4alp / 3.0);
+    // (ccc1 / rrr1)^(alp / 3) with alp = 16: x^5 cbrt(x)
+    static_assert(kD4alp == 16, "ATM damping exponent");
+    const double qd = ccc1 / rrr1, qd2 = qd * qd;
+    const double cralp = qd2 * qd2 * qd * cbrt(qd);
     const double fdmp = 1.0 / (1.0 + 6.0 * cralp);
     const double dfdmp = -(2.0 * kD4alp * cralp) * fdmp * fdmp;
     const double dE = -fdmp * ang * c9;
@@ -448,8 +462,8 @@ __global__ void __launch_bounds__(64) k_grad_pairs(const MolDesc* __restrict__ d
       const double qi = qsh[md.s0 + at_sh0[md.a0 + i] + is];
       for (int js = 0; js < ej.nsh; ++js) {
         const double gij = 0.5 * (ei.hard[is] + ej.hard[js]);
-        const double g1 = 1.0 / (pow(r1, 2.0) + pow(gij, -2.0));
-        cq += g1 * pow(g1, 0.5) * qi * qsh[md.s0 + at_sh0[md.a0 + j] + js];
+        const double g1 = 1.0 / (r1 * r1 + 1.0 / (gij * gij));
+        cq += g1 * sqrt(g1) * qi * qsh[md.s0 + at_sh0[md.a0 + j] + js];
       }
     }
     for (int k = 0; k < 3; ++k) g[k] += -rij[k] * cq;
@@ -458,7 +472,7 @@ __global__ void __launch_bounds__(64) k_grad_pairs(const MolDesc* __restrict__ d
       const double alpha = sqrt(ei.rep_alpha * ej.rep_alpha);
       const double zeff = ei.rep_zeff * ej.rep_zeff;
       const double kexp = (zi > 2 || zj > 2) ? 1.5 : 1.0;
-      const double t16 = pow(r1, kexp), t26 = exp(-alpha * t16);
+      const double t16 = (zi > 2 || zj > 2) ? r1 * sqrt(r1) : r1, t26 = exp(-alpha * t16);
       const double dE = zeff * t26 / r1;
       e += 0.5 * dE;
       for (int k = 0; k < 3; ++k) g[k] += -(alpha * t16 * kexp + 1.0) * dE * rij[k] / r2;

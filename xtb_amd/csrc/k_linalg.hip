@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(256) k_jac_pairs(const JacobiProb* __restrict_
     for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) Qp[e] = q[(e >> 6) * kLd + (e & 63)];
   if (threadIdx.x == 0) {
     rotflag[(int)(pr.q_off / 4096) + it.y] = nrot > 0 ? 1 : 0;
-    if (work && nrot > 0) atomicAdd(work, 36864ull * (unsigned long long)nrot);
+    if (work && nrot > 0) prof_add(work, 36864ull * (unsigned long long)nrot);
   }
 }
 
@@ -1024,7 +1024,7 @@ __global__ void __launch_bounds__(NT) k_jac_pairsP(const JacobiProb* __restrict_
     for (int e = threadIdx.x; e < P * P; e += NT) Qp[e] = q[(e / P) * L + (e % P)];
   if (threadIdx.x == 0) {
     rotflag[(int)(pr.q_off / (P * P)) + it.y] = nrot > 0 ? 1 : 0;
-    if (work && nrot > 0) atomicAdd(work, (unsigned long long)(9 * P * P) * (unsigned long long)nrot);
+    if (work && nrot > 0) prof_add(work, (unsigned long long)(9 * P * P) * (unsigned long long)nrot);
   }
 }
 
@@ -1493,7 +1493,7 @@ __device__ __forceinline__ void jac_pivot_body(const JacobiProb* __restrict__ pr
     }
   if (threadIdx.x == 0) {
     rotflag[(int)(pr.q_off / (32 * 32)) + it.y] = total > 0 ? 1 : 0;
-    if (work && total > 0) atomicAdd(work, (unsigned long long)(9 * 32 * 32) * (unsigned long long)total);
+    if (work && total > 0) prof_add(work, (unsigned long long)(9 * 32 * 32) * (unsigned long long)total);
   }
 }
 
@@ -1504,9 +1504,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
 k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list, int r, const double* __restrict__ A,
              double* Qbuf, int* rotflag, const double* __restrict__ scale_in, int* state, int inner_sweeps,
              unsigned long long* work, double tolrel, unsigned long long* offmax, int stride, unsigned long long* ts) {
-  if (ts && threadIdx.x == 0) atomicMin(&ts[0], (unsigned long long)__builtin_amdgcn_s_memrealtime());
+  prof_clock_start(ts);
   jac_pivot_body<CROSSK>(probs, list, r, A, Qbuf, rotflag, scale_in, state, inner_sweeps, work, tolrel, offmax, stride);
-  if (ts && threadIdx.x == 0) atomicMax(&ts[1], (unsigned long long)__builtin_amdgcn_s_memrealtime());
+  prof_clock_end(ts);
 }
 
 template __global__ void k_jac_pairsR<false>(const JacobiProb*, const int2*, int, const double*, double*, int*,
@@ -1744,18 +1744,18 @@ __global__ void __launch_bounds__(64) k_jac_apply_R(const JacobiProb* __restrict
   // sampled launches record the device wall clock at the start of their
   // earliest wave (ts[0], minimum) and the end of their latest (ts[1],
   // maximum): the launch's execution span
-  if (ts && threadIdx.x == 0) atomicMin(&ts[0], (unsigned long long)__builtin_amdgcn_s_memrealtime());
+  prof_clock_start(ts);
   const int4 it = list[blockIdx.x];
   // sampled launches count their algorithmic bytes (one atomic per tile)
   if (work && threadIdx.x == 0) {
     const unsigned long long b = jac_tile_bytes(probs, it, r, rotflag, state);
-    if (b) atomicAdd(work, b);
+    if (b) prof_add(work, b);
   }
   if (it.w == 0)
     jac_tile_a(probs, it, r, A, Qbuf, rotflag, state, offmax, stride);
   else if (it.w == 1)
     jac_tile_w(probs, it, r, W, Qbuf, rotflag, state);
-  if (ts && threadIdx.x == 0) atomicMax(&ts[1], (unsigned long long)__builtin_amdgcn_s_memrealtime());
+  prof_clock_end(ts);
 }
 
 // initialise W = I and the per-problem scale max|A_ii| and state, clear
@@ -2198,7 +2198,7 @@ __global__ void __launch_bounds__(256) k_tred_ql(const JacobiProb* __restrict__ 
   // LAPACK dsytrd/dorgtr counts), 6 N per applied rotation
   if (work && tid == 0) {
     const unsigned long long n3 = (unsigned long long)N * N * N;
-    atomicAdd(work, (8ull * n3) / 3ull + 6ull * (unsigned long long)N * nrot);
+    prof_add(work, (8ull * n3) / 3ull + 6ull * (unsigned long long)N * nrot);
   }
 }
 

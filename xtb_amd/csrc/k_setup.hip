@@ -223,7 +223,7 @@ __global__ void k_sdqh0(const MolDesc* __restrict__ desc, const Element* __restr
       const double hii = selfe[md.s0 + at_sh0[md.a0 + i] + is];
       const double hjj = selfe[md.s0 + at_sh0[md.a0 + j] + js];
       const double zi = ei.zeta[is], zj = ej.zeta[js];
-      const double zetaij = pow(2 * sqrt(zi * zj) / (zi + zj), 0.5);
+      const double zetaij = sqrt(2 * sqrt(zi * zj) / (zi + zj));
       // kScale: 0.5*(kshell_i + kshell_j), s-d and p-d pairs 2.0 (src/xtb/gfn2.f90:823-833)
       const double ks[3] = {1.85, 2.23, 2.23};
       double kscale = 0.5 * (ks[li] + ks[lj]);
@@ -311,7 +311,7 @@ __global__ void k_coulomb(const MolDesc* __restrict__ desc, const Element* __res
     const double dx = xb[0] - xa[0], dy = xb[1] - xa[1], dz = xb[2] - xa[2];
     const double r1 = sqrt(dx * dx + dy * dy + dz * dz);
     const double gij = 0.5 * (ha + hb);
-    v = 1.0 / pow(pow(r1, 2.0) + pow(gij, -2.0), 0.5);
+    v = 1.0 / sqrt(r1 * r1 + 1.0 / (gij * gij));
   }
   J[md.j0 + t] = v;
 }
