@@ -712,8 +712,9 @@ void Batch::init_common() {
   if (const char* r = std::getenv("XTBAMD_APPLY_KERNEL")) apply_lds = std::strcmp(r, "lds") == 0;
   if (const char* r = std::getenv("XTBAMD_BANDED")) banded_passes = std::max(0, std::atoi(r));
   if (const char* r = std::getenv("XTBAMD_JAC_TOL")) jac_tol = std::atof(r);
-  jac_tol_mid = jac_tol;
+  jac_tol_mid = std::max(jac_tol, jac_tol_mid);
   if (const char* r = std::getenv("XTBAMD_JAC_TOL_MID")) jac_tol_mid = std::max(jac_tol, std::atof(r));
+  if (const char* r = std::getenv("XTBAMD_JAC_TOL_Q")) jac_tol_q = std::max(0.0, std::atof(r));
   if (const char* r = std::getenv("XTBAMD_JAC_CROSS")) jac_cross = std::atoi(r) != 0;
   // bound on the sweeps of one diagonalization; a problem still above the
   // tolerance after it fails its molecule
@@ -1316,7 +1317,7 @@ bool Batch::jacobi(JacSet& js, double* Am, double* Wm, double* evout, bool check
   // W = I over the whole device (k_jac_init only sets the scales and states)
   hipLaunchKernelGGL(k_jac_init, dim3(nb), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale, js.state,
                      check_active ? (const MolState*)d_state : (const MolState*)nullptr, js.offmax, 1,
-                     jac_tol_mid / jac_tol);
+                     jac_tol_mid / jac_tol, jac_tol_q, jac_tol);
   if (!keep_w)
     hipLaunchKernelGGL(k_jac_ident, dim3(gstride((long long)js.big_max_ld * js.big_max_ld), nb), dim3(256), 0, stream,
                        js.d_probs, Wm, check_active ? (const MolState*)d_state : (const MolState*)nullptr);
@@ -1831,7 +1832,7 @@ void Batch::eigh_rows(JacSet& js, double* Am, double* Wm, double* evout) {
   const MolState* st = d_state;
   // state and scale of the small problems (W is written whole below)
   hipLaunchKernelGGL(k_jac_init, dim3(nmol), dim3(256), 0, stream, js.d_probs, (const double*)Am, Wm, js.scale,
-                     js.state, st, js.offmax, 1, 1.0);
+                     js.state, st, js.offmax, 1, 1.0, 0.0, jac_tol);
   if (js.nsmall) {
     hipLaunchKernelGGL(k_jac_small, dim3(js.nsmall), dim3(256), 0, stream, js.d_probs, js.d_small, Am, Wm,
                        (const double*)js.scale, js.state, 0, 1, jac_tol);
@@ -2148,6 +2149,12 @@ void Batch::setup() {
                        at_ao0, selfe, intcut, (const int2*)pair_list, S, H0, H0n, D, Q);
   hipLaunchKernelGGL(k_sdq_onsite, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
                      at_ao0, selfe, intcut, S, H0, H0n, D, Q);
+  {
+    // the integral kernels wrote the lower triangles: mirror them
+    const int nt = (sub_max_nao + 31) / 32;
+    hipLaunchKernelGGL(k_sym_lower, dim3(std::max(1, nt * (nt + 1) / 2), ns, 12), dim3(256), 0, stream, d_sub, S, H0,
+                       H0n, D, Q);
+  }
   hipLaunchKernelGGL(k_coulomb, dim3(grid1((long long)sub_max_nsh * sub_max_nsh, 256), ns), dim3(256), 0, stream,
                      d_sub, el, at, xyz, sh_at, sh_m, J);
   hipLaunchKernelGGL(k_d4_weights, dim3(grid1(sub_max_nd4, 64), ns), dim3(64), 0, stream, d_sub, el, at, d4_at, d4_ref,

@@ -248,16 +248,17 @@ __global__ void __launch_bounds__(256) k_trd_col(const EigProb* __restrict__ pro
   double wvdot = 0.0;
   if (r < n) {
     const double* Ar = A + p.m_off + (long long)r * ld;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    // eight independent partial sums per lane: eight loads of the row in
+    // flight per lane (the product is bound by memory latency, not issue)
+    double sp[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sp[q] = 0.0;
     int c = j + 2 + lane;
-    for (; c + 192 < n; c += 256) {
-      s0 = fma(Ar[c], a[c], s0);
-      s1 = fma(Ar[c + 64], a[c + 64], s1);
-      s2 = fma(Ar[c + 128], a[c + 128], s2);
-      s3 = fma(Ar[c + 192], a[c + 192], s3);
-    }
-    for (; c < n; c += 64) s0 = fma(Ar[c], a[c], s0);
-    double s = scale * ((s0 + s1) + (s2 + s3));
+    for (; c + 448 < n; c += 512)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) sp[q] = fma(Ar[c + 64 * q], a[c + 64 * q], sp[q]);
+    for (; c < n; c += 64) sp[0] = fma(Ar[c], a[c], sp[0]);
+    double s = scale * (((sp[0] + sp[1]) + (sp[2] + sp[3])) + ((sp[4] + sp[5]) + (sp[6] + sp[7])));
     if (lane == 0) s += Ar[j + 1];   // v[j+1] = 1
     // corrections: lane t < i takes W(r, t) y1[t], lane i + t takes V(r, t) y2[t]
     if (lane < 2 * i) {
@@ -316,26 +317,36 @@ __global__ void __launch_bounds__(256) k_trd_row(const EigProb* __restrict__ pro
   }
   if (next < 0 || next >= n) return;
   const int jn = next, ni = jn - j0;   // pairs 0..ni-1 correct the row (ni = i + 1 after a finish)
-  if (tid < ni) {
-    cw[tid] = Xp[(long long)tid * ld + jn];   // v_t[jn]
+  // all kEigNb pairs take part with fixed trip counts, so that every load is
+  // issued before the sums need it: the panel rows t >= ni are still zero
+  // (cleared at the panel start) and so are their coefficients here
+  if (tid < kEigNb) {
+    const bool on = tid < ni;
+    cw[tid] = on ? Xp[(long long)tid * ld + jn] : 0.0;   // v_t[jn]
     // w_t[jn]; the pair just finished is completed here, not yet visible in X
-    cw[kEigNb + tid] = tid == i ? fma(alpha, Xp[(long long)tid * ld + jn], V[(long long)kVw * ld + jn])
-                                : Xp[(long long)(kEigNb + tid) * ld + jn];
+    cw[kEigNb + tid] = !on ? 0.0
+                       : tid == i ? fma(alpha, Xp[(long long)tid * ld + jn], V[(long long)kVw * ld + jn])
+                                  : Xp[(long long)(kEigNb + tid) * ld + jn];
   }
   __syncthreads();
   double ar = 0.0;
   if (r < n) {
     ar = A[p.m_off + (long long)jn * ld + r];
-    for (int t = 0; t < ni; ++t) {
-      const double vt = Xp[(long long)t * ld + r];
-      const double wt = t == i ? wi : Xp[(long long)(kEigNb + t) * ld + r];
-      ar -= cw[t] * wt + cw[kEigNb + t] * vt;
+    double vt[kEigNb], wt[kEigNb];
+#pragma unroll
+    for (int t = 0; t < kEigNb; ++t) {
+      vt[t] = Xp[(long long)t * ld + r];
+      wt[t] = t == i ? wi : Xp[(long long)(kEigNb + t) * ld + r];
     }
+#pragma unroll
+    for (int t = 0; t < kEigNb; ++t) ar -= cw[t] * wt[t] + cw[kEigNb + t] * vt[t];
     V[(long long)kVy * ld + r] = ar;
     if (jn == n - 1) V[(long long)kVd * ld + jn] = ar;
   }
-  // partial sums over rows jn + 2..n-1: |a|^2, v_t . a, w_t . a (t < ni);
-  // one wave per product, lanes over this workgroup's 256 rows
+  // partial sums over rows jn + 2..n-1: |a|^2, v_t . a, w_t . a (products
+  // k = 0, 1 + t, 1 + kEigNb + t; zero for t >= ni); wave wv takes the
+  // products k = wv (mod 4), each lane four of the workgroup's 256 rows, all
+  // loads of a wave issued before its sums
   const bool tail = r < n && r >= jn + 2;
   ra[tid] = tail ? ar : 0.0;
   rw[tid] = wi;
@@ -343,23 +354,33 @@ __global__ void __launch_bounds__(256) k_trd_row(const EigProb* __restrict__ pro
   const int lane = tid & 63, wv = tid >> 6;
   const int rows = min(256, n - r0);
   double* pr = part2 + p.p_off2 + (long long)blockIdx.x * kTrdPart;
-  for (int k = wv; k < kTrdPart; k += 4) {
-    const bool used = k == 0 || (k >= 1 && k < 1 + ni) || (k >= 1 + kEigNb && k < 1 + kEigNb + ni);
-    if (!used) {
-      if (lane == 0) pr[k] = 0.0;
-      continue;
-    }
+  constexpr int kPerWave = (kTrdPart + 3) / 4;
+  double au[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) au[q] = ra[lane + 64 * q];
+  double acc[kPerWave];
+#pragma unroll
+  for (int c = 0; c < kPerWave; ++c) {
+    const int k = wv + 4 * c;
     double x = 0.0;
-    for (int u = lane; u < rows; u += 64) {
-      const double au = ra[u];
-      double o;
-      if (k == 0) o = au;
-      else if (k < 1 + kEigNb) o = Xp[(long long)(k - 1) * ld + r0 + u];
-      else o = (k - 1 - kEigNb) == i ? rw[u] : Xp[(long long)(k - 1) * ld + r0 + u];
-      x = fma(o, au, x);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int u = lane + 64 * q;
+      double o = 0.0;
+      if (k < kTrdPart && u < rows) {
+        if (k == 0) o = au[q];
+        else if (k < 1 + kEigNb) o = Xp[(long long)(k - 1) * ld + r0 + u];
+        else o = (k - 1 - kEigNb) == i ? rw[u] : Xp[(long long)(k - 1) * ld + r0 + u];
+      }
+      x = fma(o, au[q], x);
     }
-    x = wave_sum(x);
-    if (lane == 0) pr[k] = x;
+    acc[c] = x;
+  }
+#pragma unroll
+  for (int c = 0; c < kPerWave; ++c) {
+    const int k = wv + 4 * c;
+    const double x = wave_sum(acc[c]);
+    if (lane == 0 && k < kTrdPart) pr[k] = x;
   }
 }
 

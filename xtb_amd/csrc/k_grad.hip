@@ -44,7 +44,10 @@ __device__ void grad_pair(const Element& ei, int is, const Element& ej, int js, 
     for (int cj = 0; cj < NCJ; ++cj) { wf[ci][cj] = 0.0; wp[ci][cj] = 0.0; }
   for (int ii = 0; ii < NSI; ++ii)
     for (int jj = 0; jj < NSJ; ++jj) {
-      const long long pos = (long long)(aoj + jj) * in.N + (aoi + ii);
+      // lower triangle (row of atom i, the later one): P, Pew, S and H0n are
+      // exactly symmetric, and the lanes of a wave (same i, consecutive j)
+      // then read neighbouring columns of one row instead of rows N apart
+      const long long pos = (long long)(aoi + ii) * in.N + (aoj + jj);
       const double pij = in.P[pos];
       const double hp = in.Hn[pos] * kEVtoAU * pij;  // H0 without overlap, in Eh
       const double fac = 2 * hp - 2 * in.Pew[pos] - pij * (in.ves_i + in.ves_j) + pij * (in.vs_i + in.vs_j);
@@ -218,7 +221,7 @@ __global__ void __launch_bounds__(64) k_grad_h0_pairs(const MolDesc* __restrict_
       double gx[3] = {0, 0, 0}, hs = 0.0, dcn = 0.0;
       for (int ii = 0; ii < 2 * li + 1; ++ii)
         for (int jj = 0; jj < 2 * lj + 1; ++jj) {
-          const long long pos = md.m0 + (long long)(aoj + jj) * N + (aoi + ii);
+          const long long pos = md.m0 + (long long)(aoi + ii) * N + (aoj + jj);   // lower triangle
           const double hp = H0n[pos] * kEVtoAU * P[pos];
           hs += 2 * hp * S[pos];
           dcn += hp / hav * S[pos];

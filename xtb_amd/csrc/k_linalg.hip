@@ -1370,13 +1370,13 @@ constexpr int kCrossPeriod = 8;
 
 template <bool CROSSK>
 __device__ __forceinline__ void jac_pivot_body(const JacobiProb* __restrict__ probs, const int2* __restrict__ list,
-                                               int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
+                                               int idx, int r, const double* __restrict__ A, double* Qbuf, int* rotflag,
                                                const double* __restrict__ scale_in, int* state, int inner_sweeps,
                                                unsigned long long* work, double tolrel, unsigned long long* offmax,
                                                int stride) {
   __shared__ double2 rcs[16];
   __shared__ double red[2];
-  const int2 it = list[blockIdx.x];
+  const int2 it = list[idx];
   if (it.x < 0) return;   // skip entry of an XCD-ordered list
   const JacobiProb pr = probs[it.x];
   int* st = state + it.x * kJacState;
@@ -1505,7 +1505,8 @@ k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list
              double* Qbuf, int* rotflag, const double* __restrict__ scale_in, int* state, int inner_sweeps,
              unsigned long long* work, double tolrel, unsigned long long* offmax, int stride, unsigned long long* ts) {
   prof_clock_start(ts);
-  jac_pivot_body<CROSSK>(probs, list, r, A, Qbuf, rotflag, scale_in, state, inner_sweeps, work, tolrel, offmax, stride);
+  jac_pivot_body<CROSSK>(probs, list, blockIdx.x, r, A, Qbuf, rotflag, scale_in, state, inner_sweeps, work, tolrel,
+                         offmax, stride);
   prof_clock_end(ts);
 }
 
@@ -1758,13 +1759,14 @@ __global__ void __launch_bounds__(64) k_jac_apply_R(const JacobiProb* __restrict
   prof_clock_end(ts);
 }
 
+
 // initialise W = I and the per-problem scale max|A_ii| and state, clear
 // both off-diagonal maximum buffers (offmax[0, 2 * nprob)); molecules that
 // finished their SCC are marked done and skipped
 __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__ probs, const double* __restrict__ A,
                                                   double* W, double* scale, int* state,
                                                   const MolState* __restrict__ mstate, unsigned long long* offmax,
-                                                  int keep_w, double loose) {
+                                                  int keep_w, double loose, double qrel, double tol) {
   __shared__ double sh[4];
   const JacobiProb pr = probs[blockIdx.x];
   int* st = state + blockIdx.x * kJacState;
@@ -1788,8 +1790,16 @@ __global__ void __launch_bounds__(256) k_jac_init(const JacobiProb* __restrict__
   m = block_max<256>(m, sh);
   if (threadIdx.x == 0) {
     // a molecule before its final diagonalization converges to a looser
-    // tolerance (loose x the final one)
-    scale[blockIdx.x] = fmax(m, 1.0) * ((mstate && !mstate[blockIdx.x].lastdiag) ? loose : 1.0);
+    // tolerance tied to its SCC progress: off-diagonal residuals of qrel x
+    // the last iteration's RMS charge change (between tol and loose x tol;
+    // the first iteration takes the loosest), so that the eigenvector error
+    // stays far below the change the next mixing step makes anyway
+    double f = 1.0;
+    if (mstate && !mstate[blockIdx.x].lastdiag) {
+      const MolState& ms = mstate[blockIdx.x];
+      f = ms.iter > 0 ? fmin(loose, fmax(1.0, qrel * ms.rmsq / tol)) : loose;
+    }
+    scale[blockIdx.x] = fmax(m, 1.0) * f;
     for (int i = 0; i < kJacState; ++i) st[i] = 0;
     offmax[blockIdx.x] = 0ull;
     offmax[gridDim.x + blockIdx.x] = 0ull;

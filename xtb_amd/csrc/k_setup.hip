@@ -151,18 +151,20 @@ __device__ void pair_block(const Element& ei, int is, const Element& ej, int js,
         for (int cj = 0; cj < NCJ; ++cj) s += sph_w(LJ, cj, jj) * acc[cj][k];
         o[k] = s;
       }
+      // row aoi + ii (shell i, the atom later in the order), column aoj + jj:
+      // the lower triangle only; k_sym_lower mirrors it afterwards with
+      // coalesced tile transposes (a thread's mirrored stores would hit a
+      // different cache line each, stride N)
       const long long p1 = (long long)(aoi + ii) * N + (aoj + jj);
-      const long long p2 = (long long)(aoj + jj) * N + (aoi + ii);
       if (H0) {
-        const double h = hav_sp * o[0];
-        H0[p1] = h; H0[p2] = h;
-        H0n[p1] = hav_sp; H0n[p2] = hav_sp;
-        S[p1] = o[0]; S[p2] = o[0];
+        H0[p1] = hav_sp * o[0];
+        H0n[p1] = hav_sp;
+        S[p1] = o[0];
       }
 #pragma unroll
-      for (int k = 0; k < 3; ++k) { D[k * NN + p1] = o[1 + k]; D[k * NN + p2] = o[1 + k]; }
+      for (int k = 0; k < 3; ++k) D[k * NN + p1] = o[1 + k];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) { Q[k * NN + p1] = o[4 + k]; Q[k * NN + p2] = o[4 + k]; }
+      for (int k = 0; k < 6; ++k) Q[k * NN + p1] = o[4 + k];
     }
   }
 }
@@ -285,6 +287,44 @@ __global__ void k_sdq_onsite(const MolDesc* __restrict__ desc, const Element* __
       aoj += 2 * e.l[js] + 1;
     }
     aoi += 2 * e.l[is] + 1;
+  }
+}
+
+// The 12 integral matrices (S, H0, H0n, D x3, Q x6) of each molecule from
+// their lower triangle: workgroup (tile pair, molecule, matrix) reads the
+// 32 x 32 tile (I, J), I >= J, row by row and writes its transpose to tile
+// (J, I) row by row through LDS (in a diagonal tile only the entries above
+// the diagonal).  Launched after k_sdqh0 and k_sdq_onsite.
+__global__ void __launch_bounds__(256) k_sym_lower(const MolDesc* __restrict__ desc, double* S, double* H0,
+                                                   double* H0n, double* D, double* Q) {
+  __shared__ double t[32][33];
+  const MolDesc md = desc[blockIdx.y];
+  const int N = md.nao, nt = (N + 31) / 32;
+  // tile pair index -> (I, J), I >= J, row-major over the lower triangle
+  const int pidx = blockIdx.x;
+  if (pidx >= nt * (nt + 1) / 2) return;
+  int I = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
+  while (I * (I + 1) / 2 > pidx) --I;
+  while ((I + 1) * (I + 2) / 2 <= pidx) ++I;
+  const int J = pidx - I * (I + 1) / 2;
+  const long long NN = (long long)N * N;
+  const int k = blockIdx.z;
+  double* M = k == 0 ? S + md.m0 : k == 1 ? H0 + md.m0 : k == 2 ? H0n + md.m0
+            : k < 6 ? D + 3 * md.m0 + (k - 3) * NN : Q + 6 * md.m0 + (k - 6) * NN;
+  const int c = threadIdx.x & 31, r8 = threadIdx.x >> 5;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = r8 + 8 * q, gr = 32 * I + r, gc = 32 * J + c;
+    t[r][c] = (gr < N && gc < N) ? M[(long long)gr * N + gc] : 0.0;
+  }
+  __syncthreads();
+  // element (row 32 J + rr, column 32 I + cc) of the upper tile takes the
+  // lower element (32 I + cc, 32 J + rr)
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int rr = r8 + 8 * q, cc = c;
+    const int gr = 32 * J + rr, gc = 32 * I + cc;
+    if (gr < N && gc < N && (I > J || cc > rr)) M[(long long)gr * N + gc] = t[cc][rr];
   }
 }
 
