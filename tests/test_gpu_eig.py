@@ -104,3 +104,30 @@ def test_blocked_eigensolver_matches_jacobi_and_oracle(monkeypatch, blk_min, ite
     assert np.max(np.abs(g["charges"] - o["q"])) < 1e-5
     assert g["iterations"] == o["iterations"]
 
+
+
+def test_blocked_solve_graph_replays_equal_eager(monkeypatch):
+    """The blocked solve captured once as a HIP graph and replayed in every
+    SCC iteration (eager Jacobi, mixing and host synchronisations between the
+    replays), against the same launches issued eagerly (XTBAMD_GRAPH=0):
+    bitwise the same results, on two runs of the same batch in a row."""
+    import xtb_amd
+    runs = []
+    for graph in ("0", "1"):
+        monkeypatch.setenv("XTBAMD_GRAPH", graph)
+        monkeypatch.setenv("XTBAMD_TUNING", "off")
+        monkeypatch.setenv("XTBAMD_EIG_BLK_MIN", "65")
+        mols = _mols()
+        b = xtb_amd.Batch([(z, x, 0.0, 0) for z, x in mols])
+        b.run()
+        first = b.results()
+        b.run()
+        runs.append((first, b.results()))
+        b.close()
+    (e1, e2), (g1, g2) = runs
+    for k in range(len(e1)):
+        for ra, rb in ((e1[k], g1[k]), (e2[k], g2[k]), (e1[k], g2[k])):
+            assert ra["iterations"] == rb["iterations"], k
+            assert ra["energy"] == rb["energy"], k
+            assert np.array_equal(ra["gradient"], rb["gradient"]), k
+            assert np.array_equal(ra["charges"], rb["charges"]), k

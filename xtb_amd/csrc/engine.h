@@ -189,6 +189,13 @@ struct EigSet {
   DcNode* d_nodes = nullptr;
   int nnodes = 0;
   std::vector<int> lvl0, lvln, lvlmax;   // per level: first node, node count, largest node
+  // the whole solve captured once as a HIP graph and replayed (use_graph):
+  // its launches and their arguments are fixed from the build on, so a
+  // replay is the eager sequence without the host's per-launch work; the
+  // graph is re-captured when the arguments of a call change
+  bool use_graph = false;
+  hipGraphExec_t graph = nullptr;
+  const void* graph_key[5] = {};
 };
 
 // Device memory, streams, events and a host word buffer lent to a Batch: a
@@ -375,6 +382,12 @@ class Batch {
   // the block-Jacobi rounds; register path only.  Off unless the tuning file
   // or XTBAMD_EIG_BLK_MIN sets it.
   int eig_blk_min = 1 << 30;
+  // the blocked solve as a captured HIP graph (XTBAMD_GRAPH=1); off by
+  // default: replaying it measured no faster than the eager launches (one
+  // 4200-AO single point 1.523 against 1.514 s, its launches are bound by
+  // their own latency, not by the host), and the profiler's kernel trace of
+  // a replayed graph of this size ended in a host fault
+  bool eig_graph = false;
   // SCC iterations (from the first) whose eigenproblems above eig_blk_min go
   // to the blocked eigensolver; later ones, with the previous orbitals as the
   // start, to the block-Jacobi lanes

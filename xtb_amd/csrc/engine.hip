@@ -705,6 +705,7 @@ void Batch::init_common() {
   if (tu.gemm_valu >= 0) use_mfma = tu.gemm_valu == 0;
   if (const char* g = std::getenv("XTBAMD_GEMM")) use_mfma = std::string(g) != "valu";
   if (const char* g = std::getenv("XTBAMD_MIXED")) mixed = std::atoi(g) != 0;
+  if (const char* g = std::getenv("XTBAMD_GRAPH")) eig_graph = std::atoi(g) != 0;
   if (const char* r = std::getenv("XTBAMD_REFINE_STEPS")) oa_max_steps = std::atoi(r);
   if (const char* r = std::getenv("XTBAMD_INNER_SWEEPS")) inner_sweeps = std::max(1, std::atoi(r));
   if (const char* r = std::getenv("XTBAMD_PIVOT")) jac_p = std::atoi(r) == 64 ? 64 : 32;
@@ -1281,6 +1282,7 @@ void Batch::set_solvent(const std::string& name, int kernel, bool alpb, int stat
 Batch::~Batch() {
   for (auto& v : prof_ev_)
     for (hipEvent_t e : v) (void)hipEventDestroy(e);
+  if (eig_.graph) (void)hipGraphExecDestroy(eig_.graph);
   if (on_loan()) {
     // the lender owns the memory, streams and events; only wait for the work
     (void)hipStreamSynchronize(stream);
@@ -1447,9 +1449,16 @@ void Batch::eig_build() {
   std::vector<int> dims;
   for (int m : eig_.mols) dims.push_back(desc[m].nao);
   eig_set_build(eig_, dims, mem);
+  // captured on stream3 (the null stream of a loan cannot be captured)
+  eig_.use_graph = eig_graph && !on_loan();
 }
 
 void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
+  if (es.graph) {
+    // the captured launches name the old work arrays
+    HIPCHECK(hipGraphExecDestroy(es.graph));
+    es.graph = nullptr;
+  }
   es.np = (int)es.mols.size();
   if (es.np == 0) return;
   auto grow = [&](auto*& ptr, long long& capv, long long need) {
@@ -1666,8 +1675,33 @@ void Batch::eig_blocked(const JacSet& js, double* Am, double* Wm, bool check_act
               s);
 }
 
+static void eig_set_launch(EigSet& es, const JacobiProb* jprobs, int* jstate, const MolState* mstate, double* Am,
+                           double* Wm, hipStream_t s);
+
 void eig_set_run(EigSet& es, const JacobiProb* jprobs, int* jstate, const MolState* mstate, double* Am, double* Wm,
                  hipStream_t s) {
+  if (!es.use_graph || es.np == 0) {
+    eig_set_launch(es, jprobs, jstate, mstate, Am, Wm, s);
+    return;
+  }
+  const void* key[5] = {jprobs, jstate, mstate, Am, Wm};
+  if (!es.graph || !std::equal(key, key + 5, es.graph_key)) {
+    if (es.graph) HIPCHECK(hipGraphExecDestroy(es.graph));
+    es.graph = nullptr;
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    eig_set_launch(es, jprobs, jstate, mstate, Am, Wm, s);
+    HIPCHECK(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&es.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHECK(e);
+    std::copy(key, key + 5, es.graph_key);
+  }
+  HIPCHECK(hipGraphLaunch(es.graph, s));
+}
+
+static void eig_set_launch(EigSet& es, const JacobiProb* jprobs, int* jstate, const MolState* mstate, double* Am,
+                           double* Wm, hipStream_t s) {
   const int np = es.np;
   auto gemm_launch = [&](const EigSet::Launch& L, bool ta, bool tb) {
     if (L.ntile == 0) return;

@@ -3,15 +3,17 @@
 // src/mctc/lapack/eigensolve.F90:218-246, called from src/scc_core.f90:475):
 //
 //  1. Householder tridiagonalisation in panels of kEigNb columns.  Every
-//     column is one launch of k_trd_symv (w = A22 v over the trailing matrix,
-//     one wave per row, coalesced row reads) and one of k_trd_fin (one
-//     workgroup per problem: the panel corrections of w, its scaling, and the
-//     next column's reflector); a panel ends with the rank-2 nb update
-//     A22 -= X^T Y of the trailing matrix on the matrix cores (k_bgemm).
+//     column is one launch of k_trd_col (the column's reflector, formed by
+//     every workgroup from the partial sums the launch before left, and
+//     w = A22 v with its panel corrections: one wave per trailing row,
+//     coalesced row reads) and one of k_trd_row (w completed, the next
+//     column's corrected row and its partial sums, 256 rows per workgroup);
+//     a panel ends with the rank-2 nb update A22 -= X^T Y of the trailing
+//     matrix on the matrix cores (k_bgemm).
 //  2. Divide and conquer on the tridiagonal matrix: leaves of at most 64
 //     solved whole in LDS, merges level by level with deflation (one
 //     workgroup per merge), the secular equation's roots by safeguarded
-//     Newton steps (one thread per root), eigenvectors from the recomputed
+//     Newton steps (one wave per root), eigenvectors from the recomputed
 //     z of Gu and Eisenstat (SIAM J. Matrix Anal. Appl. 16 (1995) 172) and
 //     one matrix-core product per merge.
 //  3. Back-transformation of the eigenvectors with the compact WY form
@@ -187,8 +189,21 @@ __device__ __forceinline__ void trd_reflector(const double* __restrict__ a, cons
                                               const double* __restrict__ Xp, int ld, int n, int j, int i,
                                               double& beta, double& tau, double& scale, double* y) {
   const int nw = (n - j + 255) / 256;   // k_trd_row workgroups over rows j..n-1
-  double ss = 0.0;
-  for (int w = 0; w < nw; ++w) ss += part[w * kTrdPart];
+  // the partial sums with four loads in flight (a loop that adds each load
+  // before the next is issued waits nw memory latencies)
+  auto psum = [&](int k) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int w = 0;
+    for (; w + 3 < nw; w += 4) {
+      s0 += part[w * kTrdPart + k];
+      s1 += part[(w + 1) * kTrdPart + k];
+      s2 += part[(w + 2) * kTrdPart + k];
+      s3 += part[(w + 3) * kTrdPart + k];
+    }
+    for (; w < nw; ++w) s0 += part[w * kTrdPart + k];
+    return (s0 + s1) + (s2 + s3);
+  };
+  const double ss = psum(0);
   const double alpha = a[j + 1];
   beta = alpha;
   tau = 0.0;
@@ -201,9 +216,7 @@ __device__ __forceinline__ void trd_reflector(const double* __restrict__ a, cons
   for (int t = threadIdx.x; t < 2 * i; t += blockDim.x) {
     const int tt = t < i ? t : t - i;
     const int row = t < i ? tt : kEigNb + tt;   // v rows, then w rows
-    double s = 0.0;
-    for (int w = 0; w < nw; ++w) s += part[w * kTrdPart + 1 + (t < i ? 0 : kEigNb) + tt];
-    y[t] = Xp[(long long)row * ld + j + 1] + scale * s;
+    y[t] = Xp[(long long)row * ld + j + 1] + scale * psum(1 + (t < i ? 0 : kEigNb) + tt);
   }
   __syncthreads();
 }
@@ -254,6 +267,7 @@ __global__ void __launch_bounds__(256) k_trd_col(const EigProb* __restrict__ pro
 #pragma unroll
     for (int q = 0; q < 8; ++q) sp[q] = 0.0;
     int c = j + 2 + lane;
+#pragma unroll 2
     for (; c + 448 < n; c += 512)
 #pragma unroll
       for (int q = 0; q < 8; ++q) sp[q] = fma(Ar[c + 64 * q], a[c + 64 * q], sp[q]);
@@ -305,9 +319,17 @@ __global__ void __launch_bounds__(256) k_trd_row(const EigProb* __restrict__ pro
     const int j = finish;
     i = j - j0;
     const int nw = (n - 1 - j + 3) / 4;   // k_trd_col workgroups
-    double s = 0.0;
-    for (int w = tid; w < nw; w += 256) s += V[(long long)kVrc * ld + w];
-    s = sum256(s, sh);
+    const double* pw = V + (long long)kVrc * ld;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int w = tid;
+    for (; w + 768 < nw; w += 1024) {
+      s0 += pw[w];
+      s1 += pw[w + 256];
+      s2 += pw[w + 512];
+      s3 += pw[w + 768];
+    }
+    for (; w < nw; w += 256) s0 += pw[w];
+    const double s = sum256((s0 + s1) + (s2 + s3), sh);
     alpha = -0.5 * V[(long long)kVtau * ld + j] * s;
     if (r < n && r > j) {
       wi = fma(alpha, Xp[(long long)i * ld + r], V[(long long)kVw * ld + r]);
