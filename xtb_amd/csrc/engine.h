@@ -33,6 +33,7 @@ struct MolDesc {
   int nbr, nel, nopen, maxiter;
   double charge;
   int mol;                      // index of the molecule (slot) in the batch
+  int npsp;                     // leading pairs of the pair list with s and p shells only
   long long y0;                 // offset of the EEQ vectors (n+1)
 };
 
@@ -172,6 +173,8 @@ struct EigSet {
   double *X = nullptr, *P = nullptr, *P2 = nullptr, *Tf = nullptr, *vec = nullptr;
   int* ivec = nullptr;
   double* part2 = nullptr;    // partial sums of the tridiagonalisation's row kernel
+  double* G = nullptr;        // Gram matrices of the back-transformation's reflector blocks
+  int nblk = 0;               // back-transformation blocks of the largest problem
   long long cap[20] = {};     // capacities (elements) of the arrays above and below
   long long nmat = 0, nx = 0;   // elements in use of the n x ld arrays and of X
   // product launches: jobs [job0, job0 + njob), tiles [tile0, tile0 + ntile)
@@ -179,7 +182,8 @@ struct EigSet {
     int job0 = 0, njob = 0;
     long long tile0 = 0, ntile = 0;
   };
-  std::vector<Launch> upd, bt1, bt2, bt3, dcg;   // per panel (upd, bt*), per tree level (dcg)
+  std::vector<Launch> upd, bt1, bt2, bt3, dcg;   // per panel (upd), per block (bt*), per tree level (dcg)
+  Launch gram;                                   // the Gram matrices of every block
   GemmJob* d_jobs = nullptr;
   int2* d_tiles = nullptr;
   // divide and conquer: leaves (prob, start, size, -), all merges (for the
@@ -300,6 +304,7 @@ class Batch {
   MolDesc* d_sub = nullptr;     // descriptors of the selected molecules
   int nsub = 0;
   int sub_max_n = 0, sub_max_nsh = 0, sub_max_nao = 0, sub_max_nd4 = 0;
+  long long sub_max_psp = 0, sub_max_pd = 0;   // largest s/p-only and d-shell parts of the pair lists
   int* phase_act = nullptr;     // 1 for selected molecules (per slot)
   int* open_act = nullptr;      // 1 for selected open-shell molecules (per slot)
   int nsub_open = 0;

@@ -588,23 +588,45 @@ void Batch::describe(const MoleculeInput& mi, int maxiter, MolDesc& d, HostMol& 
     zsum += e.zval;
     for (int k = 0; k < 3; ++k) hm.xyz.push_back(mi.xyz[3 * i + k]);
   }
-  // atom pairs i > j grouped by element pair (groups in the order of their
-  // first pair in lower-triangle order, lower-triangle order inside a group;
-  // a counting sort, linear in the number of pairs)
+  // atom pairs i > j grouped by element pair: first the groups of two
+  // elements with s and p shells only, then those with a d shell (each part
+  // in the order of the groups' first pairs in lower-triangle order,
+  // lower-triangle order inside a group; a counting sort, linear in the
+  // number of pairs).  The pair kernels take the first npsp pairs with code
+  // for s and p shells only (far fewer registers) and the rest apart.
   {
+    const Element* elh = host_elements();
+    auto has_d = [&](int z) {
+      for (int s = 0; s < elh[z].nsh; ++s)
+        if (elh[z].l[s] == 2) return true;
+      return false;
+    };
     std::vector<int> group(((size_t)XTB_MAX_Z + 1) * (XTB_MAX_Z + 1), -1);
     std::vector<long long> count;
+    std::vector<char> gd;
     for (int i = 1; i < d.n; ++i)
       for (int j = 0; j < i; ++j) {
         int& g = group[(size_t)hm.at[i] * (XTB_MAX_Z + 1) + hm.at[j]];
         if (g < 0) {
           g = (int)count.size();
           count.push_back(0);
+          gd.push_back(has_d(hm.at[i]) || has_d(hm.at[j]));
         }
         count[g] += 1;
       }
     std::vector<long long> start(count.size(), 0);
-    for (size_t g = 1; g < count.size(); ++g) start[g] = start[g - 1] + count[g - 1];
+    long long off = 0;
+    for (size_t g = 0; g < count.size(); ++g)
+      if (!gd[g]) {
+        start[g] = off;
+        off += count[g];
+      }
+    d.npsp = (int)off;
+    for (size_t g = 0; g < count.size(); ++g)
+      if (gd[g]) {
+        start[g] = off;
+        off += count[g];
+      }
     hm.pairs.resize((size_t)d.n * (d.n - 1) / 2);
     for (int i = 1; i < d.n; ++i)
       for (int j = 0; j < i; ++j) {
@@ -1472,7 +1494,7 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
     ptr = (T_*)mem.get((size_t)std::max(1LL, capv) * sizeof(T_));
   };
   std::vector<EigProb> probs(es.np);
-  long long mat = 0, vv = 0, iv = 0, xx = 0, pp = 0, pp2 = 0, tt = 0;
+  long long mat = 0, vv = 0, iv = 0, xx = 0, pp = 0, pp2 = 0, tt = 0, gg = 0;
   es.v_off.clear();
   es.ld.clear();
   es.maxn = 0;
@@ -1493,14 +1515,16 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
     e.p_off = pp;
     e.p_off2 = pp2;
     e.t_off = tt;
+    e.g_off = gg;
     pp2 += (long long)((e.ld + 255) / 256) * kTrdPart;
-    const int npan_e = (e.n - 1 + kEigNb - 1) / kEigNb;
-    tt += (long long)std::max(1, npan_e) * kEigNb * e.nchunk * kEigNb;
+    const int nblk_e = std::max(1, (e.n - 1 + kBtNb - 1) / kBtNb);
+    tt += (long long)nblk_e * kBtNb * e.nchunk * kBtNb;
+    gg += (long long)nblk_e * kBtNb * kBtNb;
     mat += (long long)e.n * e.ld;
     vv += (long long)kEigVecs * e.ld;
     iv += (long long)kEigInts * e.ld;
     xx += 4LL * kEigNb * e.ld;
-    pp += (long long)e.n * kEigNb * e.nchunk;
+    pp += (long long)e.n * kBtNb * e.nchunk;
     int depth = 0;
     while (((e.n + (1 << depth) - 1) >> depth) > kDcLeaf) ++depth;
     e.depth = depth;
@@ -1508,6 +1532,7 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
     es.maxdepth = std::max(es.maxdepth, depth);
   }
   es.npanel = (es.maxn - 1 + kEigNb - 1) / kEigNb;
+  es.nblk = (es.maxn - 1 + kBtNb - 1) / kBtNb;
   grow(es.Aw, es.cap[0], mat);
   grow(es.Vh, es.cap[1], mat);
   grow(es.Z0, es.cap[2], mat);
@@ -1523,6 +1548,7 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
   grow(es.act, es.cap[12], es.np);
   grow(es.zf, es.cap[13], es.np);
   grow(es.part2, es.cap[18], pp2);
+  grow(es.G, es.cap[19], gg);
   es.nmat = mat;
   es.nx = xx;
   // divide and conquer trees: every leaf at the problem's depth
@@ -1601,41 +1627,61 @@ void eig_set_build(EigSet& es, const std::vector<int>& dims, EigMem& mem) {
   es.upd.clear(); es.bt1.clear(); es.bt2.clear(); es.bt3.clear(); es.dcg.clear();
   for (int pi = 0; pi < es.npanel; ++pi) {
     const int j0 = pi * kEigNb;
-    std::vector<GemmJob> u, b1, b2, b3;
+    std::vector<GemmJob> u;
     for (int k = 0; k < es.np; ++k) {
       const EigProb& e = probs[k];
       const int n = e.n, ld = e.ld;
       if (j0 >= n - 1) continue;
-      const int jb = std::min(kEigNb, n - 1 - j0);
-      const int* act = es.act + k;
       double* Xp = es.X + e.x_off;
       double* Yp = Xp + 2LL * kEigNb * ld;
       const int s0 = j0 + kEigNb;
       // trailing update A22 -= X^T Y on [s0, n) x [s0, n)
       if (s0 <= n - 1)
         u.push_back(job(n - s0, n - s0, 2 * kEigNb, Xp + s0, ld, Yp + s0, ld, es.Aw + e.m_off + (long long)s0 * ld + s0,
-                        ld, -1.0, 1.0, act));
-      // back-transformation of the eigenvector rows Z with this panel
-      double* Zf = (es.zfin[k] ? es.Z1 : es.Z0) + e.m_off;
-      const int k0 = j0 + 1;
-      const double* Vp = es.Vh + e.m_off + (long long)j0 * ld + k0;
-      double* P = es.P + e.p_off;
-      double* P2 = es.P2 + e.p_off;
-      const int tw = e.nchunk * kEigNb;
-      const double* T = es.Tf + e.t_off + (long long)pi * kEigNb * tw;
-      // P = Z V in K chunks (column block c of P), P2 = [P_1 .. P_C] [T .. T]^T, Z -= P2 V^T
-      for (int c = 0; c < e.nchunk; ++c) {
-        const int ks = k0 + c * kBtChunk, kl = std::max(0, std::min(kBtChunk, n - ks));
-        b1.push_back(job(n, jb, kl, Zf + std::min(ks, n - 1), ld, Vp + std::min(ks - k0, n - 1 - k0), ld,
-                         P + c * kEigNb, tw, 1.0, 0.0, act));
-      }
-      b2.push_back(job(n, jb, tw, P, tw, T, tw, P2, kEigNb, 1.0, 0.0, act));
-      b3.push_back(job(n, n - k0, jb, P2, kEigNb, Vp, ld, Zf + k0, ld, -1.0, 1.0, act));
+                        ld, -1.0, 1.0, es.act + k));
     }
     add(es.upd, u);
-    add(es.bt1, b1);
-    add(es.bt2, b2);
-    add(es.bt3, b3);
+  }
+  // back-transformation of the eigenvector rows Z, blocks of kBtNb reflectors
+  // (rows j0.. of Vh, each zero before its own start and 1 there): the Gram
+  // matrices G = V V^T of all blocks in one launch, then per block
+  // P = Z V^T in K chunks (column block c of P), P2 = [P_1 .. P_C] [T .. T]^T,
+  // Z -= P2 V
+  {
+    std::vector<GemmJob> gr;
+    for (int bi = 0; bi < es.nblk; ++bi) {
+      const int j0 = bi * kBtNb;
+      std::vector<GemmJob> b1, b2, b3;
+      for (int k = 0; k < es.np; ++k) {
+        const EigProb& e = probs[k];
+        const int n = e.n, ld = e.ld;
+        if (j0 >= n - 1) continue;
+        const int jb = std::min(kBtNb, n - 1 - j0);
+        const int* act = es.act + k;
+        double* Zf = (es.zfin[k] ? es.Z1 : es.Z0) + e.m_off;
+        const int k0 = j0 + 1;
+        const double* Vp = es.Vh + e.m_off + (long long)j0 * ld + k0;
+        double* P = es.P + e.p_off;
+        double* P2 = es.P2 + e.p_off;
+        const int tw = e.nchunk * kBtNb;
+        const double* T = es.Tf + e.t_off + (long long)bi * kBtNb * tw;
+        gr.push_back(job(jb, jb, n - k0, Vp, ld, Vp, ld, es.G + e.g_off + (long long)bi * kBtNb * kBtNb, kBtNb, 1.0,
+                         0.0, act));
+        for (int c = 0; c < e.nchunk; ++c) {
+          const int ks = k0 + c * kBtChunk, kl = std::max(0, std::min(kBtChunk, n - ks));
+          b1.push_back(job(n, jb, kl, Zf + std::min(ks, n - 1), ld, Vp + std::min(ks - k0, n - 1 - k0), ld,
+                           P + c * kBtNb, tw, 1.0, 0.0, act));
+        }
+        b2.push_back(job(n, jb, tw, P, tw, T, tw, P2, kBtNb, 1.0, 0.0, act));
+        b3.push_back(job(n, n - k0, jb, P2, kBtNb, Vp, ld, Zf + k0, ld, -1.0, 1.0, act));
+      }
+      add(es.bt1, b1);
+      add(es.bt2, b2);
+      add(es.bt3, b3);
+    }
+    std::vector<EigSet::Launch> g1;
+    add(g1, gr);
+    es.gram = g1[0];
   }
   for (int h = 1; h <= es.maxdepth; ++h) {
     std::vector<GemmJob> g;
@@ -1765,13 +1811,14 @@ static void eig_set_launch(EigSet& es, const JacobiProb* jprobs, int* jstate, co
     hipLaunchKernelGGL(k_dc_lam, dim3(grid1(mm, 256), nl), dim3(256), 0, s, (const EigProb*)es.d_probs,
                        (const int*)es.act, nd, es.vec);
   }
-  // 3. back-transformation, last panel first
-  hipLaunchKernelGGL(k_trd_tfactor, dim3(std::max(1, es.npanel), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
-                     (const int*)es.act, (const double*)es.Vh, (const double*)es.vec, es.Tf);
-  for (int pi = es.npanel - 1; pi >= 0; --pi) {
-    gemm_launch(es.bt1[pi], false, true);
-    gemm_launch(es.bt2[pi], false, true);
-    gemm_launch(es.bt3[pi], false, false);
+  // 3. back-transformation, last block first
+  gemm_launch(es.gram, false, true);
+  hipLaunchKernelGGL(k_trd_tblock, dim3(std::max(1, es.nblk), np), dim3(256), 0, s, (const EigProb*)es.d_probs,
+                     (const int*)es.act, (const double*)es.G, (const double*)es.vec, es.Tf);
+  for (int bi = es.nblk - 1; bi >= 0; --bi) {
+    gemm_launch(es.bt1[bi], false, true);
+    gemm_launch(es.bt2[bi], false, true);
+    gemm_launch(es.bt3[bi], false, false);
   }
   hipLaunchKernelGGL(k_eig_out, dim3(gstride(nn), np), dim3(256), 0, s, (const EigProb*)es.d_probs, jprobs,
                      (const int*)es.act, (const double*)es.vec, (const double*)es.Z0, (const double*)es.Z1,
@@ -2100,11 +2147,14 @@ void Batch::select(const std::vector<int>& mols) {
   HIPCHECK(hipMemcpyAsync(open_act, flag.data() + nmol, nmol * sizeof(int), hipMemcpyHostToDevice, stream));
   HIPCHECK(hipStreamSynchronize(stream));
   sub_max_n = sub_max_nsh = sub_max_nao = sub_max_nd4 = 0;
+  sub_max_psp = sub_max_pd = 0;
   for (const MolDesc& d : sub) {
     sub_max_n = std::max(sub_max_n, d.n);
     sub_max_nsh = std::max(sub_max_nsh, d.nsh);
     sub_max_nao = std::max(sub_max_nao, d.nao);
     sub_max_nd4 = std::max(sub_max_nd4, d.nd4);
+    sub_max_psp = std::max<long long>(sub_max_psp, d.npsp);
+    sub_max_pd = std::max<long long>(sub_max_pd, (long long)d.n * (d.n - 1) / 2 - d.npsp);
   }
 }
 
@@ -2177,9 +2227,12 @@ void Batch::setup() {
     HIPCHECK(hipEventRecord(ev_fork, stream));
     HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
   }
-  const long long maxpairs = (long long)sub_max_n * (sub_max_n - 1) / 2;
-  if (maxpairs > 0)
-    hipLaunchKernelGGL(k_sdqh0, dim3(grid1(maxpairs, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
+  // the s/p-only pairs, then the pairs with a d shell
+  if (sub_max_psp > 0)
+    hipLaunchKernelGGL(k_sdqh0<1>, dim3(grid1(sub_max_psp, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
+                       at_ao0, selfe, intcut, (const int2*)pair_list, S, H0, H0n, D, Q);
+  if (sub_max_pd > 0)
+    hipLaunchKernelGGL(k_sdqh0<2>, dim3(grid1(sub_max_pd, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
                        at_ao0, selfe, intcut, (const int2*)pair_list, S, H0, H0n, D, Q);
   hipLaunchKernelGGL(k_sdq_onsite, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
                      at_ao0, selfe, intcut, S, H0, H0n, D, Q);
@@ -2399,14 +2452,20 @@ void Batch::gradient() {
   // and D4 gradients; its per-atom sums follow them
   HIPCHECK(hipEventRecord(ev_fork, stream));
   HIPCHECK(hipStreamWaitEvent(stream2, ev_fork, 0));
-  hipLaunchKernelGGL(k_grad_h0_pairs, dim3(grid1((long long)sub_max_n * (sub_max_n - 1) / 2, 64), ns), dim3(64), 0,
-                     stream2, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)selfe, (const double*)dsedcn,
-                     (const double*)P, (const double*)T, (const double*)H0n, (const double*)S, (const double*)shift,
-                     (const double*)vs, (const double*)vd, (const double*)vq, intcut, (const int2*)pair_list, gpair);
+  if (sub_max_psp > 0)
+    hipLaunchKernelGGL(k_grad_h0_pairs<1>, dim3(grid1(sub_max_psp, 64), ns), dim3(64), 0, stream2, d_sub, el, at, xyz,
+                       at_sh0, at_ao0, (const double*)selfe, (const double*)dsedcn, (const double*)P,
+                       (const double*)T, (const double*)H0n, (const double*)S, (const double*)shift, (const double*)vs,
+                       (const double*)vd, (const double*)vq, intcut, (const int2*)pair_list, gpair);
+  if (sub_max_pd > 0)
+    hipLaunchKernelGGL(k_grad_h0_pairs<2>, dim3(grid1(sub_max_pd, 64), ns), dim3(64), 0, stream2, d_sub, el, at, xyz,
+                       at_sh0, at_ao0, (const double*)selfe, (const double*)dsedcn, (const double*)P,
+                       (const double*)T, (const double*)H0n, (const double*)S, (const double*)shift, (const double*)vs,
+                       (const double*)vd, (const double*)vq, intcut, (const int2*)pair_list, gpair);
   hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_sub, xyz, q, dipm, qp, radcn, grad, tmp2);
   hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, cn_cov, q,
                      zetavec, zerovec, zetadcn, zerodcn);
-  hipLaunchKernelGGL(k_d4_c6, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub, el,
+  hipLaunchKernelGGL(k_d4_c6, dim3(gstride((long long)sub_max_n * (sub_max_n + 1) / 2), ns), dim3(256), 0, stream, d_sub, el,
                      at, c6ref, zetavec, zerovec, zetadcn, zerodcn, c6, dc6, c60, dc60);
   hipLaunchKernelGGL(k_d4_grad2, gat, dim3(64), 0, stream, d_sub, el, at, xyz, (const double*)c6, (const double*)dc6,
                      grad, dedcn);

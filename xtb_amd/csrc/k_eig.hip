@@ -17,7 +17,9 @@
 //     z of Gu and Eisenstat (SIAM J. Matrix Anal. Appl. 16 (1995) 172) and
 //     one matrix-core product per merge.
 //  3. Back-transformation of the eigenvectors with the compact WY form
-//     I - V T V^T of each panel (three matrix-core products per panel).
+//     I - V T V^T of blocks of kBtNb reflectors (the Gram matrices of all
+//     blocks in one product launch, T by the dlarft recurrence, then three
+//     matrix-core products per block).
 //
 // Layouts: every n x n array of a problem is row-major with row stride ld;
 // the Householder vector of column j is row j of Vh (entries j+1..n-1, a 1 at
@@ -41,6 +43,7 @@ constexpr int kVd = 0, kVe = 1, kVtau = 2, kVw = 3, kVlam = 4, kVlam2 = 5, kVdn 
 constexpr int kIperm = 0, kIkeep = 1, kIpos = 2, kIorg = 3, kIrot = 4, kIrot2 = 5, kIrow = 6, kIcnt = 7;
 constexpr int kEigInts = 8;
 constexpr int kBtChunk = 512;   // K chunk of the back-transformation's product Z V
+constexpr int kBtNb = 128;      // reflectors per block of the back-transformation (four panels)
 
 // One batched product job: C = beta C + alpha op(A) op(B), op(A) M x K,
 // op(B) K x N, row-major with the given row strides.  TA: A is stored K x M;
@@ -71,7 +74,8 @@ struct EigProb {
   long long x_off;   // this problem's panel buffers X, Y (2 kEigNb x ld each)
   long long p_off;   // this problem's product buffers of the back-transformation
   long long p_off2;  // this problem's k_trd_row partial sums (ceil(n / 256) x kTrdPart)
-  long long t_off;   // this problem's T factors (npanel x kEigNb x nchunk kEigNb: T repeated nchunk times)
+  long long t_off;   // this problem's T factors (nblk x kBtNb x nchunk kBtNb: T repeated nchunk times)
+  long long g_off;   // this problem's Gram matrices of the reflector blocks (nblk x kBtNb x kBtNb)
   int nchunk;        // K chunks of the back-transformation's first product
   int pad2_;
   int mol;           // batch slot (index of its Jacobi problem)
@@ -843,50 +847,53 @@ __global__ void k_dc_lam(const EigProb* __restrict__ probs, const int* __restric
 
 // T factor of each panel's compact WY form (I - V T V^T = H_j0 ... H_j0+nb-1,
 // forward, columnwise): T_ii = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) (V^T v_i)
-__global__ void __launch_bounds__(256) k_trd_tfactor(const EigProb* __restrict__ probs, const int* __restrict__ act,
-                                                     const double* __restrict__ Vh, const double* __restrict__ vec,
-                                                     double* Tf) {
-  __shared__ double g[kEigNb][kEigNb + 1];
-  __shared__ double T[kEigNb][kEigNb + 1];
+// compact WY factor of the reflectors [j0, j0 + jb) of one back-transformation
+// block (j0 = blk kBtNb, jb <= kBtNb): T upper triangular with T(i, i) = tau_i
+// and T(a, i) = -tau_i sum_{a <= c < i} T(a, c) G(c, i) (the recurrence of
+// LAPACK's dlarft, forward and columnwise), G = V V^T the block's Gram matrix
+// (a product launch before this one).  Thread a owns row a of T, so each
+// thread reads back only its own writes; column i of G goes through LDS.  T
+// is written once per K chunk of the first product (the chunk products are
+// summed by the product with this wide T).
+__global__ void __launch_bounds__(256) k_trd_tblock(const EigProb* __restrict__ probs, const int* __restrict__ act,
+                                                    const double* __restrict__ G, const double* __restrict__ vec,
+                                                    double* Tf) {
+  __shared__ double g[kBtNb];
   if (!act[blockIdx.y]) return;
-  const int pan = blockIdx.x;
+  const int blk = blockIdx.x;
   const EigProb p = probs[blockIdx.y];
   const int n = p.n, ld = p.ld;
-  const int j0 = pan * kEigNb;
-  const int tw = p.nchunk * kEigNb;   // row stride of the repeated T
-  double* To = Tf + p.t_off + (long long)pan * kEigNb * tw;
+  const int j0 = blk * kBtNb;
   if (j0 >= n - 1) return;
-  const int jb = min(kEigNb, n - 1 - j0);
-  const double* tau = vec + p.v_off + (long long)kVtau * ld;
-  const double* Vp = Vh + p.m_off + (long long)j0 * ld;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // Gram matrix of the panel's vectors (upper part), one wave per entry
-  for (int x = wv; x < jb * jb; x += 4) {
-    const int a = x / jb, b = x % jb;
-    if (a >= b) continue;
-    // v_a is zero before row j0 + a + 1 (and 1 there), v_b before j0 + b + 1
-    double s = 0.0;
-    for (int r = j0 + b + 1 + lane; r < n; r += 64) s = fma(Vp[(long long)a * ld + r], Vp[(long long)b * ld + r], s);
-    s = wave_sum(s);
-    if (lane == 0) g[a][b] = s;
-  }
-  for (int x = threadIdx.x; x < kEigNb * kEigNb; x += blockDim.x) T[x / kEigNb][x % kEigNb] = 0.0;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < jb; ++i) {
-      const double ti = tau[j0 + i];
-      T[i][i] = ti;
-      for (int a = 0; a < i; ++a) {
-        double s = 0.0;
-        for (int c = a; c < i; ++c) s += T[a][c] * g[c][i];
-        T[a][i] = -ti * s;
+  const int jb = min(kBtNb, n - 1 - j0);
+  const int tw = p.nchunk * kBtNb;   // row stride of the repeated T
+  double* To = Tf + p.t_off + (long long)blk * kBtNb * tw;
+  const double* Gp = G + p.g_off + (long long)blk * kBtNb * kBtNb;
+  const double* tau = vec + p.v_off + (long long)kVtau * ld + j0;
+  const int tid = threadIdx.x;
+  if (tid < kBtNb)
+    for (int b = 0; b < kBtNb; ++b) To[(long long)tid * tw + b] = 0.0;
+  for (int i = 0; i < jb; ++i) {
+    if (tid < i) g[tid] = Gp[tid * kBtNb + i];
+    __syncthreads();
+    if (tid < i) {
+      const double* Ta = To + (long long)tid * tw;
+      double s0 = 0.0, s1 = 0.0;
+      int c = tid;
+      for (; c + 1 < i; c += 2) {
+        s0 = fma(Ta[c], g[c], s0);
+        s1 = fma(Ta[c + 1], g[c + 1], s1);
       }
+      if (c < i) s0 = fma(Ta[c], g[c], s0);
+      To[(long long)tid * tw + i] = -tau[i] * (s0 + s1);
+    } else if (tid == i) {
+      To[(long long)i * tw + i] = tau[i];
     }
+    __syncthreads();
   }
-  __syncthreads();
-  // T repeated once per K chunk: the products of the chunks are summed by
-  // the product with this wide T
-  for (int x = threadIdx.x; x < kEigNb * tw; x += blockDim.x) To[x] = T[x / tw][(x % tw) % kEigNb];
+  if (tid < kBtNb)
+    for (int c = 1; c < p.nchunk; ++c)
+      for (int b = 0; b < kBtNb; ++b) To[(long long)tid * tw + c * kBtNb + b] = To[(long long)tid * tw + b];
 }
 
 // the blocked path's problems are done for the block-Jacobi lanes

@@ -148,9 +148,20 @@ __device__ void grad_pair(const Element& ei, int is, const Element& ej, int js, 
   }
 }
 
+// LMAX 1: s and p shells only (no d-shell code, far fewer registers)
+template <int LMAX>
 __device__ void grad_dispatch(int li, int lj, const Element& ei, int is, const Element& ej, int js, const double* ri,
                               const double* rj, double intcut, int aoi, int aoj, const GradPairIn& in, double* g) {
 #define GP(a, b) grad_pair<a, b>(ei, is, ej, js, ri, rj, intcut, aoi, aoj, in, g)
+  if (LMAX < 2) {
+    switch (li * 2 + lj) {
+      case 0: GP(0, 0); break;
+      case 1: GP(0, 1); break;
+      case 2: GP(1, 0); break;
+      default: GP(1, 1); break;
+    }
+    return;
+  }
   switch (li * 3 + lj) {
     case 0: GP(0, 0); break;
     case 1: GP(0, 1); break;
@@ -171,6 +182,7 @@ __device__ void grad_dispatch(int li, int lj, const Element& ei, int is, const E
 // added to i and subtracted from j) and the dE/dCN sums of either side,
 // stored at gpair[5 * (p0 + i n + j)].  k_grad_h0 then gives atom a (one
 // workgroup, thread = partner b) the sum over its pairs in a fixed order.
+template <int LMAX>
 __global__ void __launch_bounds__(64) k_grad_h0_pairs(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
                                                       const int* __restrict__ at, const double* __restrict__ xyz,
                                                       const int* __restrict__ at_sh0, const int* __restrict__ at_ao0,
@@ -182,8 +194,9 @@ __global__ void __launch_bounds__(64) k_grad_h0_pairs(const MolDesc* __restrict_
                                                       const double* __restrict__ vq, double intcut,
                                                       const int2* __restrict__ pairs, double* gpair) {
   const MolDesc md = desc[blockIdx.y];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)md.n * (md.n - 1) / 2) return;
+  // LMAX 1 takes the pair list's leading s/p-only pairs, LMAX 2 the rest
+  const long long t = (LMAX < 2 ? 0 : md.npsp) + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (LMAX < 2 ? (long long)md.npsp : (long long)md.n * (md.n - 1) / 2)) return;
   const int2 pr = pairs[md.p0 + t];   // grouped by element pair: one code path per wave
   const int iat = pr.x, jat = pr.y;
   const int N = md.nao;
@@ -226,7 +239,7 @@ __global__ void __launch_bounds__(64) k_grad_h0_pairs(const MolDesc* __restrict_
           hs += 2 * hp * S[pos];
           dcn += hp / hav * S[pos];
         }
-      grad_dispatch(li, lj, ei, is, ej, js, ri, rj, intcut, aoi, aoj, in, gx);
+      grad_dispatch<LMAX>(li, lj, ei, is, ej, js, ri, rj, intcut, aoi, aoj, in, gx);
       for (int k = 0; k < 3; ++k) gp[k] += gx[k] + hs * dfac * rij[k] / shpoly;
       dcn_i += dcn * dsedcn[shi];
       dcn_j += dcn * dsedcn[shj];
@@ -289,29 +302,44 @@ __global__ void k_d4_c6(const MolDesc* __restrict__ desc, const Element* __restr
                         const double* __restrict__ c6ref, const double* __restrict__ zetavec,
                         const double* __restrict__ zerovec, const double* __restrict__ zetadcn,
                         const double* __restrict__ zerodcn, double* c6, double* dc6, double* c60, double* dc60) {
+  // one atom pair i >= j per thread (grid-stride over the lower triangle):
+  // C6 is symmetric, its CN derivatives are formed for both atoms of the pair
   const MolDesc md = desc[blockIdx.y];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long n = md.n;
-  if (t >= n * n) return;
-  const int i = (int)(t / n), j = (int)(t % n);
-  const int zi = at[md.a0 + i], zj = at[md.a0 + j];
-  const Element& ei = el[zi];
-  const Element& ej = el[zj];
-  const int bi = (md.a0 + i) * XTB_MAX_REF, bj = (md.a0 + j) * XTB_MAX_REF;
-  double s1 = 0, d1 = 0, s0 = 0, d0 = 0;
-  const double* tab = c6ref + ((long long)zi * (XTB_MAX_Z + 1) + zj) * 49;
-  for (int r = 0; r < ei.nref; ++r)
-    for (int s = 0; s < ej.nref; ++s) {
-      const double rc = tab[r * 7 + s];
-      s1 += zetavec[bi + r] * zetavec[bj + s] * rc;
-      d1 += zetadcn[bi + r] * zetavec[bj + s] * rc;
-      s0 += zerovec[bi + r] * zerovec[bj + s] * rc;
-      d0 += zerodcn[bi + r] * zerovec[bj + s] * rc;
+  const long long n = md.n, npair = n * (n + 1) / 2;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < npair;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((long long)i * (i + 1) / 2 > t) --i;
+    while ((long long)(i + 1) * (i + 2) / 2 <= t) ++i;
+    const int j = (int)(t - (long long)i * (i + 1) / 2);
+    const int zi = at[md.a0 + i], zj = at[md.a0 + j];
+    const int nri = el[zi].nref, nrj = el[zj].nref;
+    const int bi = (md.a0 + i) * XTB_MAX_REF, bj = (md.a0 + j) * XTB_MAX_REF;
+    const double* tab = c6ref + ((long long)zi * (XTB_MAX_Z + 1) + zj) * 49;
+    double s1 = 0, di = 0, dj = 0, s0 = 0, d0i = 0, d0j = 0;
+    for (int r = 0; r < nri; ++r) {
+      const double zr = zetavec[bi + r], zdr = zetadcn[bi + r], or_ = zerovec[bi + r], odr = zerodcn[bi + r];
+      for (int q = 0; q < nrj; ++q) {
+        const double rc = tab[r * 7 + q];
+        const double zq = zetavec[bj + q] * rc, oq = zerovec[bj + q] * rc;
+        s1 = fma(zr, zq, s1);
+        di = fma(zdr, zq, di);
+        dj = fma(zr, zetadcn[bj + q] * rc, dj);
+        s0 = fma(or_, oq, s0);
+        d0i = fma(odr, oq, d0i);
+        d0j = fma(or_, zerodcn[bj + q] * rc, d0j);
+      }
     }
-  c6[md.p0 + t] = s1;
-  dc6[md.p0 + t] = d1;  // dC6(i,j)/dCN_i
-  c60[md.p0 + t] = s0;
-  dc60[md.p0 + t] = d0;
+    const long long ij = md.p0 + (long long)i * n + j, ji = md.p0 + (long long)j * n + i;
+    c6[ij] = s1;
+    c6[ji] = s1;
+    dc6[ij] = di;   // dC6(i,j)/dCN_i
+    dc6[ji] = dj;   // dC6(j,i)/dCN_j
+    c60[ij] = s0;
+    c60[ji] = s0;
+    dc60[ij] = d0i;
+    dc60[ji] = d0j;
+  }
 }
 
 // D4 two-body gradient at fixed charges (disp_gradient_latp: src/disp/dftd4.F90:1832-1933)

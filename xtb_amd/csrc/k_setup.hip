@@ -169,10 +169,21 @@ __device__ void pair_block(const Element& ei, int is, const Element& ej, int js,
   }
 }
 
+// LMAX 1: s and p shells only (no d-shell code, far fewer registers)
+template <int LMAX = 2>
 __device__ void pair_dispatch(int li, int lj, const Element& ei, int is, const Element& ej, int js, const double* ri,
                               const double* rj, double intcut, double hav_sp, int aoi, int aoj, int N, long long NN,
                               double* S, double* H0, double* H0n, double* D, double* Q) {
 #define PB(a, b) pair_block<a, b>(ei, is, ej, js, ri, rj, intcut, hav_sp, aoi, aoj, N, NN, S, H0, H0n, D, Q)
+  if (LMAX < 2) {
+    switch (li * 2 + lj) {
+      case 0: PB(0, 0); break;
+      case 1: PB(0, 1); break;
+      case 2: PB(1, 0); break;
+      default: PB(1, 1); break;
+    }
+    return;
+  }
   switch (li * 3 + lj) {
     case 0: PB(0, 0); break;
     case 1: PB(0, 1); break;
@@ -188,15 +199,17 @@ __device__ void pair_dispatch(int li, int lj, const Element& ei, int is, const E
 }
 
 // one thread per atom pair (i > j), pairs taken from the molecule's list
-// grouped by element pair (one code path per wave)
+// grouped by element pair (one code path per wave); LMAX 1 takes the list's
+// leading s/p-only pairs, LMAX 2 the rest
+template <int LMAX>
 __global__ void k_sdqh0(const MolDesc* __restrict__ desc, const Element* __restrict__ el, const int* __restrict__ at,
                         const double* __restrict__ xyz, const int* __restrict__ at_sh0, const int* __restrict__ at_ao0,
                         const double* __restrict__ selfe, double intcut, const int2* __restrict__ pairs, double* S,
                         double* H0, double* H0n, double* D, double* Q) {
   const MolDesc md = desc[blockIdx.y];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long npair = (long long)md.n * (md.n - 1) / 2;
-  if (t >= npair) return;
+  const long long t = (LMAX < 2 ? 0 : md.npsp) + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (LMAX < 2 ? md.npsp : npair)) return;
   const int2 pr = pairs[md.p0 + t];
   const int i = pr.x, j = pr.y;
   const int* a = at + md.a0;
@@ -235,7 +248,8 @@ __global__ void k_sdqh0(const MolDesc* __restrict__ desc, const Element* __restr
       const double rf1 = 1.0 + 0.01 * ei.shpoly[li] * sqrt(rr);
       const double rf2 = 1.0 + 0.01 * ej.shpoly[lj] * sqrt(rr);
       const double shpoly = rf1 * rf2;
-      pair_dispatch(li, lj, ei, is, ej, js, ri, rj, intcut, hav * shpoly, aoi, aoj, N, NN, Sm, H0m, Hnm, Dm, Qm);
+      pair_dispatch<LMAX>(li, lj, ei, is, ej, js, ri, rj, intcut, hav * shpoly, aoi, aoj, N, NN, Sm, H0m, Hnm, Dm,
+                          Qm);
       aoj += 2 * lj + 1;
     }
     aoi += 2 * li + 1;
