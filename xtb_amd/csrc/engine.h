@@ -293,6 +293,7 @@ class Batch {
   hipStream_t stream3 = nullptr;   // the small (dim <= 64) eigenproblems beside the lanes
   hipStream_t stream4 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr, ev_join4 = nullptr;
+  hipEvent_t ev_atm = nullptr;   // the ATM dispersion terms of the set-up (stream3) are done
   int jac_lanes = 3;
   // the block-Jacobi problems work on padded copies (rows 128-byte aligned);
   // the register path only (no refinement, banded rounds, LDS kernels or QL)
@@ -330,6 +331,7 @@ class Batch {
   double *q = nullptr, *dipm = nullptr, *qp = nullptr, *radcn = nullptr;
   double *vs = nullptr, *vd = nullptr, *vq = nullptr, *ashift = nullptr;
   double *grad = nullptr, *dhdcn = nullptr, *dedcn = nullptr, *tmp2 = nullptr, *eat = nullptr;
+  double *grad_atm = nullptr, *dedcn_atm = nullptr;   // ATM gradient and dE/dCN terms of the set-up
   double *ae01 = nullptr;   // per-atom AES energy partials (3 per atom)
   // shell vectors
   int *sh_at = nullptr, *sh_l = nullptr, *sh_ao0 = nullptr, *sh_m = nullptr;

@@ -259,6 +259,17 @@ __global__ void __launch_bounds__(256) k_eeq_lu_update(const MolDesc* __restrict
 }
 
 // zero the atomic dipoles and quadrupoles (or the gradient) of the selected molecules
+// the ATM gradient and CN derivative terms computed beside the SCC, added
+// where the gradient takes them
+__global__ void k_add_atm(const MolDesc* __restrict__ desc, const double* __restrict__ gatm,
+                          const double* __restrict__ datm, double* grad, double* dedcn) {
+  const MolDesc md = desc[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= md.n) return;
+  for (int k = 0; k < 3; ++k) grad[3 * (md.a0 + i) + k] += gatm[3 * (md.a0 + i) + k];
+  dedcn[md.a0 + i] += datm[md.a0 + i];
+}
+
 __global__ void k_zero_atoms(const MolDesc* __restrict__ desc, double* v3, double* v6) {
   const MolDesc md = desc[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -764,6 +775,7 @@ void Batch::init_common() {
     HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&ev_join3, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&ev_atm, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&ev_join4, hipEventDisableTiming));
   }
   // XTBAMD_EIG=ql selects the tridiagonal QL eigensolver instead of block Jacobi
@@ -803,6 +815,7 @@ void Batch::allocate(const Capacity& c) {
   q = alloc<double>(tn); dipm = alloc<double>(3 * tn); qp = alloc<double>(6 * tn); radcn = alloc<double>(tn);
   vs = alloc<double>(tn); vd = alloc<double>(3 * tn); vq = alloc<double>(6 * tn); ashift = alloc<double>(tn);
   grad = alloc<double>(3 * tn); dhdcn = alloc<double>(tn); dedcn = alloc<double>(tn);
+  grad_atm = alloc<double>(3 * tn); dedcn_atm = alloc<double>(tn);
   tmp2 = alloc<double>(tn); eat = alloc<double>(tn); ae01 = alloc<double>(2 * tn);
   sh_at = alloc<int>(tsh); sh_l = alloc<int>(tsh); sh_ao0 = alloc<int>(tsh); sh_m = alloc<int>(tsh);
   at_sh0 = alloc<int>(tn); at_ao0 = alloc<int>(tn);
@@ -1319,6 +1332,7 @@ Batch::~Batch() {
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (ev_join3) (void)hipEventDestroy(ev_join3);
   if (ev_join4) (void)hipEventDestroy(ev_join4);
+  if (ev_atm) (void)hipEventDestroy(ev_atm);
   if (stream2) (void)hipStreamDestroy(stream2);
   if (stream3) (void)hipStreamDestroy(stream3);
   if (stream4) (void)hipStreamDestroy(stream4);
@@ -2214,6 +2228,23 @@ void Batch::setup() {
   hipLaunchKernelGGL(k_cn, gat, dim3(64), 0, stream, d_sub, el, at, xyz, cn, cn_cov, cn_d4, cn_eeq);
   hipLaunchKernelGGL(k_shell_setup, dim3(grid1(sub_max_nsh, 64), ns), dim3(64), 0, stream, d_sub, el, at, sh_at,
                      sh_m, cn, selfe, dsedcn, shgam, zsh);
+  // the three-body (ATM) dispersion takes the zero-charge C6 only, which the
+  // geometry and the CN fix: its energy and gradient run on stream3 beside
+  // the SCC (into their own buffers, added where the gradient takes them)
+  {
+    const bool fork = stream3 != stream;
+    if (fork) {
+      HIPCHECK(hipEventRecord(ev_atm, stream));
+      HIPCHECK(hipStreamWaitEvent(stream3, ev_atm, 0));
+    }
+    hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream3, d_sub, el, at, cn_cov,
+                       (const double*)nullptr, zetavec, zerovec, zetadcn, zerodcn);
+    hipLaunchKernelGGL(k_d4_c6<1>, dim3(gstride((long long)sub_max_n * (sub_max_n + 1) / 2), ns), dim3(256), 0,
+                       stream3, d_sub, el, at, c6ref, zetavec, zerovec, zetadcn, zerodcn, c6, dc6, c60, dc60);
+    hipLaunchKernelGGL(k_d4_atm, dim3(sub_max_n, ns), dim3(256), 0, stream3, d_sub, el, at, xyz, (const double*)c60,
+                       (const double*)dc60, grad_atm, dedcn_atm, eat);
+    if (fork) HIPCHECK(hipEventRecord(ev_atm, stream3));
+  }
   // a resident wavefunction: every selected molecule starts from the shell
   // charges and multipole moments the previous run left on the device, as
   // the reference starts from the wavefunction of a reused results object
@@ -2463,14 +2494,17 @@ void Batch::gradient() {
                        (const double*)T, (const double*)H0n, (const double*)S, (const double*)shift, (const double*)vs,
                        (const double*)vd, (const double*)vq, intcut, (const int2*)pair_list, gpair);
   hipLaunchKernelGGL(k_aes_grad, gat, dim3(64), 0, stream, d_sub, xyz, q, dipm, qp, radcn, grad, tmp2);
+  // the ATM terms from the set-up are done (and the weights below rewrite
+  // their zero-charge inputs with the same values)
+  if (stream3 != stream) HIPCHECK(hipStreamWaitEvent(stream, ev_atm, 0));
   hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, cn_cov, q,
                      zetavec, zerovec, zetadcn, zerodcn);
-  hipLaunchKernelGGL(k_d4_c6, dim3(gstride((long long)sub_max_n * (sub_max_n + 1) / 2), ns), dim3(256), 0, stream, d_sub, el,
-                     at, c6ref, zetavec, zerovec, zetadcn, zerodcn, c6, dc6, c60, dc60);
+  hipLaunchKernelGGL(k_d4_c6<2>, dim3(gstride((long long)sub_max_n * (sub_max_n + 1) / 2), ns), dim3(256), 0, stream,
+                     d_sub, el, at, c6ref, zetavec, zerovec, zetadcn, zerodcn, c6, dc6, c60, dc60);
   hipLaunchKernelGGL(k_d4_grad2, gat, dim3(64), 0, stream, d_sub, el, at, xyz, (const double*)c6, (const double*)dc6,
                      grad, dedcn);
-  hipLaunchKernelGGL(k_d4_atm, gat, dim3(256), 0, stream, d_sub, el, at, xyz, (const double*)c60, (const double*)dc60,
-                     grad, dedcn, eat);
+  hipLaunchKernelGGL(k_add_atm, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub,
+                     (const double*)grad_atm, (const double*)dedcn_atm, grad, dedcn);
   HIPCHECK(hipEventRecord(ev_join, stream2));
   HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
   hipLaunchKernelGGL(k_grad_h0, gat, dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0, at_ao0, (const double*)dsedcn,
@@ -2504,6 +2538,7 @@ void Batch::gradient() {
       hipLaunchKernelGGL(k_wbo, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub,
                          xyz, at_ao0, at, el, (const double*)A, wbo, 1 + spin, (const int*)open_act);
     }
+  if (stream3 != stream) HIPCHECK(hipStreamWaitEvent(stream, ev_atm, 0));   // the ATM energies
   hipLaunchKernelGGL(k_final, dim3(ns), dim3(256), 0, stream, d_sub, d_state, el, at, xyz, (const double*)grad,
                      (const double*)erep, (const double*)eat, (const double*)row_d);
   HIPCHECK(hipGetLastError());

@@ -74,18 +74,21 @@ __global__ void k_d4_refweights(const MolDesc* __restrict__ desc, const Element*
   const Element& e = el[at[md.a0 + i]];
   double w[XTB_MAX_REF], dw[XTB_MAX_REF];
   d4_cn_weights(e, cn[md.a0 + i], w, dw);
-  const double z = e.d4zeff, hard = kD4gc * e.d4gam, qi = q[md.a0 + i];
+  // q null: the zero-charge weights only (they do not depend on the charges)
+  const double z = e.d4zeff, hard = kD4gc * e.d4gam, qi = q ? q[md.a0 + i] : 0.0;
   const long long base = (long long)(md.a0 + i) * XTB_MAX_REF;
   for (int r = 0; r < XTB_MAX_REF; ++r) {
     double sq = 0.0, s0 = 0.0;
     if (r < e.nref) {
-      sq = zeta_d(kD4ga, hard, e.refq[r] + z, qi + z);
+      if (q) sq = zeta_d(kD4ga, hard, e.refq[r] + z, qi + z);
       s0 = zeta_d(kD4ga, hard, e.refq[r] + z, z);
     }
     const double wr = r < e.nref ? w[r] : 0.0, dr = r < e.nref ? dw[r] : 0.0;
-    zetavec[base + r] = sq * wr;
+    if (q) {
+      zetavec[base + r] = sq * wr;
+      zetadcn[base + r] = sq * dr;
+    }
     zerovec[base + r] = s0 * wr;
-    zetadcn[base + r] = sq * dr;
     zerodcn[base + r] = s0 * dr;
   }
 }

@@ -298,7 +298,11 @@ __global__ void __launch_bounds__(64) k_grad_h0(const MolDesc* __restrict__ desc
 
 // atomic C6 and dC6/dCN for atom pairs (get_atomic_c6: src/disp/dftd4.F90:1069-1147),
 // for the charge-scaled (c6, dc6) and the zero-charge (c60, dc60) weights
-__global__ void k_d4_c6(const MolDesc* __restrict__ desc, const Element* __restrict__ el, const int* __restrict__ at,
+// PART 1: the zero-charge C6 and its CN derivatives (c60, dc60), PART 2:
+// those at the SCC charges (c6, dc6)
+template <int PART>
+__global__ void __launch_bounds__(256) k_d4_c6(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
+                                               const int* __restrict__ at,
                         const double* __restrict__ c6ref, const double* __restrict__ zetavec,
                         const double* __restrict__ zerovec, const double* __restrict__ zetadcn,
                         const double* __restrict__ zerodcn, double* c6, double* dc6, double* c60, double* dc60) {
@@ -316,29 +320,37 @@ __global__ void k_d4_c6(const MolDesc* __restrict__ desc, const Element* __restr
     const int nri = el[zi].nref, nrj = el[zj].nref;
     const int bi = (md.a0 + i) * XTB_MAX_REF, bj = (md.a0 + j) * XTB_MAX_REF;
     const double* tab = c6ref + ((long long)zi * (XTB_MAX_Z + 1) + zj) * 49;
-    double s1 = 0, di = 0, dj = 0, s0 = 0, d0i = 0, d0j = 0;
-    for (int r = 0; r < nri; ++r) {
-      const double zr = zetavec[bi + r], zdr = zetadcn[bi + r], or_ = zerovec[bi + r], odr = zerodcn[bi + r];
-      for (int q = 0; q < nrj; ++q) {
-        const double rc = tab[r * 7 + q];
-        const double zq = zetavec[bj + q] * rc, oq = zerovec[bj + q] * rc;
-        s1 = fma(zr, zq, s1);
-        di = fma(zdr, zq, di);
-        dj = fma(zr, zetadcn[bj + q] * rc, dj);
-        s0 = fma(or_, oq, s0);
-        d0i = fma(odr, oq, d0i);
-        d0j = fma(or_, zerodcn[bj + q] * rc, d0j);
-      }
+    // every operand loaded before the sums (fixed trip counts; the table is
+    // zero beyond each element's references, and so are the loads here)
+    const double* wv = PART == 1 ? zerovec : zetavec;   // weights of this part
+    const double* wd = PART == 1 ? zerodcn : zetadcn;
+    double zI[XTB_MAX_REF], zdI[XTB_MAX_REF], zJ[XTB_MAX_REF], zdJ[XTB_MAX_REF];
+#pragma unroll
+    for (int r = 0; r < XTB_MAX_REF; ++r) {
+      const bool oni = r < nri, onj = r < nrj;
+      zI[r] = oni ? wv[bi + r] : 0.0;
+      zdI[r] = oni ? wd[bi + r] : 0.0;
+      zJ[r] = onj ? wv[bj + r] : 0.0;
+      zdJ[r] = onj ? wd[bj + r] : 0.0;
     }
+    double s1 = 0, di = 0, dj = 0;
+#pragma unroll
+    for (int r = 0; r < XTB_MAX_REF; ++r)
+#pragma unroll
+      for (int q = 0; q < XTB_MAX_REF; ++q) {
+        const double rc = tab[r * 7 + q];
+        const double zq = zJ[q] * rc;
+        s1 = fma(zI[r], zq, s1);
+        di = fma(zdI[r], zq, di);
+        dj = fma(zI[r], zdJ[q] * rc, dj);
+      }
     const long long ij = md.p0 + (long long)i * n + j, ji = md.p0 + (long long)j * n + i;
-    c6[ij] = s1;
-    c6[ji] = s1;
-    dc6[ij] = di;   // dC6(i,j)/dCN_i
-    dc6[ji] = dj;   // dC6(j,i)/dCN_j
-    c60[ij] = s0;
-    c60[ji] = s0;
-    dc60[ij] = d0i;
-    dc60[ji] = d0j;
+    double* cc = PART == 1 ? c60 : c6;
+    double* dc = PART == 1 ? dc60 : dc6;
+    cc[ij] = s1;
+    cc[ji] = s1;
+    dc[ij] = di;   // dC6(i,j)/dCN_i
+    dc[ji] = dj;   // dC6(j,i)/dCN_j
   }
 }
 
@@ -381,7 +393,9 @@ __global__ void __launch_bounds__(64) k_d4_grad2(const MolDesc* __restrict__ des
 }
 
 // ATM three-body term with zero-charge C6 (atm_gradient_latp, molecular branch:
-// src/disp/dftd4.F90:2121-2174, deriv_atm_triple :2354-2433): the triples of atom a
+// src/disp/dftd4.F90:2121-2174, deriv_atm_triple :2354-2433): the triples of atom a;
+// writes atom a's gradient, dE/dCN and energy terms (no accumulation: it runs
+// beside the SCC, and the gradient adds them later)
 constexpr int kAtmLds = 4096;   // molecules up to this size keep per-atom factors in LDS
 
 __global__ void __launch_bounds__(256) k_d4_atm(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
@@ -459,8 +473,8 @@ __global__ void __launch_bounds__(256) k_d4_atm(const MolDesc* __restrict__ desc
   dcn = block_sum<256>(dcn, sh);
   e = block_sum<256>(e, sh);
   if (threadIdx.x == 0) {
-    for (int k = 0; k < 3; ++k) grad[3 * (md.a0 + a) + k] += g[k];
-    dedcn[md.a0 + a] += dcn;
+    for (int k = 0; k < 3; ++k) grad[3 * (md.a0 + a) + k] = g[k];
+    dedcn[md.a0 + a] = dcn;
     eatm[md.a0 + a] = e;
   }
 }

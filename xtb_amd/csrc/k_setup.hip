@@ -202,7 +202,7 @@ __device__ void pair_dispatch(int li, int lj, const Element& ei, int is, const E
 // grouped by element pair (one code path per wave); LMAX 1 takes the list's
 // leading s/p-only pairs, LMAX 2 the rest
 template <int LMAX>
-__global__ void k_sdqh0(const MolDesc* __restrict__ desc, const Element* __restrict__ el, const int* __restrict__ at,
+__global__ void __launch_bounds__(64) k_sdqh0(const MolDesc* __restrict__ desc, const Element* __restrict__ el, const int* __restrict__ at,
                         const double* __restrict__ xyz, const int* __restrict__ at_sh0, const int* __restrict__ at_ao0,
                         const double* __restrict__ selfe, double intcut, const int2* __restrict__ pairs, double* S,
                         double* H0, double* H0n, double* D, double* Q) {
