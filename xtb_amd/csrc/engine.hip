@@ -2236,6 +2236,7 @@ void Batch::setup() {
     if (fork) {
       HIPCHECK(hipEventRecord(ev_atm, stream));
       HIPCHECK(hipStreamWaitEvent(stream3, ev_atm, 0));
+      if (stream4 != stream) HIPCHECK(hipStreamWaitEvent(stream4, ev_atm, 0));
     }
     hipLaunchKernelGGL(k_d4_refweights, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream3, d_sub, el, at, cn_cov,
                        (const double*)nullptr, zetavec, zerovec, zetadcn, zerodcn);
@@ -2245,6 +2246,32 @@ void Batch::setup() {
                        (const double*)dc60, grad_atm, dedcn_atm, eat);
     if (fork) HIPCHECK(hipEventRecord(ev_atm, stream3));
   }
+  // the matrices and factors that depend on the geometry and CN only
+  // (Coulomb, D4 dispersion matrix, AES damping, implicit solvent) on
+  // stream4 beside the integrals and S^(-1/2); joined before the SCC
+  const hipStream_t s4 = stream4 != stream ? stream4 : stream;
+  hipLaunchKernelGGL(k_coulomb, dim3(grid1((long long)sub_max_nsh * sub_max_nsh, 256), ns), dim3(256), 0, s4,
+                     d_sub, el, at, xyz, sh_at, sh_m, J);
+  hipLaunchKernelGGL(k_d4_weights, dim3(grid1(sub_max_nd4, 64), ns), dim3(64), 0, s4, d_sub, el, at, d4_at, d4_ref,
+                     cn_d4, gw);
+  hipLaunchKernelGGL(k_dispmat, dim3(grid1((long long)sub_max_nd4 * sub_max_nd4, 256), ns), dim3(256), 0, s4,
+                     d_sub, el, at, xyz, d4_at, d4_ref, gw, c6ref, dispmat);
+  hipLaunchKernelGGL(k_aes_setup, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, s4, d_sub,
+                     el, at, xyz, cn, radcn, gab3, gab5);
+  // implicit solvent: Born radii, surface, H-bond weights, shape, matrix B
+  if (solv_on_ && sub_max_n > 0) {
+    hipLaunchKernelGGL(k_eeq_reset, dim3(grid1(ns, 64)), dim3(64), 0, s4, d_sub, ns, sv_failed);
+    hipLaunchKernelGGL(k_solv_born, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, s4, d_sub,
+                       (const SolvDev*)d_solv, at, xyz, sv_brad, sv_dpsi);
+    hipLaunchKernelGGL(k_solv_sasa, dim3(sub_max_n, ns), dim3(64), 0, s4, d_sub, (const SolvDev*)d_solv, at, xyz,
+                       sv_w, sv_sasa, sv_failed);
+    hipLaunchKernelGGL(k_solv_mol, dim3(ns), dim3(256), 0, s4, d_sub, (const SolvDev*)d_solv, at, xyz,
+                       (const double*)sv_sasa, sv_hbw, sv_gsasa, sv_adet);
+    hipLaunchKernelGGL(k_solv_bmat, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, s4,
+                       d_sub, (const SolvDev*)d_solv, xyz, (const double*)sv_brad, (const double*)sv_hbw,
+                       (const double*)sv_adet, sv_B);
+  }
+  if (s4 != stream) HIPCHECK(hipEventRecord(ev_join4, s4));
   // a resident wavefunction: every selected molecule starts from the shell
   // charges and multipole moments the previous run left on the device, as
   // the reference starts from the wavefunction of a reused results object
@@ -2272,27 +2299,6 @@ void Batch::setup() {
     const int nt = (sub_max_nao + 31) / 32;
     hipLaunchKernelGGL(k_sym_lower, dim3(std::max(1, nt * (nt + 1) / 2), ns, 12), dim3(256), 0, stream, d_sub, S, H0,
                        H0n, D, Q);
-  }
-  hipLaunchKernelGGL(k_coulomb, dim3(grid1((long long)sub_max_nsh * sub_max_nsh, 256), ns), dim3(256), 0, stream,
-                     d_sub, el, at, xyz, sh_at, sh_m, J);
-  hipLaunchKernelGGL(k_d4_weights, dim3(grid1(sub_max_nd4, 64), ns), dim3(64), 0, stream, d_sub, el, at, d4_at, d4_ref,
-                     cn_d4, gw);
-  hipLaunchKernelGGL(k_dispmat, dim3(grid1((long long)sub_max_nd4 * sub_max_nd4, 256), ns), dim3(256), 0, stream,
-                     d_sub, el, at, xyz, d4_at, d4_ref, gw, c6ref, dispmat);
-  hipLaunchKernelGGL(k_aes_setup, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream, d_sub,
-                     el, at, xyz, cn, radcn, gab3, gab5);
-  // implicit solvent: Born radii, surface, H-bond weights, shape, matrix B
-  if (solv_on_ && sub_max_n > 0) {
-    hipLaunchKernelGGL(k_eeq_reset, dim3(grid1(ns, 64)), dim3(64), 0, stream, d_sub, ns, sv_failed);
-    hipLaunchKernelGGL(k_solv_born, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub,
-                       (const SolvDev*)d_solv, at, xyz, sv_brad, sv_dpsi);
-    hipLaunchKernelGGL(k_solv_sasa, dim3(sub_max_n, ns), dim3(64), 0, stream, d_sub, (const SolvDev*)d_solv, at, xyz,
-                       sv_w, sv_sasa, sv_failed);
-    hipLaunchKernelGGL(k_solv_mol, dim3(ns), dim3(256), 0, stream, d_sub, (const SolvDev*)d_solv, at, xyz,
-                       (const double*)sv_sasa, sv_hbw, sv_gsasa, sv_adet);
-    hipLaunchKernelGGL(k_solv_bmat, dim3(grid1((long long)sub_max_n * sub_max_n, 256), ns), dim3(256), 0, stream,
-                       d_sub, (const SolvDev*)d_solv, xyz, (const double*)sv_brad, (const double*)sv_hbw,
-                       (const double*)sv_adet, sv_B);
   }
   // initial charges: EEQ (default) or SAD.  The EEQ solve runs on stream2
   // beside the integrals and S^(-1/2) (it reads only CN, zsh and geometry,
@@ -2339,6 +2345,7 @@ void Batch::setup() {
     HIPCHECK(hipEventRecord(ev_join, stream2));
     HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
   }
+  if (stream4 != stream) HIPCHECK(hipStreamWaitEvent(stream, ev_join4, 0));   // the geometry-only set-up
   // molecules whose S^(-1/2) or EEQ guess failed do not iterate
   hipLaunchKernelGGL(k_fail_fold, dim3(grid1(nmol, 64)), dim3(64), 0, stream, d_state, nmol, (const int*)ns_failed, 1,
                      kFailOrtho, (const int*)phase_act);
