@@ -91,16 +91,20 @@ def tune(steps=3, out=HERE, log=print):
         opts = {1 << 30: tb}
         opts.update({it: t for it, t in zip(hybrid, th)})
         it_best = min(opts, key=lambda k: opts[k])
-        best_per_bin.append((nao, opts[it_best] < tj, it_best))
+        best_per_bin.append((nao, opts[it_best] < tj, it_best, tj, opts))
         log(f"tune eig: {kind} {size} nao={nao} jacobi {1e3 * tj:.1f} ms blocked {1e3 * tb:.1f} ms "
             + " ".join(f"first-{it} {1e3 * t:.1f} ms" for it, t in zip(hybrid, th)) + f" dE={dE:.1e}")
-    best_per_bin.sort()
+    best_per_bin.sort(key=lambda r: r[0])
+    # one setting for every size from the cut on: the iteration count that
+    # is best for the largest molecule, and the smallest cut from which that
+    # count beats block Jacobi at every size measured
     cut, iters = 1 << 30, 1 << 30
-    for i, (nao, win, _) in enumerate(best_per_bin):
-        if all(w for _, w, _ in best_per_bin[i:]):
-            cut = nao
-            iters = best_per_bin[-1][2]   # the choice of the largest molecule
-            break
+    if best_per_bin and best_per_bin[-1][1]:
+        it_l = best_per_bin[-1][2]
+        for i in range(len(best_per_bin)):
+            if all(opts[it_l] < tj for _, _, _, tj, opts in best_per_bin[i:]):
+                cut, iters = best_per_bin[i][0], it_l
+                break
     # 2. batch schedule on the 512-molecule workload
     mols = workloads.mixed_batch(512, seed=2024)
     best = None
