@@ -1248,11 +1248,13 @@ __device__ __forceinline__ void cross_rows(double (&B)[4][2], int lane) {
 }
 
 // column move of a cross round: bottom(c) <- bottom((c + 1) mod 16), the
-// row_ror:15 rotation within each row of 16 lanes
+// row_ror:15 rotation within each row of 16 lanes (every lane has a source,
+// so the old value passed is the source itself: no register to clear)
 __device__ __forceinline__ double cross_col(double bot) {
   const long long b = __double_as_longlong(bot);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned int)(b & 0xffffffffll), 0x12F, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x12F, 0xF, 0xF, false);
+  const int blo = (int)(unsigned int)(b & 0xffffffffll), bhi = (int)(b >> 32);
+  const int lo = __builtin_amdgcn_update_dpp(blo, blo, 0x12F, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(bhi, bhi, 0x12F, 0xF, 0xF, false);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
 
@@ -1498,9 +1500,17 @@ __device__ __forceinline__ void jac_pivot_body(const JacobiProb* __restrict__ pr
 }
 
 // sampled launches (ts != nullptr) record the device wall clock at the start
-// of their earliest wave and the end of their latest: the execution span
+// of their earliest wave and the end of their latest: the execution span.
+// The occupancy target sets the register budget: the cross-round kernel needs
+// about 180 VGPRs, so a target of 3 waves per SIMD (168 VGPRs) spills the
+// rotation buffer's address and reloads it from scratch in every inner round
+// (26.6 us per launch against 23.0 us with 2 waves per SIMD on the
+// 512-molecule batch, which has about 2 pivot waves per SIMD per round).
+#ifndef XTBAMD_PIVOT_WAVES
+#define XTBAMD_PIVOT_WAVES 2
+#endif
 template <bool CROSSK>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(XTBAMD_PIVOT_WAVES)))
 k_jac_pairsR(const JacobiProb* __restrict__ probs, const int2* __restrict__ list, int r, const double* __restrict__ A,
              double* Qbuf, int* rotflag, const double* __restrict__ scale_in, int* state, int inner_sweeps,
              unsigned long long* work, double tolrel, unsigned long long* offmax, int stride, unsigned long long* ts) {
