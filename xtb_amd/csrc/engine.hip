@@ -2395,7 +2395,8 @@ bool Batch::scc_iteration() {
                      (const MolState*)d_state);
   HIPCHECK(hipEventRecord(ev_join, stream2));
   HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
-  hipLaunchKernelGGL(k_fock, dim3(gstride((long long)max_nao * max_nao), nmol), dim3(256), 0, stream, d_desc, S, H0,
+  hipLaunchKernelGGL(k_fock, dim3(std::max(1, ((max_nao + 31) / 32) * ((max_nao + 31) / 32 + 1) / 2), nmol), dim3(256),
+                     0, stream, d_desc, S, H0,
                      D, Q, shift, vs, vd, vq, ao_sh, ao_at, thr2, H, (const MolState*)d_state);
   // generalized eigenproblem in the current orbital basis C (S-orthonormal)
   mixed_here_ = true;   // the Fock transforms and the density product may run mixed

@@ -85,20 +85,17 @@ __global__ void k_shift_final(const MolDesc* __restrict__ desc, const Element* _
   shift[md.s0 + s] += v;
 }
 
-// Fock matrix in eV (buildIsoAnisotropicH1: src/scc_core.f90:154-251), with the
-// dipole/quadrupole neglect lists of setdqlist (src/aespot.F90:34-85) applied per element
-__global__ void __launch_bounds__(256) k_fock(const MolDesc* __restrict__ desc, const double* __restrict__ S,
-                                              const double* __restrict__ H0, const double* __restrict__ D,
-                                              const double* __restrict__ Q, const double* __restrict__ shift,
-                                              const double* __restrict__ vs, const double* __restrict__ vd,
-                                              const double* __restrict__ vq, const int* __restrict__ ao_sh,
-                                              const int* __restrict__ ao_at, double thr2, double* H,
-                                              const MolState* __restrict__ state) {
-  const MolDesc md = desc[blockIdx.y];
-  if (!state[blockIdx.y].active) return;
+// Fock matrix element (i, j) in eV (buildIsoAnisotropicH1: src/scc_core.f90:154-251),
+// with the dipole/quadrupole neglect lists of setdqlist (src/aespot.F90:34-85)
+// applied per element
+__device__ __forceinline__ double fock_elem(const MolDesc& md, int i, int j, const double* __restrict__ S,
+                                           const double* __restrict__ H0, const double* __restrict__ D,
+                                           const double* __restrict__ Q, const double* __restrict__ shift,
+                                           const double* __restrict__ vs, const double* __restrict__ vd,
+                                           const double* __restrict__ vq, const int* __restrict__ ao_sh,
+                                           const int* __restrict__ ao_at, double thr2) {
   const long long N = md.nao, NN = N * N;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < NN; t += (long long)gridDim.x * blockDim.x) {
-  const int i = (int)(t / N), j = (int)(t % N);
+  const long long t = (long long)i * N + j;
   const long long p = md.m0 + t;
   const double dum = S[p] * kAutoEV * 0.5;
   const int si = ao_sh[md.o0 + i], sj = ao_sh[md.o0 + j];
@@ -129,7 +126,53 @@ __global__ void __launch_bounds__(256) k_fock(const MolDesc* __restrict__ desc, 
     for (int l = 0; l < 6; ++l) e1 += qv[l] * (vq[6 * ia + l] + vq[6 * ja + l]);
     h += 0.50 * e1 * kAutoEV;
   }
-  H[p] = h;
+  return h;
+}
+
+// The Fock matrix is symmetric bit for bit: S, H0, D and Q are mirrored
+// copies of their lower triangles and every term adds the two AOs' potentials
+// in a commutative sum.  Workgroup (tile pair, molecule) forms the 32 x 32
+// tile (I, J), I >= J, of H from the lower triangles of the 11 inputs (rows
+// of 32 consecutive elements, each read once) and writes it and, through LDS,
+// its transpose (I, J) -> (J, I); a diagonal tile is formed whole.
+__global__ void __launch_bounds__(256) k_fock(const MolDesc* __restrict__ desc, const double* __restrict__ S,
+                                              const double* __restrict__ H0, const double* __restrict__ D,
+                                              const double* __restrict__ Q, const double* __restrict__ shift,
+                                              const double* __restrict__ vs, const double* __restrict__ vd,
+                                              const double* __restrict__ vq, const int* __restrict__ ao_sh,
+                                              const int* __restrict__ ao_at, double thr2, double* H,
+                                              const MolState* __restrict__ state) {
+  __shared__ double tile[32][33];
+  const MolDesc md = desc[blockIdx.y];
+  if (!state[blockIdx.y].active) return;
+  const int N = md.nao;
+  const int nt = (N + 31) / 32;
+  const int npairs = nt * (nt + 1) / 2;
+  const int c = threadIdx.x & 31, r0 = threadIdx.x >> 5;
+  double* Hm = H + md.m0;
+  for (int tp = blockIdx.x; tp < npairs; tp += gridDim.x) {
+    // tp = I (I + 1) / 2 + J, 0 <= J <= I
+    int I = (int)((sqrt(8.0 * tp + 1.0) - 1.0) * 0.5);
+    while (I * (I + 1) / 2 > tp) --I;
+    while ((I + 1) * (I + 2) / 2 <= tp) ++I;
+    const int J = tp - I * (I + 1) / 2;
+    for (int rr = r0; rr < 32; rr += 8) {
+      const int i = 32 * I + rr, j = 32 * J + c;
+      double h = 0.0;
+      if (i < N && j < N) {
+        h = fock_elem(md, i, j, S, H0, D, Q, shift, vs, vd, vq, ao_sh, ao_at, thr2);
+        Hm[(long long)i * N + j] = h;
+      }
+      tile[rr][c] = h;
+    }
+    if (I != J) {
+      __syncthreads();
+      for (int rr = r0; rr < 32; rr += 8) {
+        const int i = 32 * J + rr, j = 32 * I + c;
+        if (i < N && j < N) Hm[(long long)i * N + j] = tile[c][rr];
+      }
+    }
+    __syncthreads();
   }
 }
 

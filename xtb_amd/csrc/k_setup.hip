@@ -257,8 +257,9 @@ __global__ void __launch_bounds__(64) k_sdqh0(const MolDesc* __restrict__ desc, 
 }
 
 // on-site blocks: S = 1, H0 = self energy, dipole/quadrupole moments of the
-// atom's own shell pairs about the origin (build_SDQH0 :298-369)
-__global__ void k_sdq_onsite(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
+// atom's own shell pairs about the origin (build_SDQH0 :298-369); atoms
+// without d shells take the s/p-only code (no d-shell temporaries)
+__global__ void __launch_bounds__(64) k_sdq_onsite(const MolDesc* __restrict__ desc, const Element* __restrict__ el,
                              const int* __restrict__ at, const double* __restrict__ xyz, const int* __restrict__ at_sh0,
                              const int* __restrict__ at_ao0, const double* __restrict__ selfe, double intcut, double* S,
                              double* H0, double* H0n, double* D, double* Q) {
@@ -285,6 +286,8 @@ __global__ void k_sdq_onsite(const MolDesc* __restrict__ desc, const Element* __
       H0m[p] = 0.0;
       Hnm[p] = 0.0;
     }
+  bool has_d = false;
+  for (int s = 0; s < e.nsh; ++s) has_d = has_d || e.l[s] > 1;
   int aoi = ao0;
   for (int is = 0; is < e.nsh; ++is) {
     const double se = selfe[md.s0 + at_sh0[md.a0 + i] + is];
@@ -295,9 +298,14 @@ __global__ void k_sdq_onsite(const MolDesc* __restrict__ desc, const Element* __
     }
     int aoj = ao0;
     for (int js = 0; js < e.nsh; ++js) {
-      if (js <= is)
-        pair_dispatch(e.l[is], e.l[js], e, is, e, js, ri, ri, intcut, 0.0, aoi, aoj, N, NN, nullptr, nullptr, nullptr,
-                      Dm, Qm);
+      if (js <= is) {
+        if (has_d)
+          pair_dispatch<2>(e.l[is], e.l[js], e, is, e, js, ri, ri, intcut, 0.0, aoi, aoj, N, NN, nullptr, nullptr,
+                           nullptr, Dm, Qm);
+        else
+          pair_dispatch<1>(e.l[is], e.l[js], e, is, e, js, ri, ri, intcut, 0.0, aoi, aoj, N, NN, nullptr, nullptr,
+                           nullptr, Dm, Qm);
+      }
       aoj += 2 * e.l[js] + 1;
     }
     aoi += 2 * e.l[is] + 1;
