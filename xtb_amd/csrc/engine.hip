@@ -1846,8 +1846,9 @@ static void eig_set_launch(EigSet& es, const JacobiProb* jprobs, int* jstate, co
 // at a sweep boundary against the off-diagonal maximum the tile updates
 // measured at the end of the sweep before) and the A and W tile updates.
 // Once per period (the rounds of a sweep of the lanes' largest problem),
-// after the pivots, the host reads the lanes' counts of problems still
-// running and stops a lane that has none left.
+// after the pivots, the lanes count their problems still running; the host
+// reads the counts once that round's tile updates and the next round's
+// pivots are queued and stops a lane that has none left.
 constexpr int kProfEvery = 51;
 
 void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
@@ -1864,11 +1865,12 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
   }
   const int period = std::max(1, jac_period_);
   const int rmax = (jac_max_sweeps + 1) * js.max_rounds;
-  for (int r = 0; r <= rmax; ++r) {
-    // timing events, device spans and work counts on every kProfEvery-th
-    // round only, counted over all eigensolves (51 is prime to the
-    // cross-round period 8, so the sample meets every kind of round alike):
-    // an event pair per launch would cost more host time than the launch
+  // the pivots of round rr on every live lane; timing events, device spans
+  // and work counts on every kProfEvery-th round only, counted over all
+  // eigensolves (51 is prime to the cross-round period 8, so the sample meets
+  // every kind of round alike): an event pair per launch would cost more host
+  // time than the launch
+  auto launch_pivots = [&](int rr) {
     const bool sample = (prof_round_ % kProfEvery) == 0;
     for (int g = 0; g < kMaxLanes; ++g) {
       if (!live[g]) continue;
@@ -1879,15 +1881,27 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
         pts = prof_ts_ + (size_t)kProfSlotU64 * (prof_ts_used_[0]++);
       if (jac_cross)
         hipLaunchKernelGGL(k_jac_pairsR<true>, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
-                           js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
+                           js.d_xp + js.gxp[g], rr, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
                            js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol, pts);
       else
         hipLaunchKernelGGL(k_jac_pairsR<false>, dim3(js.gxp[g + 1] - js.gxp[g]), dim3(64), 0, ls[g], js.d_probs,
-                           js.d_xp + js.gxp[g], r, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
+                           js.d_xp + js.gxp[g], rr, (const double*)Am, js.qbuf, js.rot, (const double*)js.scale,
                            js.state, inner_sweeps, sample ? jw : nullptr, jac_tol, js.offmax, nmol, pts);
       if (sample) prof_end(2, nullptr, 0, ls[g]);
     }
-    if (r % period == 0) {
+  };
+  bool pivots_queued = false;   // round r's pivots went out before the host read the counts
+  for (int r = 0; r <= rmax; ++r) {
+    const bool sample = (prof_round_ % kProfEvery) == 0;
+    if (!pivots_queued) launch_pivots(r);
+    pivots_queued = false;
+    // once per period the lanes' counts of problems still running go to the
+    // host; it reads them only after this round's tile updates and the next
+    // round's pivots are queued, so that the GPU works while the host waits.
+    // A lane found empty has only finished problems, which those launches
+    // skip (the pivots and tile updates test the problem state first).
+    const bool check = r % period == 0;
+    if (check) {
       for (int g = 0; g < kMaxLanes; ++g)
         if (live[g])
           hipLaunchKernelGGL(k_jac_count, dim3(1), dim3(256), 0, ls[g], js.d_probs, nmol, (const int*)js.state, g,
@@ -1897,13 +1911,7 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       HIPCHECK(hipEventRecord(ev_join4, stream4));
       HIPCHECK(hipStreamWaitEvent(stream, ev_join4, 0));
       HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, kMaxLanes * sizeof(int), hipMemcpyDeviceToHost, stream));
-      HIPCHECK(hipStreamSynchronize(stream));
-      bool any = false;
-      for (int g = 0; g < kMaxLanes; ++g) {
-        live[g] = live[g] && h_pinned[g] > 0;
-        any = any || live[g];
-      }
-      if (!any) break;
+      HIPCHECK(hipEventRecord(ev_join3, stream));
     }
     unsigned long long* tw = (sample && (prof_mask & 8)) ? prof_work_ + 3 : (unsigned long long*)nullptr;
     for (int g = 0; g < kMaxLanes; ++g) {
@@ -1920,6 +1928,19 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
     }
     jacobi_rounds_ += 1;
     prof_round_ += 1;
+    if (check) {
+      if (r < rmax) {
+        launch_pivots(r + 1);
+        pivots_queued = true;
+      }
+      HIPCHECK(hipEventSynchronize(ev_join3));
+      bool any = false;
+      for (int g = 0; g < kMaxLanes; ++g) {
+        live[g] = live[g] && h_pinned[g] > 0;
+        any = any || live[g];
+      }
+      if (!any) break;
+    }
   }
 }
 
