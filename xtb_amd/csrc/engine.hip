@@ -2212,6 +2212,11 @@ void Batch::orthogonalizer() {
   // parity; a molecule's final pair is at the parity recorded when it stops
   double* Yb[2] = {A, H};
   double* Zb[2] = {W, T};
+  // The per-molecule state machine (k_ns_mark) and the activity mask of the
+  // products live on the device, so an iteration queued after every molecule
+  // has stopped changes nothing.  The host therefore reads iteration it's
+  // count of running molecules only after iteration it + 1 is queued: the GPU
+  // never waits for the host's read.
   for (int it = 0; it <= maxit; ++it) {
     double *Y = Yb[it & 1], *Z = Zb[it & 1];
     // Y and Z are polynomials in S: they commute, and Z Y, Y T and T Z are symmetric
@@ -2219,11 +2224,14 @@ void Batch::orthogonalizer() {
     hipLaunchKernelGGL(k_ns_form, gmm, dim3(256), 0, stream, d_sub, (const int*)ns_act, P, ns_res);
     hipLaunchKernelGGL(k_ns_mark, dim3(1), dim3(256), 0, stream, nmol, ns_stage, ns_act, ns_res, it, maxit, ns_failed,
                        jac_nactive, ns_fin);
-    HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHECK(hipStreamSynchronize(stream));
-    if (h_pinned[0] <= 0) break;
+    HIPCHECK(hipMemcpyAsync(h_pinned + (it & 1), jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
     gemm(0, Y, P, Yb[(it + 1) & 1], nullptr, nullptr, false, 1.0, ns_act, true);   // Y T
     gemm(0, P, Z, Zb[(it + 1) & 1], nullptr, nullptr, false, 1.0, ns_act, true);   // T Z
+    if (it > 0) {
+      HIPCHECK(hipEventSynchronize(ev_join3));   // iteration it - 1's count is on the host
+      if (h_pinned[(it - 1) & 1] <= 0) break;
+    }
+    HIPCHECK(hipEventRecord(ev_join3, stream));
   }
   hipLaunchKernelGGL(k_ns_final, gmm, dim3(256), 0, stream, d_sub, (const double*)Zb[0], (const double*)Zb[1],
                      (const int*)ns_fin, (const double*)ns_c, C);
