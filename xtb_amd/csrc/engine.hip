@@ -1847,8 +1847,8 @@ static void eig_set_launch(EigSet& es, const JacobiProb* jprobs, int* jstate, co
 // measured at the end of the sweep before) and the A and W tile updates.
 // Once per period (the rounds of a sweep of the lanes' largest problem),
 // after the pivots, the lanes count their problems still running; the host
-// reads the counts once that round's tile updates and the next round's
-// pivots are queued and stops a lane that has none left.
+// reads the counts once the round's tile updates and the next round's pivots
+// are queued, with the GPU still busy, and stops a lane that has none left.
 constexpr int kProfEvery = 51;
 
 void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
@@ -1890,28 +1890,32 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
       if (sample) prof_end(2, nullptr, 0, ls[g]);
     }
   };
+  // Once per period the lanes count their problems still running and copy the
+  // counts to the host on their own streams (no join between the lanes).  The
+  // host reads them kCheckLag rounds later, after queueing that round's tile
+  // updates and the next round's pivots, so that the GPU works while the host
+  // waits; a lane found empty has only finished problems, which the launches
+  // queued meanwhile skip (the pivots and tile updates test the problem state
+  // first).
+  constexpr int kCheckLag = 0;   // 0-2 measured alike on the 512-molecule batch
+  const hipEvent_t lane_ev[kMaxLanes] = {ev_join3, ev_join, ev_join4};
   bool pivots_queued = false;   // round r's pivots went out before the host read the counts
+  int pending = -1;             // round of the counts on their way to the host
+  bool waiting[kMaxLanes] = {};   // lanes whose counts are on their way
   for (int r = 0; r <= rmax; ++r) {
     const bool sample = (prof_round_ % kProfEvery) == 0;
     if (!pivots_queued) launch_pivots(r);
     pivots_queued = false;
-    // once per period the lanes' counts of problems still running go to the
-    // host; it reads them only after this round's tile updates and the next
-    // round's pivots are queued, so that the GPU works while the host waits.
-    // A lane found empty has only finished problems, which those launches
-    // skip (the pivots and tile updates test the problem state first).
-    const bool check = r % period == 0;
-    if (check) {
-      for (int g = 0; g < kMaxLanes; ++g)
-        if (live[g])
-          hipLaunchKernelGGL(k_jac_count, dim3(1), dim3(256), 0, ls[g], js.d_probs, nmol, (const int*)js.state, g,
-                             jac_nactive + g);
-      HIPCHECK(hipEventRecord(ev_join, stream2));
-      HIPCHECK(hipStreamWaitEvent(stream, ev_join, 0));
-      HIPCHECK(hipEventRecord(ev_join4, stream4));
-      HIPCHECK(hipStreamWaitEvent(stream, ev_join4, 0));
-      HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, kMaxLanes * sizeof(int), hipMemcpyDeviceToHost, stream));
-      HIPCHECK(hipEventRecord(ev_join3, stream));
+    if (r % period == 0 && pending < 0) {
+      for (int g = 0; g < kMaxLanes; ++g) {
+        if (!live[g]) continue;
+        hipLaunchKernelGGL(k_jac_count, dim3(1), dim3(256), 0, ls[g], js.d_probs, nmol, (const int*)js.state, g,
+                           jac_nactive + g);
+        HIPCHECK(hipMemcpyAsync(h_pinned + g, jac_nactive + g, sizeof(int), hipMemcpyDeviceToHost, ls[g]));
+        HIPCHECK(hipEventRecord(lane_ev[g], ls[g]));
+        waiting[g] = true;
+      }
+      pending = r;
     }
     unsigned long long* tw = (sample && (prof_mask & 8)) ? prof_work_ + 3 : (unsigned long long*)nullptr;
     for (int g = 0; g < kMaxLanes; ++g) {
@@ -1928,17 +1932,21 @@ void Batch::jacobi_lanes(JacSet& js, double* Am, double* Wm) {
     }
     jacobi_rounds_ += 1;
     prof_round_ += 1;
-    if (check) {
+    if (pending >= 0 && (r == pending + kCheckLag || r == rmax)) {
       if (r < rmax) {
         launch_pivots(r + 1);
         pivots_queued = true;
       }
-      HIPCHECK(hipEventSynchronize(ev_join3));
       bool any = false;
       for (int g = 0; g < kMaxLanes; ++g) {
-        live[g] = live[g] && h_pinned[g] > 0;
+        if (waiting[g]) {
+          HIPCHECK(hipEventSynchronize(lane_ev[g]));
+          live[g] = live[g] && h_pinned[g] > 0;
+          waiting[g] = false;
+        }
         any = any || live[g];
       }
+      pending = -1;
       if (!any) break;
     }
   }

@@ -174,7 +174,7 @@ def _solve_phase(name):
 def single_table(run_dir, n, top=22):
     """Kernel table of one single point of an n-orbital molecule with the
     algorithmic rates of the kernels whose work has a closed count:
-    the full-size products (2 n^3 flops per launch), the tridiagonalisation's
+    the full-size products (2 n^3 flops per full product), the tridiagonalisation's
     column kernel (8 (n-2-j)^2 bytes of trailing rows for column j), its panel
     updates (2 (n-j0-32)^2 * 64 flops per 32-column panel) and the Jacobi tile
     updates (28 n^2 bytes per round if every pair rotates: an upper bound);
@@ -198,8 +198,14 @@ def single_table(run_dir, n, top=22):
             rate = (f"{nsolve * trd_bytes / sec / 1e12:.2f} TB/s (trailing rows, "
                     f"{trd_bytes / (n - 1) / 1e6:.0f} MB per column on average)")
         elif name == "k_gemm_mfma<false, false, false, false>":
-            r = c * 2.0 * n ** 3 / sec / 1e12
-            rate = f"{r:.1f} TFLOP/s FP64 MFMA (2N^3 per launch), {100 * r / 78.6:.0f}% of 78.6"
+            # full products only: the symmetric ones form half the tiles (about
+            # half the time) and launches masked off for every molecule take a
+            # few us; a full product is one within 25% of the longest launch
+            durs = [e - s for s, e, nm in rows if nm == name]
+            full = [d for d in durs if d >= 0.75 * max(durs)]
+            r = len(full) * 2.0 * n ** 3 / (sum(full) * 1e-9) / 1e12
+            rate = (f"{r:.1f} TFLOP/s FP64 MFMA over its {len(full)} full products (2N^3 each), "
+                    f"{100 * r / 78.6:.0f}% of 78.6")
         elif name == "k_bgemm<true, false>" and nsolve:
             r = nsolve * upd_flops / sec / 1e12
             rate = f"{r:.1f} TFLOP/s FP64 MFMA (panel updates), {100 * r / 78.6:.0f}% of 78.6"
