@@ -2308,7 +2308,6 @@ void Batch::setup() {
                        d_sub, (const SolvDev*)d_solv, xyz, (const double*)sv_brad, (const double*)sv_hbw,
                        (const double*)sv_adet, sv_B);
   }
-  if (s4 != stream) HIPCHECK(hipEventRecord(ev_join4, s4));
   // a resident wavefunction: every selected molecule starts from the shell
   // charges and multipole moments the previous run left on the device, as
   // the reference starts from the wavefunction of a reused results object
@@ -2332,11 +2331,19 @@ void Batch::setup() {
   hipLaunchKernelGGL(k_sdq_onsite, dim3(grid1(sub_max_n, 64), ns), dim3(64), 0, stream, d_sub, el, at, xyz, at_sh0,
                      at_ao0, selfe, intcut, S, H0, H0n, D, Q);
   {
-    // the integral kernels wrote the lower triangles: mirror them
+    // the integral kernels wrote the lower triangles: mirror them, S first
+    // (S^(-1/2) needs it next), the other 11 on stream4 after the
+    // geometry-only matrices (nothing reads them before the SCC)
     const int nt = (sub_max_nao + 31) / 32;
-    hipLaunchKernelGGL(k_sym_lower, dim3(std::max(1, nt * (nt + 1) / 2), ns, 12), dim3(256), 0, stream, d_sub, S, H0,
-                       H0n, D, Q);
+    const int ntp = std::max(1, nt * (nt + 1) / 2);
+    if (s4 != stream) {
+      HIPCHECK(hipEventRecord(ev_join3, stream));
+      HIPCHECK(hipStreamWaitEvent(s4, ev_join3, 0));
+    }
+    hipLaunchKernelGGL(k_sym_lower, dim3(ntp, ns, 1), dim3(256), 0, stream, d_sub, S, H0, H0n, D, Q, 0);
+    hipLaunchKernelGGL(k_sym_lower, dim3(ntp, ns, 11), dim3(256), 0, s4, d_sub, S, H0, H0n, D, Q, 1);
   }
+  if (s4 != stream) HIPCHECK(hipEventRecord(ev_join4, s4));
   // initial charges: EEQ (default) or SAD.  The EEQ solve runs on stream2
   // beside the integrals and S^(-1/2) (it reads only CN, zsh and geometry,
   // writes only its own system and the charges); joined below.

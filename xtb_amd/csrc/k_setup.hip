@@ -316,9 +316,10 @@ __global__ void __launch_bounds__(64) k_sdq_onsite(const MolDesc* __restrict__ d
 // their lower triangle: workgroup (tile pair, molecule, matrix) reads the
 // 32 x 32 tile (I, J), I >= J, row by row and writes its transpose to tile
 // (J, I) row by row through LDS (in a diagonal tile only the entries above
-// the diagonal).  Launched after k_sdqh0 and k_sdq_onsite.
+// the diagonal); matrix k0 + blockIdx.z of the 12.  Launched after k_sdqh0 and
+// k_sdq_onsite.
 __global__ void __launch_bounds__(256) k_sym_lower(const MolDesc* __restrict__ desc, double* S, double* H0,
-                                                   double* H0n, double* D, double* Q) {
+                                                   double* H0n, double* D, double* Q, int k0) {
   __shared__ double t[32][33];
   const MolDesc md = desc[blockIdx.y];
   const int N = md.nao, nt = (N + 31) / 32;
@@ -330,7 +331,7 @@ __global__ void __launch_bounds__(256) k_sym_lower(const MolDesc* __restrict__ d
   while ((I + 1) * (I + 2) / 2 <= pidx) ++I;
   const int J = pidx - I * (I + 1) / 2;
   const long long NN = (long long)N * N;
-  const int k = blockIdx.z;
+  const int k = k0 + (int)blockIdx.z;
   double* M = k == 0 ? S + md.m0 : k == 1 ? H0 + md.m0 : k == 2 ? H0n + md.m0
             : k < 6 ? D + 3 * md.m0 + (k - 3) * NN : Q + 6 * md.m0 + (k - 6) * NN;
   const int c = threadIdx.x & 31, r8 = threadIdx.x >> 5;
