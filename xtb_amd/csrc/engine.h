@@ -249,7 +249,11 @@ class Batch {
   void select(const std::vector<int>& mols);
   void select_all();
   void setup();
-  bool scc_iteration();   // returns true while any molecule is active
+  // returns true while any molecule is active; queue_next: queue the next
+  // iteration's first part before waiting for the host's count
+  bool scc_iteration(bool queue_next = false);
+  void scc_prologue();
+  bool prologue_queued_ = false;
   void gradient();
   // restart guess for molecule m: shell charges, atomic dipoles [n][3] and
   // quadrupoles [n][6] of a previous wavefunction (used by the next setup)

@@ -2420,8 +2420,11 @@ void Batch::setup() {
   setup_done_ = true;
 }
 
-bool Batch::scc_iteration() {
-  const long long rounds_before = jacobi_rounds_;
+// The first part of an SCC iteration: potentials, the Fock matrix and its
+// transform to the current orbital basis.  Every kernel and product in it
+// skips molecules that are no longer active (MolState on the device), so it
+// may be queued before the host knows whether any molecule still is.
+void Batch::scc_prologue() {
   const Element* el = elements_dev;
   const double thr2 = neglect * 1.0e-2;
   // the AES potentials (read only charges and moments, write only vs, vd,
@@ -2452,6 +2455,13 @@ bool Batch::scc_iteration() {
   gemm(0, H, C, T, nullptr, nullptr, true);   // T = H C
   gemm(1, C, T, A, nullptr, nullptr, true, 1.0, nullptr, true);   // A = C^T H C (symmetric)
   mixed_here_ = false;
+}
+
+bool Batch::scc_iteration(bool queue_next) {
+  const long long rounds_before = jacobi_rounds_;
+  const Element* el = elements_dev;
+  if (prologue_queued_) prologue_queued_ = false;
+  else scc_prologue();
   // C holds the previous eigenvectors from the second iteration on: A is
   // nearly diagonal, refine W = I towards its eigenvectors with GEMMs first
   const bool warm = iterations_ > 0 && oa_max_steps > 0;
@@ -2519,7 +2529,16 @@ bool Batch::scc_iteration() {
   // any molecule still active?
   hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, stream, (const MolState*)d_state, nmol, jac_nactive);
   HIPCHECK(hipMemcpyAsync(h_pinned, jac_nactive, sizeof(int), hipMemcpyDeviceToHost, stream));
-  HIPCHECK(hipStreamSynchronize(stream));
+  if (queue_next) {
+    // the next iteration's first part keeps the GPU busy while the host
+    // waits for the count (it changes nothing if no molecule is active)
+    HIPCHECK(hipEventRecord(ev_join3, stream));
+    scc_prologue();
+    prologue_queued_ = true;
+    HIPCHECK(hipEventSynchronize(ev_join3));
+  } else {
+    HIPCHECK(hipStreamSynchronize(stream));
+  }
   return h_pinned[0] > 0;
 }
 
@@ -2606,9 +2625,12 @@ void Batch::run() {
   select_all();
   setup();
   int guard = 0;
-  while (scc_iteration()) {
+  prologue_queued_ = false;
+  // (no iteration is queued ahead of the guard's stop)
+  while (scc_iteration(guard + 1 <= opt.maxiter + 2)) {
     if (++guard > opt.maxiter + 2) break;
   }
+  prologue_queued_ = false;
   gradient();
   // the final shell charges and moments of every molecule are on the device
   wfn_valid_ = true;
