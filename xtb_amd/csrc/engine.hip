@@ -327,7 +327,7 @@ __global__ void k_sad_guess(const MolDesc* __restrict__ desc, const Element* __r
 // of Matrices", 2008, eq. 6.35): Y0 = S/c, Z0 = I,
 // T = (3I - Z Y)/2, Y <- Y T, Z <- T Z; Z -> (S/c)^(-1/2).
 // c = max row sum |S_ij| bounds the spectrum of S from above.
-// grid (row chunks of 64, molecules): thread = row; half the maximum over
+// grid (row chunks of 64, molecules); half the maximum over
 // the molecule's rows by an atomic max on the bit patterns of cbound (cleared
 // by k_ns_begin; non-negative doubles order like their bit patterns).  The
 // maximum absolute row sum bounds the largest eigenvalue of S (Gershgorin),
@@ -335,17 +335,27 @@ __global__ void k_sad_guess(const MolDesc* __restrict__ desc, const Element* __r
 // each mu to mu (3 - mu)^2 / 4 and converges for mu in (0, 3), and the small
 // ones, which set the number of iterations (each multiplies them by about
 // 9/4), start twice as large as with the full bound.
-__global__ void __launch_bounds__(64) k_ns_bound(const MolDesc* __restrict__ desc, const double* __restrict__ S,
-                                                 double* cbound) {
+__global__ void __launch_bounds__(256) k_ns_bound(const MolDesc* __restrict__ desc, const double* __restrict__ S,
+                                                  double* cbound) {
+  // 64 rows per workgroup, one wave per row at a time with its lanes along
+  // the row (coalesced reads); one atomic per workgroup (same-address
+  // atomics from every row had serialised)
+  __shared__ double sh[4];
   const MolDesc md = desc[blockIdx.y];
   const int N = md.nao;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double r = 0.0;
-  if (i < N)
-    for (int j = 0; j < N; ++j) r += fabs(S[md.m0 + (long long)i * N + j]);
-  for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
-  if (threadIdx.x == 0 && r > 0.0)
-    atomicMax((unsigned long long*)&cbound[md.mol], (unsigned long long)__double_as_longlong(0.5 * r));
+  const int i0 = blockIdx.x * 64;
+  if (i0 >= N) return;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double m = 0.0;
+  for (int i = i0 + w; i < min(N, i0 + 64); i += 4) {
+    const double* row = S + md.m0 + (long long)i * N;
+    double r = 0.0;
+    for (int j = lane; j < N; j += 64) r += fabs(row[j]);
+    m = fmax(m, wave_sum(r));
+  }
+  m = block_max<256>(m, sh);
+  if (threadIdx.x == 0 && m > 0.0)
+    atomicMax((unsigned long long*)&cbound[md.mol], (unsigned long long)__double_as_longlong(0.5 * m));
 }
 
 __global__ void k_ns_init(const MolDesc* __restrict__ desc, const double* __restrict__ S,
@@ -2217,7 +2227,7 @@ void Batch::orthogonalizer() {
   const dim3 gmm(gstride((long long)sub_max_nao * sub_max_nao), nsub);
   hipLaunchKernelGGL(k_ns_begin, dim3(grid1(nmol, 64)), dim3(64), 0, stream, nmol, (const int*)phase_act, ns_stage,
                      ns_act, ns_failed, ns_res, ns_c);
-  hipLaunchKernelGGL(k_ns_bound, dim3(grid1(sub_max_nao, 64), nsub), dim3(64), 0, stream, d_sub, (const double*)S,
+  hipLaunchKernelGGL(k_ns_bound, dim3(grid1(sub_max_nao, 64), nsub), dim3(256), 0, stream, d_sub, (const double*)S,
                      ns_c);
   hipLaunchKernelGGL(k_ns_init, gmm, dim3(256), 0, stream, d_sub, (const double*)S, (const double*)ns_c, A, W);
   const int maxit = 60;
